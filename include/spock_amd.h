@@ -1,0 +1,123 @@
+/*
+ * C ABI of the MI355X-native SPOCK solver (libspock_amd.so).
+ *
+ * The reference exposes its solver as C++ classes over CUDA (src/tree.cuh:102,
+ * src/problem.cuh:126, src/cache.cuh:1138-1245); there is no C FFI in the reference.
+ * The entry points below replace those classes one for one and are what the
+ * C++ drop-in header (spock_amd.hpp) and the Python bindings call:
+ *
+ *   spk_problem_load      <- ScenarioTree<T>(path) + ProblemData<T>(tree)
+ *                            (src/tree.cuh:102-179, src/problem.cuh:126-153)
+ *   spk_batch_create      <- CacheBuilder<T>::build() / make_unique()  (src/cache.cuh:1215-1244)
+ *   spk_batch_set_x0      <- Cache::initialiseState (src/cache.cuh:592-605)
+ *   spk_batch_set_iterate <- runSpock/runCp's previousSolution argument (src/cache.cuh:611-632)
+ *   spk_batch_solve       <- Cache::runSpock / Cache::runCp (src/cache.cuh:967-1132)
+ *   spk_batch_results     <- status(), solveIter(), solveTime() (src/cache.cuh:357-361)
+ *   spk_batch_get_iterate <- solution() (src/cache.cuh:378)
+ *   spk_batch_reset       <- Cache::reset (src/cache.cuh:436-479)
+ *   spk_batch_debug_op    <- the operator pieces the reference's gtest suite reaches through
+ *                            friend functions (LinearOperator::op/adj, Dynamics::project,
+ *                            projectPrimalWorkspaceOnKernels, iter(), dotM;
+ *                            tests/cuda/testOperator.cu, testProblem.cu, testCache.cu)
+ *
+ * Conventions: every function returns 0 on success and a negative code on error;
+ * spk_last_error(ctx) returns the message.  No exception crosses the ABI.  A context
+ * owns one HIP stream (default priority); solves are enqueued on it and
+ * spk_batch_results() waits for them.  One context per host thread.
+ * A batch holds `count` instances; each instance refers to a loaded problem (several
+ * instances may share one problem, e.g. many initial states of one tree).
+ */
+#ifndef SPOCK_AMD_H
+#define SPOCK_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spk_ctx spk_ctx;
+typedef struct spk_problem spk_problem;
+typedef struct spk_batch spk_batch;
+
+enum spk_dtype { SPK_F64 = 0, SPK_F32 = 1 };
+enum spk_method { SPK_SPOCK = 0, SPK_CP = 1 };
+/* solve status codes, identical to enum CacheStatus (src/cache.cuh:68-73) */
+enum spk_status { SPK_NOT_RUN = -1, SPK_CONVERGED = 0, SPK_OUT_OF_ITER = 1, SPK_OUT_OF_TIME = 2 };
+enum spk_debug_op { SPK_OP_L = 0, SPK_OP_LADJ = 1, SPK_OP_DYN = 2, SPK_OP_KERNEL = 3, SPK_OP_T = 4, SPK_OP_ELL = 5 };
+
+/* Builder options (src/cache.cuh:1154-1210).  tol_abs/tol_rel are multiplied by
+ * min(scaling) for preconditioned problems (src/cache.cuh:302-306).  max_iters = 0 means
+ * no iteration cap from the caller (a device-side cap of 2e6 outer iterations still applies).
+ * check_period is the error-check period (25 in the reference, src/cache.cuh:112). */
+typedef struct spk_options {
+    double tol_abs;
+    double tol_rel;
+    double max_time_secs;
+    long max_iters;
+    int max_inner;
+    int anderson_buffer;
+    int check_period;
+    int method;
+    int block_size;
+} spk_options;
+
+typedef struct spk_result {
+    int status;
+    long iters;
+    double err;
+    double tol;
+    double w;
+    long k0, k1, k2, k2bt, k3;
+    long num_T;   /* applications of the CP operator T */
+    long num_ell; /* applications of [L*; L] inside norms and error checks */
+} spk_result;
+
+void spk_default_options(spk_options *opts);
+
+int spk_ctx_create(int device, spk_ctx **out);
+int spk_ctx_destroy(spk_ctx *ctx);
+const char *spk_last_error(spk_ctx *ctx);
+/* the context's HIP stream, as a hipStream_t stored in *stream */
+int spk_ctx_stream(spk_ctx *ctx, void **stream);
+int spk_ctx_synchronize(spk_ctx *ctx);
+
+int spk_problem_load(spk_ctx *ctx, const char *folder, spk_problem **out);
+int spk_problem_destroy(spk_problem *p);
+/* out[0..15] = N, nl, S, E, nx, nu, size_prim, size_dual, size_iterate, preconditioned,
+ * con3, con5, #distinct dynamics factors (ABKTr), #distinct cost matrices (sqrtQ), 0, 0 */
+int spk_problem_sizes(const spk_problem *p, long *out);
+/* preconditioning scaling (nx + nu values; all ones when not preconditioned) */
+int spk_problem_scaling(const spk_problem *p, double *out);
+double spk_problem_step_size(const spk_problem *p);
+
+int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int dtype, spk_batch **out);
+int spk_batch_destroy(spk_batch *b);
+int spk_batch_count(const spk_batch *b);
+/* zero every iterate and workspace (Cache::reset) */
+int spk_batch_reset(spk_batch *b);
+/* initial states, unscaled, concatenated over instances (nx_k values each) */
+int spk_batch_set_x0(spk_batch *b, const double *x0);
+/* host iterate (size_iterate_k values) of instance k, used as the warm start of the next solve */
+int spk_batch_set_iterate(spk_batch *b, int k, const double *z);
+int spk_batch_get_iterate(spk_batch *b, int k, double *z);
+/* all iterates at once: host arrays concatenated over instances */
+int spk_batch_set_iterates(spk_batch *b, const double *z);
+int spk_batch_get_iterates(spk_batch *b, double *z);
+/* device pointer to instance k's accepted iterate (dtype of the batch) */
+void *spk_batch_iterate_ptr(spk_batch *b, int k);
+/* copy iterates between batches of the same problems (converting dtype) on the device */
+int spk_batch_copy_iterates(spk_batch *dst, const spk_batch *src);
+/* enqueue the solve of every instance; warm != 0 keeps the current iterates as previous solution */
+int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm);
+/* wait for the last solve; fills count results and the kernel time in seconds (nullable) */
+int spk_batch_results(spk_batch *b, spk_result *out, double *kernel_seconds);
+/* single operator on every instance: in/out are host arrays concatenated over instances
+ * (size_iterate_k values each).  L reads the primal part of `in` and writes the dual part of
+ * `out`; LADJ the reverse; DYN/KERNEL project the primal part; T writes T(in); ELL writes
+ * scalars[2k] = <in, M out_in> and scalars[2k+1] = max|out_in/gamma - ell(out_in)| where out_in
+ * is taken from `out` on entry. */
+int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, double *scalars);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

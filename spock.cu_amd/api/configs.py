@@ -1,0 +1,111 @@
+"""Seeded generators for the benchmark problem families and the unit-test fixture.
+
+Every family writes the reference's file set (``data.json`` + ``*.bt``) into a
+folder and returns the generated ``Problem``.  Random draws use
+``numpy.random.default_rng(seed)``; initial states come from an independent
+stream ``default_rng([seed, 1])`` so that the problem data do not depend on how
+many initial states are requested.
+
+Choices the family descriptions leave open, fixed here:
+* Markov initial distributions and transition rows are Dirichlet(1) draws.
+* The terminal weight of the 50/20 family is the discrete Riccati solution of
+  event 0's (A, B) with Q = I, R = 0.1 I.
+* Leaf boxes repeat the nonleaf state bounds.
+"""
+import os
+import numpy as np
+import scipy.linalg as sla
+
+from . import build, problem, tree as tree_mod
+
+
+def fixture_problem(folder, dt="d"):
+    """The reference's unit-test problem (``tests/cuda/test.py:15-99``).
+
+    Its nonleaf weights are built as ``q *= i`` on one shared array, so both
+    events end up with Q = 2I (``tests/cuda/test.py:57-61``); that is kept.
+    """
+    p = np.array([[0.3, 0.7], [0.3, 0.7]])
+    v = np.array([0.3, 0.7])
+    tr = tree_mod.MarkovChain(p, v, 3, 2, dt=dt).build()
+    nx, nu = 3, 2
+    A, B = np.eye(nx), 0.5 * np.ones((nx, nu))
+    dyn = [build.Dynamics(1.5 * A, 1.5 * B), build.Dynamics(A, B)]
+    Q2 = 2.0 * np.eye(nx)
+    costs = [build.CostQuadratic(Q2, np.eye(nu)), build.CostQuadratic(Q2, np.eye(nu))]
+    leaf = build.CostQuadratic(3 * np.eye(nx), leaf=True)
+    si_lb = np.vstack((-1.0 * np.ones((nx, 1)), -5.0 * np.ones((nu, 1))))
+    ncon = build.Rectangle(si_lb, -si_lb)
+    lcon = build.Rectangle(-0.1 * np.ones((nx, 1)), 0.1 * np.ones((nx, 1)))
+    return (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(ncon).with_constraint_leaf(lcon)
+            .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
+
+
+def _markov(rng, E, horizon, stop, dt):
+    P = rng.dirichlet(np.ones(E), size=E)
+    v = rng.dirichlet(np.ones(E))
+    return tree_mod.MarkovChain(P, v, horizon, stop, dt=dt).build()
+
+
+def _box_family(folder, rng, tr, nx, nu, a_std, b_std, x_lim, u_lim, alpha, leaf_dare, dt):
+    dyn = []
+    for _ in range(tr.num_events):
+        A = np.eye(nx) + a_std * rng.standard_normal((nx, nx))
+        B = b_std * rng.standard_normal((nx, nu))
+        dyn.append(build.Dynamics(A, B))
+    Q, R = np.eye(nx), 0.1 * np.eye(nu)
+    if leaf_dare:
+        Pf = sla.solve_discrete_are(dyn[0].A0, dyn[0].B0, Q, R)
+        Pf = 0.5 * (Pf + Pf.T)
+    else:
+        Pf = np.eye(nx)
+    costs = [build.CostQuadratic(Q, R) for _ in range(tr.num_events)]
+    leaf = build.CostQuadratic(Pf, leaf=True)
+    lim = np.vstack((x_lim * np.ones((nx, 1)), u_lim * np.ones((nu, 1))))
+    ncon = build.Rectangle(-lim, lim)
+    lcon = build.Rectangle(-x_lim * np.ones((nx, 1)), x_lim * np.ones((nx, 1)))
+    return (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(ncon).with_constraint_leaf(lcon)
+            .with_risk(build.AVaR(alpha)).with_preconditioning().generate_problem(folder))
+
+
+def small_markov(folder, seed=1, dt="d"):
+    """2 events, horizon 4, branching at every stage; nx=3, nu=2; |x|<=5, |u|<=2; AVaR 0.95."""
+    rng = np.random.default_rng(seed)
+    tr = _markov(rng, 2, 4, 4, dt)
+    return _box_family(folder, rng, tr, 3, 2, 0.1, 0.5, 5.0, 2.0, 0.95, False, dt)
+
+
+def mid_markov(folder, seed=2, dt="d"):
+    """3 events, horizon 10, branching stops at stage 3; nx=50, nu=20; |x|<=10, |u|<=1; AVaR 0.9."""
+    rng = np.random.default_rng(seed)
+    tr = _markov(rng, 3, 10, 3, dt)
+    return _box_family(folder, rng, tr, 50, 20, 0.02, 0.1, 10.0, 1.0, 0.9, True, dt)
+
+
+def uneven_instance(folder, k, dt="d"):
+    """Instance k of the uneven batch: horizon U{5..12}, events U{2..4}, stop U{1..3},
+    nx U{10..40}, nu U{4..16}, AVaR alpha U[0.8, 0.99]; data drawn like the 50/20 family."""
+    rng = np.random.default_rng(4000 + k)
+    horizon = int(rng.integers(5, 12, endpoint=True))
+    E = int(rng.integers(2, 4, endpoint=True))
+    stop = int(rng.integers(1, 3, endpoint=True))
+    nx = int(rng.integers(10, 40, endpoint=True))
+    nu = int(rng.integers(4, 16, endpoint=True))
+    alpha = float(rng.uniform(0.8, 0.99))
+    tr = _markov(rng, E, horizon, stop, dt)
+    return _box_family(folder, rng, tr, nx, nu, 0.02, 0.1, 10.0, 1.0, alpha, True, dt)
+
+
+def initial_states(seed, nx, count, kind="normal"):
+    """Initial states: ``normal`` ~ N(0, 0.5^2 I), ``uniform`` ~ U[-1, 1]."""
+    rng = np.random.default_rng([seed, 1])
+    if kind == "uniform":
+        return rng.uniform(-1.0, 1.0, size=(count, nx))
+    return 0.5 * rng.standard_normal((count, nx))
+
+
+def generate(name, folder, **kw):
+    os.makedirs(folder, exist_ok=True)
+    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov}[name](folder, **kw)

@@ -1,0 +1,645 @@
+// Host side of the C ABI (include/spock_amd.h): contexts, device-resident problems,
+// batches of solve instances and the launches of the fused engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/spock_amd.h"
+#include "engine.hpp"
+#include "loader.hpp"
+
+namespace spk {
+template <typename T>
+__global__ void k_solve(const Inst<T> *, Out *, int, double);
+template <typename T>
+__global__ void k_debug(const Inst<T> *, Out *, int, int);
+__global__ void k_f32_to_f64(const float *, double *, long);
+__global__ void k_f64_to_f32(const double *, float *, long);
+template <typename T>
+__global__ void k_fill(T *, T, long);
+}  // namespace spk
+
+using spk::HostProblem;
+using spk::Table;
+
+struct spk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double tickRateHz = 1e8;
+};
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        release();
+        if (b == 0) b = 16;
+        check(hipMalloc(&p, b), "hipMalloc");
+        bytes = b;
+    }
+};
+
+template <typename T>
+struct DevTableSet {  // device copies of one table: data as T, index as int
+    std::unique_ptr<DevBuf> data, idx;
+    int rows = 0, cols = 0;
+};
+
+}  // namespace
+
+struct spk_problem {
+    spk_ctx *ctx = nullptr;
+    HostProblem hp;
+    DevBuf anc, chFrom, numCh, stageFrom, stageTo;
+    std::map<std::string, DevTableSet<double>> t64;
+    std::map<std::string, DevTableSet<float>> t32;
+    bool up64 = false, up32 = false;
+};
+
+struct spk_batch {
+    spk_ctx *ctx = nullptr;
+    int dtype = SPK_F64;
+    int count = 0;
+    std::vector<spk_problem *> probs;
+    std::vector<size_t> coreOff, andOff;  // element offsets of each instance's regions
+    size_t coreElems = 0, andElems = 0;
+    int mAlloc = 0;
+    DevBuf core, anders, dInst, dOut;
+    std::vector<char> hInst;  // array of Inst<T>
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool pending = false;
+    size_t elSize() const { return dtype == SPK_F64 ? 8 : 4; }
+};
+
+namespace {
+
+template <typename T>
+std::map<std::string, DevTableSet<T>> &tablesOf(spk_problem *p);
+template <>
+std::map<std::string, DevTableSet<double>> &tablesOf<double>(spk_problem *p) { return p->t64; }
+template <>
+std::map<std::string, DevTableSet<float>> &tablesOf<float>(spk_problem *p) { return p->t32; }
+
+void uploadInts(DevBuf &b, const std::vector<int> &v) {
+    b.alloc(v.size() * sizeof(int));
+    if (!v.empty()) check(hipMemcpy(b.p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice), "upload ints");
+}
+
+template <typename T>
+void uploadTable(spk_problem *p, const std::string &name, const Table &t) {
+    DevTableSet<T> s;
+    s.rows = t.rows;
+    s.cols = t.cols;
+    std::vector<T> d(t.data.begin(), t.data.end());
+    s.data = std::make_unique<DevBuf>();
+    s.data->alloc(d.size() * sizeof(T));
+    if (!d.empty()) check(hipMemcpy(s.data->p, d.data(), d.size() * sizeof(T), hipMemcpyHostToDevice), "upload table");
+    s.idx = std::make_unique<DevBuf>();
+    s.idx->alloc(t.index.size() * sizeof(int));
+    if (!t.index.empty())
+        check(hipMemcpy(s.idx->p, t.index.data(), t.index.size() * sizeof(int), hipMemcpyHostToDevice), "upload index");
+    tablesOf<T>(p)[name] = std::move(s);
+}
+
+#define SPK_TABLES(X) \
+    X(BTr) X(AB) X(K) X(KTr) X(ABKTr) X(APB) X(cholL) X(cholU) X(NN) X(b) X(sQ) X(sR) X(q) X(r) X(trans) \
+    X(lsQ) X(lq) X(ltrans) X(nG) X(nLB) X(nUB) X(lG) X(lLB) X(lUB) X(nGT) X(lGT)
+
+template <typename T>
+void ensureUploaded(spk_problem *p) {
+    bool &done = std::is_same<T, double>::value ? p->up64 : p->up32;
+    if (done) return;
+    check(hipSetDevice(p->ctx->device), "hipSetDevice");
+#define UP(name) uploadTable<T>(p, #name, p->hp.name);
+    SPK_TABLES(UP)
+#undef UP
+    done = true;
+}
+
+template <typename T>
+spk::DTab<T> dtab(spk_problem *p, const std::string &name) {
+    auto &m = tablesOf<T>(p);
+    auto it = m.find(name);
+    spk::DTab<T> t;
+    if (it == m.end()) return t;
+    t.data = static_cast<const T *>(it->second.data->p);
+    t.idx = static_cast<const int *>(it->second.idx->p);
+    t.rows = it->second.rows;
+    t.cols = it->second.cols;
+    return t;
+}
+
+size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Per-instance element counts of the two state regions.
+size_t coreCount(const HostProblem &h) {
+    return alignUp(8 * (size_t) h.sIter + (size_t) h.nx * h.N + (size_t) h.nu * h.nl + (size_t) h.nx, 64);
+}
+size_t andCount(const HostProblem &h, int m) { return alignUp((size_t) (3 * m + 1) * h.sIter, 64); }
+
+template <typename T>
+void buildInsts(spk_batch *b, const spk_options &o, int warm) {
+    b->hInst.assign(sizeof(spk::Inst<T>) * b->count, 0);
+    auto *arr = reinterpret_cast<spk::Inst<T> *>(b->hInst.data());
+    T *core = static_cast<T *>(b->core.p);
+    T *and_ = static_cast<T *>(b->anders.p);
+    for (int k = 0; k < b->count; k++) {
+        spk_problem *p = b->probs[k];
+        const HostProblem &h = p->hp;
+        spk::Inst<T> &I = arr[k];
+        I.N = h.N; I.nl = h.nl; I.nlf = h.nlf; I.S = h.S; I.E = h.E; I.nx = h.nx; I.nu = h.nu;
+        I.yD = h.yD; I.nD = h.nD; I.ncost = h.ncost; I.lcost = h.lcost; I.ncon = h.ncon; I.lcon = h.lcon;
+        I.con3 = h.con3; I.con5 = h.con5;
+        I.oU = 0; I.oX = h.sU; I.oY = I.oX + h.sX; I.oT = I.oY + h.sY; I.oS = I.oT + h.sT; I.sPrim = h.sPrim;
+        I.oI = 0; I.oII = h.sI; I.oIII = I.oII + h.sII; I.oIV = I.oIII + h.sIII; I.oV = I.oIV + h.sIV;
+        I.oVI = I.oV + h.sV; I.sDual = h.sDual; I.sIter = h.sIter;
+        I.gamma = (T) h.stepSize;
+        I.gammaRecip = (T) (1. / (double) I.gamma);
+        I.anc = static_cast<const int *>(p->anc.p);
+        I.chFrom = static_cast<const int *>(p->chFrom.p);
+        I.numCh = static_cast<const int *>(p->numCh.p);
+        I.stageFrom = static_cast<const int *>(p->stageFrom.p);
+        I.stageTo = static_cast<const int *>(p->stageTo.p);
+#define DT(name) I.name = dtab<T>(p, #name);
+        SPK_TABLES(DT)
+#undef DT
+        const size_t n = h.sIter;
+        T *c = core + b->coreOff[k];
+        I.z = c; I.zPrev = c + n; I.cand = c + 2 * n; I.backup = c + 3 * n; I.res = c + 4 * n;
+        I.resPrev = c + 5 * n; I.dir = c + 6 * n; I.dirS = c + 7 * n;
+        I.qd = c + 8 * n;
+        I.dd = I.qd + (size_t) h.nx * h.N;
+        I.x0s = I.dd + (size_t) h.nu * h.nl;
+        T *a = and_ + b->andOff[k];
+        const int m = o.anderson_buffer;
+        I.hS = a; I.hY = a + (size_t) m * n; I.W = a + (size_t) 2 * m * n;
+        double tolScale = 1.;
+        if (h.precond) tolScale = *std::min_element(h.scaling.begin(), h.scaling.end());
+        I.tolAbs = (T) (o.tol_abs * tolScale);
+        I.tolRel = (T) (o.tol_rel * tolScale);
+        I.maxIters = o.max_iters;
+        I.maxInner = o.max_inner;
+        I.andBuff = m;
+        I.checkPeriod = o.check_period > 0 ? o.check_period : 25;
+        I.method = o.method;
+        I.warm = warm;
+        I.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
+    }
+    b->dInst.alloc(b->hInst.size());
+    check(hipMemcpyAsync(b->dInst.p, b->hInst.data(), b->hInst.size(), hipMemcpyHostToDevice, b->ctx->stream),
+          "upload instances");
+}
+
+void ensureAnderson(spk_batch *b, int m) {
+    if (m <= b->mAlloc) return;
+    b->andOff.resize(b->count);
+    size_t off = 0;
+    for (int k = 0; k < b->count; k++) {
+        b->andOff[k] = off;
+        off += andCount(b->probs[k]->hp, m);
+    }
+    b->andElems = off;
+    b->anders.alloc(off * b->elSize());
+    check(hipMemsetAsync(b->anders.p, 0, off * b->elSize(), b->ctx->stream), "zero anderson");
+    b->mAlloc = m;
+}
+
+template <typename T>
+void toDevice(spk_batch *b, T *dst, const double *src, size_t n) {
+    std::vector<T> tmp(src, src + n);
+    check(hipMemcpy(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice), "copy to device");
+}
+
+template <typename T>
+void toHost(spk_batch *b, double *dst, const T *src, size_t n) {
+    std::vector<T> tmp(n);
+    check(hipMemcpy(tmp.data(), src, n * sizeof(T), hipMemcpyDeviceToHost), "copy to host");
+    std::copy(tmp.begin(), tmp.end(), dst);
+}
+
+template <typename T>
+T *corePtr(spk_batch *b, int k) {
+    return static_cast<T *>(b->core.p) + b->coreOff[k];
+}
+
+int fail(spk_ctx *ctx, const std::exception &e) {
+    if (ctx) ctx->err = e.what();
+    return -1;
+}
+
+void validateProblem(const HostProblem &h) {
+    if (h.nu > 64) throw std::runtime_error("numInputs > 64 is not supported by the fused engine");
+    if (h.nx > 64 * spk::MAXR || h.nx + h.nu > 64 * spk::MAXV)
+        throw std::runtime_error("numStates too large for the fused engine (nx <= 256, nx + nu <= 256)");
+    if (h.nD > 64) throw std::runtime_error("numEvents > 15 is not supported by the fused engine");
+    if (h.con3 > 64 * spk::MAXR || h.con5 > 64 * spk::MAXR)
+        throw std::runtime_error("constraint rows > 256 are not supported by the fused engine");
+}
+
+template <typename T>
+void launchSolve(spk_batch *b, const spk_options &o, int warm) {
+    buildInsts<T>(b, o, warm);
+    int bs = o.block_size > 0 ? o.block_size : 256;
+    bs = std::max(64, std::min(1024, bs / 64 * 64));
+    b->dOut.alloc(sizeof(spk::Out) * b->count);
+    check(hipEventRecord(b->ev0, b->ctx->stream), "event");
+    hipLaunchKernelGGL(spk::k_solve<T>, dim3(b->count), dim3(bs), 0, b->ctx->stream,
+                       static_cast<const spk::Inst<T> *>(b->dInst.p), static_cast<spk::Out *>(b->dOut.p), b->count,
+                       b->ctx->tickRateHz);
+    check(hipGetLastError(), "launch k_solve");
+    check(hipEventRecord(b->ev1, b->ctx->stream), "event");
+    b->pending = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void spk_default_options(spk_options *o) {
+    o->tol_abs = 1e-3;
+    o->tol_rel = 0.;
+    o->max_time_secs = 0.;
+    o->max_iters = 0;
+    o->max_inner = 8;
+    o->anderson_buffer = 3;
+    o->check_period = 25;
+    o->method = SPK_SPOCK;
+    o->block_size = 256;
+}
+
+int spk_ctx_create(int device, spk_ctx **out) {
+    try {
+        auto *c = new spk_ctx();
+        c->device = device;
+        check(hipSetDevice(device), "hipSetDevice");
+        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            c->tickRateHz = khz * 1000.0;
+        *out = c;
+        return 0;
+    } catch (const std::exception &e) {
+        *out = nullptr;
+        return -1;
+    }
+}
+
+int spk_ctx_destroy(spk_ctx *ctx) {
+    if (!ctx) return 0;
+    if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+const char *spk_last_error(spk_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int spk_ctx_stream(spk_ctx *ctx, void **stream) {
+    *stream = ctx->stream;
+    return 0;
+}
+
+int spk_ctx_synchronize(spk_ctx *ctx) {
+    try {
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(ctx, e);
+    }
+}
+
+int spk_problem_load(spk_ctx *ctx, const char *folder, spk_problem **out) {
+    *out = nullptr;
+    try {
+        auto p = std::make_unique<spk_problem>();
+        p->ctx = ctx;
+        p->hp = spk::loadProblem(folder);
+        validateProblem(p->hp);
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        uploadInts(p->anc, p->hp.anc);
+        uploadInts(p->chFrom, p->hp.chFrom);
+        uploadInts(p->numCh, p->hp.numCh);
+        uploadInts(p->stageFrom, p->hp.stageFrom);
+        uploadInts(p->stageTo, p->hp.stageTo);
+        *out = p.release();
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(ctx, e);
+    }
+}
+
+int spk_problem_destroy(spk_problem *p) {
+    delete p;
+    return 0;
+}
+
+int spk_problem_sizes(const spk_problem *p, long *out) {
+    const HostProblem &h = p->hp;
+    long v[16] = {h.N, h.nl, h.S, h.E, h.nx, h.nu, h.sPrim, h.sDual, h.sIter, h.precond ? 1 : 0, h.con3, h.con5,
+                  h.ABKTr.count(), h.sQ.count(), 0, 0};
+    std::memcpy(out, v, sizeof(v));
+    return 0;
+}
+
+int spk_problem_scaling(const spk_problem *p, double *out) {
+    const HostProblem &h = p->hp;
+    for (int k = 0; k < h.nx + h.nu; k++) out[k] = h.precond ? h.scaling[k] : 1.;
+    return 0;
+}
+
+double spk_problem_step_size(const spk_problem *p) { return p->hp.stepSize; }
+
+int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int dtype, spk_batch **out) {
+    *out = nullptr;
+    try {
+        if (count <= 0) throw std::runtime_error("batch count must be positive");
+        if (dtype != SPK_F64 && dtype != SPK_F32) throw std::runtime_error("dtype must be SPK_F64 or SPK_F32");
+        auto b = std::make_unique<spk_batch>();
+        b->ctx = ctx;
+        b->dtype = dtype;
+        b->count = count;
+        b->probs.assign(problems, problems + count);
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        for (auto *p : b->probs) {
+            if (!p) throw std::runtime_error("null problem in batch");
+            if (dtype == SPK_F64) ensureUploaded<double>(p);
+            else ensureUploaded<float>(p);
+        }
+        b->coreOff.resize(count);
+        size_t off = 0;
+        for (int k = 0; k < count; k++) {
+            b->coreOff[k] = off;
+            off += coreCount(b->probs[k]->hp);
+        }
+        b->coreElems = off;
+        b->core.alloc(off * b->elSize());
+        check(hipMemset(b->core.p, 0, off * b->elSize()), "zero state");
+        check(hipEventCreate(&b->ev0), "event");
+        check(hipEventCreate(&b->ev1), "event");
+        ensureAnderson(b.get(), 3);
+        check(hipStreamSynchronize(ctx->stream), "sync");
+        *out = b.release();
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(ctx, e);
+    }
+}
+
+int spk_batch_destroy(spk_batch *b) {
+    if (!b) return 0;
+    if (b->ctx) (void) hipStreamSynchronize(b->ctx->stream);
+    if (b->ev0) (void) hipEventDestroy(b->ev0);
+    if (b->ev1) (void) hipEventDestroy(b->ev1);
+    delete b;
+    return 0;
+}
+
+int spk_batch_count(const spk_batch *b) { return b->count; }
+
+int spk_batch_reset(spk_batch *b) {
+    try {
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        check(hipMemset(b->core.p, 0, b->coreElems * b->elSize()), "reset");
+        if (b->anders.p) check(hipMemset(b->anders.p, 0, b->andElems * b->elSize()), "reset");
+        // x0 is part of the core region; the reference's reset also zeroes the initial state
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_set_x0(spk_batch *b, const double *x0) {
+    try {
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        size_t pos = 0;
+        for (int k = 0; k < b->count; k++) {
+            const HostProblem &h = b->probs[k]->hp;
+            std::vector<double> s(x0 + pos, x0 + pos + h.nx);
+            pos += h.nx;
+            if (h.precond)
+                for (int i = 0; i < h.nx; i++) s[i] = s[i] * h.scaling[i];  // src/cache.cuh:599-601
+            const size_t o = 8 * (size_t) h.sIter + (size_t) h.nx * h.N + (size_t) h.nu * h.nl;
+            if (b->dtype == SPK_F64) toDevice<double>(b, corePtr<double>(b, k) + o, s.data(), h.nx);
+            else toDevice<float>(b, corePtr<float>(b, k) + o, s.data(), h.nx);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_set_iterate(spk_batch *b, int k, const double *z) {
+    try {
+        if (k < 0 || k >= b->count) throw std::runtime_error("instance index out of range");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        const size_t n = b->probs[k]->hp.sIter;
+        if (b->dtype == SPK_F64) toDevice<double>(b, corePtr<double>(b, k), z, n);
+        else toDevice<float>(b, corePtr<float>(b, k), z, n);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_get_iterate(spk_batch *b, int k, double *z) {
+    try {
+        if (k < 0 || k >= b->count) throw std::runtime_error("instance index out of range");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        const size_t n = b->probs[k]->hp.sIter;
+        if (b->dtype == SPK_F64) toHost<double>(b, z, corePtr<double>(b, k), n);
+        else toHost<float>(b, z, corePtr<float>(b, k), n);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_set_iterates(spk_batch *b, const double *z) {
+    size_t pos = 0;
+    for (int k = 0; k < b->count; k++) {
+        if (spk_batch_set_iterate(b, k, z + pos)) return -1;
+        pos += b->probs[k]->hp.sIter;
+    }
+    return 0;
+}
+
+int spk_batch_get_iterates(spk_batch *b, double *z) {
+    size_t pos = 0;
+    for (int k = 0; k < b->count; k++) {
+        if (spk_batch_get_iterate(b, k, z + pos)) return -1;
+        pos += b->probs[k]->hp.sIter;
+    }
+    return 0;
+}
+
+void *spk_batch_iterate_ptr(spk_batch *b, int k) {
+    if (k < 0 || k >= b->count) return nullptr;
+    return static_cast<char *>(b->core.p) + b->coreOff[k] * b->elSize();
+}
+
+int spk_batch_copy_iterates(spk_batch *dst, const spk_batch *src) {
+    try {
+        if (dst->count != src->count) throw std::runtime_error("batch sizes differ");
+        check(hipStreamSynchronize(src->ctx->stream), "sync");
+        for (int k = 0; k < dst->count; k++) {
+            const size_t n = dst->probs[k]->hp.sIter;
+            if (src->probs[k]->hp.sIter != (long) n) throw std::runtime_error("instance sizes differ");
+            void *d = static_cast<char *>(dst->core.p) + dst->coreOff[k] * dst->elSize();
+            const void *s = static_cast<const char *>(src->core.p) + src->coreOff[k] * src->elSize();
+            if (src->dtype == dst->dtype) {
+                check(hipMemcpyAsync(d, s, n * dst->elSize(), hipMemcpyDeviceToDevice, dst->ctx->stream), "copy");
+            } else if (src->dtype == SPK_F32) {
+                hipLaunchKernelGGL(spk::k_f32_to_f64, dim3(256), dim3(256), 0, dst->ctx->stream,
+                                   static_cast<const float *>(s), static_cast<double *>(d), (long) n);
+            } else {
+                hipLaunchKernelGGL(spk::k_f64_to_f32, dim3(256), dim3(256), 0, dst->ctx->stream,
+                                   static_cast<const double *>(s), static_cast<float *>(d), (long) n);
+            }
+            // fresh residual history for the continued solve
+            void *res = static_cast<char *>(d) + 4 * n * dst->elSize();
+            if (dst->dtype == SPK_F64)
+                hipLaunchKernelGGL(spk::k_fill<double>, dim3(256), dim3(256), 0, dst->ctx->stream,
+                                   static_cast<double *>(res), 0.0, (long) n);
+            else
+                hipLaunchKernelGGL(spk::k_fill<float>, dim3(256), dim3(256), 0, dst->ctx->stream,
+                                   static_cast<float *>(res), 0.0f, (long) n);
+            check(hipGetLastError(), "copy kernels");
+        }
+        check(hipStreamSynchronize(dst->ctx->stream), "sync");
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(dst->ctx, e);
+    }
+}
+
+int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm) {
+    try {
+        spk_options o;
+        if (opts) o = *opts;
+        else spk_default_options(&o);
+        if (o.anderson_buffer < 1 || o.anderson_buffer > 8) throw std::runtime_error("anderson_buffer must be 1..8");
+        if (o.max_inner < 0) throw std::runtime_error("max_inner must be >= 0");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        ensureAnderson(b, o.anderson_buffer);
+        if (b->dtype == SPK_F64) launchSolve<double>(b, o, warm);
+        else launchSolve<float>(b, o, warm);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_results(spk_batch *b, spk_result *out, double *kernelSeconds) {
+    try {
+        check(hipStreamSynchronize(b->ctx->stream), "sync after solve");
+        if (!b->pending) throw std::runtime_error("no solve has been run on this batch");
+        if (kernelSeconds) {
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, b->ev0, b->ev1), "event time");
+            *kernelSeconds = ms * 1e-3;
+        }
+        if (out) {
+            std::vector<spk::Out> tmp(b->count);
+            check(hipMemcpy(tmp.data(), b->dOut.p, sizeof(spk::Out) * b->count, hipMemcpyDeviceToHost), "results");
+            for (int k = 0; k < b->count; k++) {
+                out[k].status = tmp[k].status;
+                out[k].iters = tmp[k].iters;
+                out[k].err = tmp[k].err;
+                out[k].tol = tmp[k].tol;
+                out[k].w = tmp[k].w;
+                out[k].k0 = tmp[k].k0;
+                out[k].k1 = tmp[k].k1;
+                out[k].k2 = tmp[k].k2;
+                out[k].k2bt = tmp[k].k2bt;
+                out[k].k3 = tmp[k].k3;
+                out[k].num_T = tmp[k].nT;
+                out[k].num_ell = tmp[k].nEll;
+            }
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, double *scalars) {
+    try {
+        if (op < SPK_OP_L || op > SPK_OP_ELL) throw std::runtime_error("unknown debug op");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        size_t pos = 0;
+        for (int k = 0; k < b->count; k++) {
+            const size_t n = b->probs[k]->hp.sIter;
+            if (b->dtype == SPK_F64) {
+                toDevice<double>(b, corePtr<double>(b, k), in + pos, n);
+                toDevice<double>(b, corePtr<double>(b, k) + 2 * n, out + pos, n);
+            } else {
+                toDevice<float>(b, corePtr<float>(b, k), in + pos, n);
+                toDevice<float>(b, corePtr<float>(b, k) + 2 * n, out + pos, n);
+            }
+            pos += n;
+        }
+        spk_options o;
+        spk_default_options(&o);
+        ensureAnderson(b, o.anderson_buffer);
+        b->dOut.alloc(sizeof(spk::Out) * b->count);
+        check(hipMemset(b->dOut.p, 0, sizeof(spk::Out) * b->count), "zero out");
+        if (b->dtype == SPK_F64) {
+            buildInsts<double>(b, o, 0);
+            hipLaunchKernelGGL(spk::k_debug<double>, dim3(b->count), dim3(256), 0, b->ctx->stream,
+                               static_cast<const spk::Inst<double> *>(b->dInst.p), static_cast<spk::Out *>(b->dOut.p),
+                               b->count, op);
+        } else {
+            buildInsts<float>(b, o, 0);
+            hipLaunchKernelGGL(spk::k_debug<float>, dim3(b->count), dim3(256), 0, b->ctx->stream,
+                               static_cast<const spk::Inst<float> *>(b->dInst.p), static_cast<spk::Out *>(b->dOut.p),
+                               b->count, op);
+        }
+        check(hipGetLastError(), "launch k_debug");
+        check(hipStreamSynchronize(b->ctx->stream), "sync debug op");
+        pos = 0;
+        std::vector<spk::Out> res(b->count);
+        check(hipMemcpy(res.data(), b->dOut.p, sizeof(spk::Out) * b->count, hipMemcpyDeviceToHost), "results");
+        for (int k = 0; k < b->count; k++) {
+            const size_t n = b->probs[k]->hp.sIter;
+            if (b->dtype == SPK_F64) toHost<double>(b, out + pos, corePtr<double>(b, k) + 2 * n, n);
+            else toHost<float>(b, out + pos, corePtr<float>(b, k) + 2 * n, n);
+            if (scalars) {
+                scalars[2 * k] = res[k].w;
+                scalars[2 * k + 1] = res[k].err;
+            }
+            pos += n;
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// Device-side data model of the fused SPOCK engine (shared by the kernels and the host glue).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace spk {
+
+// Per-node factor table on the device: entry for node n starts at data + idx[n] * rows * cols.
+template <typename T>
+struct DTab {
+    const T *data = nullptr;
+    const int *idx = nullptr;
+    int rows = 0, cols = 0;
+    __device__ __forceinline__ const T *at(int n) const { return data + (size_t) idx[n] * rows * cols; }
+};
+
+// One solve instance: problem sizes, device tables, state buffers and options.
+template <typename T>
+struct Inst {
+    int N, nl, nlf, S, E, nx, nu, yD, nD, ncost, lcost, ncon, lcon, con3, con5;
+    long oU, oX, oY, oT, oS, sPrim, oI, oII, oIII, oIV, oV, oVI, sDual, sIter;
+    T gamma, gammaRecip;
+    const int *anc, *chFrom, *numCh, *stageFrom, *stageTo;
+    DTab<T> BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, NN, b, sQ, sR, q, r, trans, lsQ, lq, ltrans;
+    DTab<T> nG, nLB, nUB, lG, lLB, lUB, nGT, lGT;
+    // state (iterate-sized unless noted)
+    T *z, *zPrev, *cand, *backup, *res, *resPrev, *dir, *dirS;
+    T *hS, *hY;    // Anderson histories (iterate and residual differences), andBuff columns each
+    T *W;          // Householder scratch, andBuff + 1 columns
+    T *qd, *dd;    // dynamics sweep intermediates: nx*N and nu*nl
+    const T *x0s;  // scaled initial state (nx)
+    // options
+    T tolAbs, tolRel;  // already multiplied by min(scaling) when preconditioned
+    long maxIters;
+    int maxInner, andBuff, checkPeriod, method, warm;
+    double maxTicks;  // wall-clock cap in device ticks (0 = none)
+};
+
+struct Out {
+    int status;
+    int pad;
+    long iters;
+    double err, tol, w;
+    long k0, k1, k2, k2bt, k3, nT, nEll;
+};
+
+enum Method { METHOD_SPOCK = 0, METHOD_CP = 1 };
+enum DevKinds { CON_NO_DEV = 0, CON_RECT_DEV = 1, CON_POLY_DEV = 2, COST_QUAD_DEV = 0, COST_QUADLIN_DEV = 1 };
+enum Status { ST_NOTRUN = -1, ST_CONVERGED = 0, ST_OUTOFITER = 1, ST_OUTOFTIME = 2 };
+
+constexpr int MAXR = 4;  // row chunks of 64 per wave task (rows <= 256)
+constexpr int MAXV = 4;  // vector chunks of 64 held in registers (length <= 256)
+
+}  // namespace spk

@@ -1,0 +1,221 @@
+"""ctypes binding of libspock_amd.so (the HIP solver's C ABI, ``include/spock_amd.h``).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C spock.cu_amd/csrc``).
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libspock_amd.so")
+
+F64, F32 = 0, 1
+SPOCK, CP = 0, 1
+OP_L, OP_LADJ, OP_DYN, OP_KERNEL, OP_T, OP_ELL = range(6)
+STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
+
+EXPORTS = [
+    "spk_default_options", "spk_ctx_create", "spk_ctx_destroy", "spk_last_error", "spk_ctx_stream",
+    "spk_ctx_synchronize", "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
+    "spk_problem_step_size", "spk_batch_create", "spk_batch_destroy", "spk_batch_count", "spk_batch_reset",
+    "spk_batch_set_x0", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
+    "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve",
+    "spk_batch_results", "spk_batch_debug_op",
+]
+
+
+class Options(C.Structure):
+    _fields_ = [("tol_abs", C.c_double), ("tol_rel", C.c_double), ("max_time_secs", C.c_double),
+                ("max_iters", C.c_long), ("max_inner", C.c_int), ("anderson_buffer", C.c_int),
+                ("check_period", C.c_int), ("method", C.c_int), ("block_size", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_long), ("err", C.c_double), ("tol", C.c_double),
+                ("w", C.c_double), ("k0", C.c_long), ("k1", C.c_long), ("k2", C.c_long), ("k2bt", C.c_long),
+                ("k3", C.c_long), ("num_T", C.c_long), ("num_ell", C.c_long)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_dp = C.POINTER(C.c_double)
+_lp = C.POINTER(C.c_long)
+_LIB = None
+
+
+class SpockError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise SpockError(f"{path} is missing: build it with __graft_entry__.build() (make -C spock.cu_amd/csrc)")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.spk_default_options.argtypes = [C.POINTER(Options)]
+    lib.spk_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.spk_ctx_destroy.argtypes = [vp]
+    lib.spk_last_error.argtypes = [vp]
+    lib.spk_last_error.restype = C.c_char_p
+    lib.spk_ctx_stream.argtypes = [vp, C.POINTER(vp)]
+    lib.spk_ctx_synchronize.argtypes = [vp]
+    lib.spk_problem_load.argtypes = [vp, C.c_char_p, C.POINTER(vp)]
+    lib.spk_problem_destroy.argtypes = [vp]
+    lib.spk_problem_sizes.argtypes = [vp, _lp]
+    lib.spk_problem_scaling.argtypes = [vp, _dp]
+    lib.spk_problem_step_size.argtypes = [vp]
+    lib.spk_problem_step_size.restype = C.c_double
+    lib.spk_batch_create.argtypes = [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    lib.spk_batch_destroy.argtypes = [vp]
+    lib.spk_batch_count.argtypes = [vp]
+    lib.spk_batch_reset.argtypes = [vp]
+    lib.spk_batch_set_x0.argtypes = [vp, _dp]
+    lib.spk_batch_set_iterate.argtypes = [vp, C.c_int, _dp]
+    lib.spk_batch_get_iterate.argtypes = [vp, C.c_int, _dp]
+    lib.spk_batch_set_iterates.argtypes = [vp, _dp]
+    lib.spk_batch_get_iterates.argtypes = [vp, _dp]
+    lib.spk_batch_iterate_ptr.argtypes = [vp, C.c_int]
+    lib.spk_batch_iterate_ptr.restype = vp
+    lib.spk_batch_copy_iterates.argtypes = [vp, vp]
+    lib.spk_batch_solve.argtypes = [vp, C.POINTER(Options), C.c_int]
+    lib.spk_batch_results.argtypes = [vp, C.POINTER(Result), _dp]
+    lib.spk_batch_debug_op.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    _LIB = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def options(**kw):
+    lib = load_library()
+    o = Options()
+    lib.spk_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """A device context with its own HIP stream (``spk_ctx``)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        if self.lib.spk_ctx_create(device, C.byref(h)) != 0:
+            raise SpockError("spk_ctx_create failed (no HIP device?)")
+        self.h = h
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise SpockError(f"{what}: {self.lib.spk_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.spk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Problem:
+    """A problem folder loaded to the device (``spk_problem``)."""
+
+    def __init__(self, ctx, folder):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(ctx.lib.spk_problem_load(ctx.h, folder.encode(), C.byref(h)), f"load {folder}")
+        self.h = h
+        s = np.zeros(16, dtype=np.int64)
+        ctx.lib.spk_problem_sizes(h, s.ctypes.data_as(_lp))
+        (self.N, self.nl, self.S, self.E, self.nx, self.nu, self.n_prim, self.n_dual, self.n_iter,
+         self.preconditioned, self.con3, self.con5, self.distinct_factors, self.distinct_costs) = (int(v) for v in s[:14])
+        self.scaling = np.zeros(self.nx + self.nu)
+        ctx.lib.spk_problem_scaling(h, _d(self.scaling))
+        self.step_size = ctx.lib.spk_problem_step_size(h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.spk_problem_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Batch:
+    """A set of solve instances (``spk_batch``); one instance = one Cache of the reference."""
+
+    def __init__(self, ctx, problems, dtype=F64):
+        self.ctx = ctx
+        self.problems = list(problems)
+        self.dtype = dtype
+        arr = (C.c_void_p * len(self.problems))(*[p.h for p in self.problems])
+        h = C.c_void_p()
+        ctx.check(ctx.lib.spk_batch_create(ctx.h, len(self.problems), arr, dtype, C.byref(h)), "batch create")
+        self.h = h
+        self.sizes = [p.n_iter for p in self.problems]
+        self.total = int(sum(self.sizes))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.spk_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def reset(self):
+        self.ctx.check(self.ctx.lib.spk_batch_reset(self.h), "reset")
+
+    def set_x0(self, x0):
+        x0 = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in x0])
+                                  if isinstance(x0, (list, tuple)) else np.asarray(x0, dtype=np.float64).ravel())
+        self.ctx.check(self.ctx.lib.spk_batch_set_x0(self.h, _d(x0)), "set_x0")
+
+    def set_iterates(self, z):
+        z = np.ascontiguousarray(np.asarray(z, dtype=np.float64).ravel())
+        assert z.size == self.total
+        self.ctx.check(self.ctx.lib.spk_batch_set_iterates(self.h, _d(z)), "set_iterates")
+
+    def iterates(self):
+        z = np.zeros(self.total)
+        self.ctx.check(self.ctx.lib.spk_batch_get_iterates(self.h, _d(z)), "get_iterates")
+        return z
+
+    def iterate(self, k):
+        z = np.zeros(self.sizes[k])
+        self.ctx.check(self.ctx.lib.spk_batch_get_iterate(self.h, k, _d(z)), "get_iterate")
+        return z
+
+    def copy_iterates_from(self, other):
+        self.ctx.check(self.ctx.lib.spk_batch_copy_iterates(self.h, other.h), "copy_iterates")
+
+    def solve(self, opts=None, warm=False):
+        o = opts if opts is not None else options()
+        self.ctx.check(self.ctx.lib.spk_batch_solve(self.h, C.byref(o), 1 if warm else 0), "solve")
+
+    def results(self):
+        res = (Result * len(self.problems))()
+        sec = C.c_double()
+        self.ctx.check(self.ctx.lib.spk_batch_results(self.h, res, C.byref(sec)), "results")
+        return [r.as_dict() for r in res], sec.value
+
+    def debug_op(self, op, inp, out=None):
+        inp = np.ascontiguousarray(np.asarray(inp, dtype=np.float64).ravel())
+        o = np.zeros(self.total) if out is None else np.ascontiguousarray(np.asarray(out, dtype=np.float64).ravel())
+        sc = np.zeros(2 * len(self.problems))
+        self.ctx.check(self.ctx.lib.spk_batch_debug_op(self.h, op, _d(inp), _d(o), _d(sc)), "debug_op")
+        return o, sc

@@ -1,0 +1,164 @@
+"""HIP solver vs the FP64 CPU oracle, through the C ABI (libspock_amd.so).
+
+Operators (L, L*, dynamics and kernel projections, T, the M-inner product): agreement to
+FP64 rounding (1e-12 relative).  Whole iterations: the iterates agree to 1e-9 relative
+with identical SuperMann branch counts.  SuperMann with Anderson directions amplifies
+last-bit differences by about 10x per 25 outer iterations (the oracle perturbed by one
+ulp in x0 drifts 5e-12 after 100 iterations and 6e-6 after 250), so SPOCK iterate parity
+is checked over the first 75 outer iterations and plain CP over 250; converged solves
+are checked on status, cost s0 and the oracle's own residual of the GPU solution.
+Batched solves must reproduce single-instance GPU solves bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from oracle.oracle import Oracle
+from spock_amd import runtime as rt
+from spock_amd.api import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return rt.Context(0)
+
+
+def x0_for(name, nx, k=0):
+    if name == "fixture":
+        return np.full(nx, 1.1)
+    if name == "small":
+        return configs.initial_states(1, nx, k + 1, "uniform")[k]
+    return configs.initial_states(2, nx, k + 1)[k]
+
+
+def inert_zero(o, z):
+    """t[0] and node 0's IV block are never written by the reference from a zero start."""
+    z = z.copy()
+    z[o.sU + o.sX + o.sY] = 0.0
+    iv0 = o.n_prim + o.sI + o.sII + o.sIII
+    z[iv0:iv0 + o.nx + o.info["numInputs"] + 2] = 0.0
+    return z
+
+
+@pytest.mark.parametrize("name", ["fixture", "small", "mid"])
+def test_operators_match_oracle(ctx, problem_dirs, name):
+    d = problem_dirs[name]
+    o = Oracle(d)
+    p = rt.Problem(ctx, d)
+    b = rt.Batch(ctx, [p], rt.F64)
+    rng = np.random.default_rng(7)
+    z = inert_zero(o, 3.0 * rng.standard_normal(o.n_iter))
+    x0 = x0_for(name, o.nx)
+    b.set_x0([x0])
+    x0s = o.scaled_x0(x0)
+    out, _ = b.debug_op(rt.OP_L, z)
+    assert rel_err(out[o.n_prim:], o.L(z[:o.n_prim])) < 1e-13
+    out, _ = b.debug_op(rt.OP_LADJ, z)
+    assert rel_err(out[:o.n_prim], o.Ladj(z[o.n_prim:])) < 1e-13
+    out, _ = b.debug_op(rt.OP_DYN, z)
+    xo, uo = o.dyn_project(x0s, z[o.sU:o.sU + o.sX], z[:o.sU])
+    assert rel_err(out[o.sU:o.sU + o.sX], xo) < 1e-12
+    assert rel_err(out[:o.sU], uo) < 1e-12
+    out, _ = b.debug_op(rt.OP_KERNEL, z)
+    ref = o.kernel_project(z[:o.n_prim])
+    assert rel_err(out[o.sU + o.sX:o.n_prim], ref[o.sU + o.sX:]) < 1e-13
+    out, _ = b.debug_op(rt.OP_T, z)
+    assert rel_err(out, o.T(x0s, z)) < 1e-12
+    y = inert_zero(o, rng.standard_normal(o.n_iter))
+    _, sc = b.debug_op(rt.OP_ELL, z, y)
+    ell = np.concatenate([o.Ladj(y[o.n_prim:]), o.L(y[:o.n_prim])])
+    assert abs(sc[0] - o.dotM(z, y)) <= 1e-12 * max(1.0, abs(o.dotM(z, y)))
+    assert abs(sc[1] - np.max(np.abs(y / o.gamma - ell))) <= 1e-12 * np.max(np.abs(y / o.gamma))
+
+
+@pytest.mark.parametrize("name,iters,method", [("fixture", 75, "spock"), ("small", 75, "spock"),
+                                               ("small", 250, "cp"), ("fixture", 250, "cp"), ("mid", 50, "spock")])
+def test_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method):
+    d = problem_dirs[name]
+    o = Oracle(d)
+    x0 = x0_for(name, o.nx)
+    ref = o.solve(x0, method=method, tol_abs=1e-30, max_iters=iters)
+    p = rt.Problem(ctx, d)
+    b = rt.Batch(ctx, [p], rt.F64)
+    b.set_x0([x0])
+    b.solve(rt.options(tol_abs=1e-30, max_iters=iters, method=rt.SPOCK if method == "spock" else rt.CP,
+                       block_size=256))
+    res, _ = b.results()
+    r = res[0]
+    assert r["status"] == ref["status"] == 1
+    assert r["iters"] == ref["iters"]
+    assert (r["k0"], r["k1"], r["k2"], r["k2bt"], r["k3"]) == (ref["k0"], ref["k1"], ref["k2"], ref["k2bt"], ref["k3"])
+    assert rel_err(b.iterate(0), ref["z"]) < 1e-9
+
+
+def fixed_point_residual(o, x0, z):
+    """||z - T(z)||_inf / ||z||_inf computed by the oracle for a GPU iterate."""
+    return float(np.max(np.abs(z - o.T(o.scaled_x0(x0), z))) / np.max(np.abs(z)))
+
+
+def gpu_solve(ctx, problems, x0s, dtype=rt.F64, **opts):
+    b = rt.Batch(ctx, problems, dtype)
+    b.set_x0(x0s)
+    b.solve(rt.options(**opts))
+    res, sec = b.results()
+    return b, res
+
+
+@pytest.mark.parametrize("name", ["fixture", "small"])
+def test_converged_solve_matches_oracle(ctx, problem_dirs, name):
+    d = problem_dirs[name]
+    o = Oracle(d)
+    x0 = x0_for(name, o.nx)
+    ref = o.solve(x0, tol_abs=1e-3, max_iters=3000)
+    b, res = gpu_solve(ctx, [rt.Problem(ctx, d)], [x0], tol_abs=1e-3, max_iters=3000)
+    r = res[0]
+    z = b.iterate(0)
+    assert ref["status"] == 0 and r["status"] == 0
+    assert abs(r["iters"] - ref["iters"]) <= 50
+    s0, s0_ref = z[o.n_prim - o.sS], ref["z"][o.n_prim - o.sS]
+    assert abs(s0 - s0_ref) <= 1e-2 * max(1.0, abs(s0_ref))
+    assert fixed_point_residual(o, x0, z) <= 5 * fixed_point_residual(o, x0, ref["z"]) + 1e-6
+
+
+def test_batch_of_initial_states_matches_single_solves(ctx, problem_dirs):
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    p = rt.Problem(ctx, d)
+    B = 6
+    x0s = [x0_for("small", o.nx, k) for k in range(B)]
+    opts = dict(tol_abs=1e-3, max_iters=3000)
+    b, res = gpu_solve(ctx, [p] * B, x0s, **opts)
+    for k in range(B):
+        bk, rk = gpu_solve(ctx, [p], [x0s[k]], **opts)
+        assert res[k]["status"] == rk[0]["status"] == 0
+        assert res[k]["iters"] == rk[0]["iters"]
+        assert np.array_equal(b.iterate(k), bk.iterate(0))
+
+
+def test_uneven_batch_matches_single_solves(ctx, problem_dirs):
+    names = ["fixture", "small", "mid", "small", "fixture"]
+    probs = {n: rt.Problem(ctx, problem_dirs[n]) for n in set(names)}
+    oracles = {n: Oracle(problem_dirs[n]) for n in set(names)}
+    x0s = [x0_for(n, oracles[n].nx, k) for k, n in enumerate(names)]
+    opts = dict(tol_abs=1e-3, tol_rel=1e-3, max_iters=400)
+    b, res = gpu_solve(ctx, [probs[n] for n in names], x0s, **opts)
+    for k, n in enumerate(names):
+        bk, rk = gpu_solve(ctx, [probs[n]], [x0s[k]], **opts)
+        assert res[k]["status"] == rk[0]["status"]
+        assert res[k]["iters"] == rk[0]["iters"]
+        assert np.array_equal(b.iterate(k), bk.iterate(0))
+
+
+def test_fp32_solve_tracks_fp64(ctx, problem_dirs):
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    x0 = x0_for("small", o.nx)
+    ref = o.solve(x0, tol_abs=1e-3, max_iters=3000)
+    b, res = gpu_solve(ctx, [rt.Problem(ctx, d)], [x0], rt.F32, tol_abs=1e-3, max_iters=3000)
+    assert res[0]["status"] == 0
+    z = b.iterate(0)
+    s0 = z[o.n_prim - o.sS]
+    s0_ref = ref["z"][o.n_prim - o.sS]
+    assert abs(s0 - s0_ref) <= 2e-2 * max(1.0, abs(s0_ref))
