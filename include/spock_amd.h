@@ -42,6 +42,9 @@ enum spk_dtype { SPK_F64 = 0, SPK_F32 = 1 };
 enum spk_method { SPK_SPOCK = 0, SPK_CP = 1 };
 /* solve status codes, identical to enum CacheStatus (src/cache.cuh:68-73) */
 enum spk_status { SPK_NOT_RUN = -1, SPK_CONVERGED = 0, SPK_OUT_OF_ITER = 1, SPK_OUT_OF_TIME = 2 };
+/* Anderson least squares: Householder QR as the reference (geqrf/ormqr/trsm), or the Gram
+ * matrix kept incrementally and solved by Cholesky in FP64 (one fused pass per iteration). */
+enum spk_ls_method { SPK_LS_HOUSEHOLDER = 0, SPK_LS_GRAM = 1 };
 enum spk_debug_op { SPK_OP_L = 0, SPK_OP_LADJ = 1, SPK_OP_DYN = 2, SPK_OP_KERNEL = 3, SPK_OP_T = 4, SPK_OP_ELL = 5 };
 
 /* Builder options (src/cache.cuh:1154-1210).  tol_abs/tol_rel are multiplied by
@@ -58,6 +61,8 @@ typedef struct spk_options {
     int check_period;
     int method;
     int block_size;
+    int ls_method;   /* enum spk_ls_method */
+    int dyn_inverse; /* 1: apply (L L')^-1 as one matvec; 0: two triangular solves as the reference */
 } spk_options;
 
 typedef struct spk_result {
@@ -69,6 +74,10 @@ typedef struct spk_result {
     long k0, k1, k2, k2bt, k3;
     long num_T;   /* applications of the CP operator T */
     long num_ell; /* applications of [L*; L] inside norms and error checks */
+    /* device wall-clock seconds per phase: L* + axpy, dynamics/kernel projections, L + dual prox,
+     * residual, [L*; L] reductions, Anderson column, Anderson least squares, Anderson direction,
+     * line-search updates, other */
+    double phase_seconds[10];
 } spk_result;
 
 void spk_default_options(spk_options *opts);

@@ -16,10 +16,10 @@
 #include "loader.hpp"
 
 namespace spk {
-template <typename T>
-__global__ void k_solve(const Inst<T> *, Out *, int, double);
-template <typename T>
-__global__ void k_debug(const Inst<T> *, Out *, int, int);
+template <typename T, int R, int V>
+__global__ void k_solve(const Inst<T> *, Out *, int, double, int);
+template <typename T, int R, int V>
+__global__ void k_debug(const Inst<T> *, Out *, int, int, int);
 __global__ void k_f32_to_f64(const float *, double *, long);
 __global__ void k_f64_to_f32(const double *, float *, long);
 template <typename T>
@@ -129,7 +129,7 @@ void uploadTable(spk_problem *p, const std::string &name, const Table &t) {
 }
 
 #define SPK_TABLES(X) \
-    X(BTr) X(AB) X(K) X(KTr) X(ABKTr) X(APB) X(cholL) X(cholU) X(NN) X(b) X(sQ) X(sR) X(q) X(r) X(trans) \
+    X(BTr) X(AB) X(K) X(KTr) X(ABKTr) X(APB) X(cholL) X(cholU) X(RtInv) X(NN) X(b) X(sQ) X(sR) X(q) X(r) X(trans) \
     X(lsQ) X(lq) X(ltrans) X(nG) X(nLB) X(nUB) X(lG) X(lLB) X(lUB) X(nGT) X(lGT)
 
 template <typename T>
@@ -210,6 +210,8 @@ void buildInsts(spk_batch *b, const spk_options &o, int warm) {
         I.checkPeriod = o.check_period > 0 ? o.check_period : 25;
         I.method = o.method;
         I.warm = warm;
+        I.lsMethod = o.ls_method;
+        I.dynInverse = o.dyn_inverse;
         I.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
     }
     b->dInst.alloc(b->hInst.size());
@@ -263,16 +265,54 @@ void validateProblem(const HostProblem &h) {
         throw std::runtime_error("constraint rows > 256 are not supported by the fused engine");
 }
 
+// Smallest register chunk class covering every per-node dimension of the batch.
+int chunkClass(const spk_batch *b) {
+    int mx = 1;
+    for (auto *p : b->probs) {
+        const HostProblem &h = p->hp;
+        mx = std::max({mx, h.nx, h.nu, h.con3, h.con5, h.nD, h.yD});
+    }
+    return mx <= 64 ? 1 : 4;
+}
+
+// Ints of per-node index and tree arrays staged in LDS by the kernels (0 when they do not
+// fit in the budget, then the kernels read them from global memory).
+int ldsPoolInts(const spk_batch *b) {
+    int mx = 0;
+    for (auto *p : b->probs) {
+        const HostProblem &h = p->hp;
+        const int need = 10 * h.N + 15 * h.nl + 7 * h.nlf + 2 * h.S;
+        mx = std::max(mx, need);
+    }
+    const int budget = 48 * 1024 / 4;
+    return mx <= budget ? mx : 0;
+}
+
+template <typename T>
+void launchKernel(spk_batch *b, int bs, bool debug, int op) {
+    const int cls = chunkClass(b);
+    const int pool = ldsPoolInts(b);
+    const size_t shm = (size_t) pool * sizeof(int);
+    auto *ins = static_cast<const spk::Inst<T> *>(b->dInst.p);
+    auto *outs = static_cast<spk::Out *>(b->dOut.p);
+    hipStream_t st = b->ctx->stream;
+    if (!debug) {
+        if (cls == 1) hipLaunchKernelGGL((spk::k_solve<T, 1, 1>), dim3(b->count), dim3(bs), shm, st, ins, outs, b->count, b->ctx->tickRateHz, pool);
+        else hipLaunchKernelGGL((spk::k_solve<T, 4, 4>), dim3(b->count), dim3(bs), shm, st, ins, outs, b->count, b->ctx->tickRateHz, pool);
+    } else {
+        if (cls == 1) hipLaunchKernelGGL((spk::k_debug<T, 1, 1>), dim3(b->count), dim3(bs), shm, st, ins, outs, b->count, op, pool);
+        else hipLaunchKernelGGL((spk::k_debug<T, 4, 4>), dim3(b->count), dim3(bs), shm, st, ins, outs, b->count, op, pool);
+    }
+}
+
 template <typename T>
 void launchSolve(spk_batch *b, const spk_options &o, int warm) {
     buildInsts<T>(b, o, warm);
     int bs = o.block_size > 0 ? o.block_size : 256;
-    bs = std::max(64, std::min(1024, bs / 64 * 64));
+    bs = std::max(64, std::min(512, bs / 64 * 64));
     b->dOut.alloc(sizeof(spk::Out) * b->count);
     check(hipEventRecord(b->ev0, b->ctx->stream), "event");
-    hipLaunchKernelGGL(spk::k_solve<T>, dim3(b->count), dim3(bs), 0, b->ctx->stream,
-                       static_cast<const spk::Inst<T> *>(b->dInst.p), static_cast<spk::Out *>(b->dOut.p), b->count,
-                       b->ctx->tickRateHz);
+    launchKernel<T>(b, bs, false, 0);
     check(hipGetLastError(), "launch k_solve");
     check(hipEventRecord(b->ev1, b->ctx->stream), "event");
     b->pending = true;
@@ -292,6 +332,8 @@ void spk_default_options(spk_options *o) {
     o->check_period = 25;
     o->method = SPK_SPOCK;
     o->block_size = 256;
+    o->ls_method = SPK_LS_HOUSEHOLDER;
+    o->dyn_inverse = 1;
 }
 
 int spk_ctx_create(int device, spk_ctx **out) {
@@ -545,6 +587,8 @@ int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm) {
         else spk_default_options(&o);
         if (o.anderson_buffer < 1 || o.anderson_buffer > 8) throw std::runtime_error("anderson_buffer must be 1..8");
         if (o.max_inner < 0) throw std::runtime_error("max_inner must be >= 0");
+        if (o.ls_method != SPK_LS_HOUSEHOLDER && o.ls_method != SPK_LS_GRAM)
+            throw std::runtime_error("ls_method must be SPK_LS_HOUSEHOLDER or SPK_LS_GRAM");
         check(hipSetDevice(b->ctx->device), "hipSetDevice");
         ensureAnderson(b, o.anderson_buffer);
         if (b->dtype == SPK_F64) launchSolve<double>(b, o, warm);
@@ -580,6 +624,8 @@ int spk_batch_results(spk_batch *b, spk_result *out, double *kernelSeconds) {
                 out[k].k3 = tmp[k].k3;
                 out[k].num_T = tmp[k].nT;
                 out[k].num_ell = tmp[k].nEll;
+                for (int ph = 0; ph < spk::NUM_PHASES; ph++)
+                    out[k].phase_seconds[ph] = tmp[k].ticks[ph] / b->ctx->tickRateHz;
             }
         }
         return 0;
@@ -612,14 +658,10 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
         check(hipMemset(b->dOut.p, 0, sizeof(spk::Out) * b->count), "zero out");
         if (b->dtype == SPK_F64) {
             buildInsts<double>(b, o, 0);
-            hipLaunchKernelGGL(spk::k_debug<double>, dim3(b->count), dim3(256), 0, b->ctx->stream,
-                               static_cast<const spk::Inst<double> *>(b->dInst.p), static_cast<spk::Out *>(b->dOut.p),
-                               b->count, op);
+            launchKernel<double>(b, 256, true, op);
         } else {
             buildInsts<float>(b, o, 0);
-            hipLaunchKernelGGL(spk::k_debug<float>, dim3(b->count), dim3(256), 0, b->ctx->stream,
-                               static_cast<const spk::Inst<float> *>(b->dInst.p), static_cast<spk::Out *>(b->dOut.p),
-                               b->count, op);
+            launchKernel<float>(b, 256, true, op);
         }
         check(hipGetLastError(), "launch k_debug");
         check(hipStreamSynchronize(b->ctx->stream), "sync debug op");

@@ -20,7 +20,7 @@ struct Inst {
     long oU, oX, oY, oT, oS, sPrim, oI, oII, oIII, oIV, oV, oVI, sDual, sIter;
     T gamma, gammaRecip;
     const int *anc, *chFrom, *numCh, *stageFrom, *stageTo;
-    DTab<T> BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, NN, b, sQ, sR, q, r, trans, lsQ, lq, ltrans;
+    DTab<T> BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, RtInv, NN, b, sQ, sR, q, r, trans, lsQ, lq, ltrans;
     DTab<T> nG, nLB, nUB, lG, lLB, lUB, nGT, lGT;
     // state (iterate-sized unless noted)
     T *z, *zPrev, *cand, *backup, *res, *resPrev, *dir, *dirS;
@@ -31,9 +31,13 @@ struct Inst {
     // options
     T tolAbs, tolRel;  // already multiplied by min(scaling) when preconditioned
     long maxIters;
-    int maxInner, andBuff, checkPeriod, method, warm;
+    int maxInner, andBuff, checkPeriod, method, warm, lsMethod, dynInverse;
     double maxTicks;  // wall-clock cap in device ticks (0 = none)
 };
+
+// Phases timed inside the solve (wall_clock64 ticks, summed over the solve).
+enum Phase { PH_ADJ = 0, PH_PROJ = 1, PH_DUAL = 2, PH_RES = 3, PH_ELL = 4, PH_ACOL = 5, PH_ALS = 6, PH_AAPPLY = 7,
+             PH_LINE = 8, PH_OTHER = 9, NUM_PHASES = 10 };
 
 struct Out {
     int status;
@@ -41,6 +45,7 @@ struct Out {
     long iters;
     double err, tol, w;
     long k0, k1, k2, k2bt, k3, nT, nEll;
+    long ticks[NUM_PHASES];  // device wall-clock ticks spent per phase (thread 0's view)
 };
 
 enum Method { METHOD_SPOCK = 0, METHOD_CP = 1 };

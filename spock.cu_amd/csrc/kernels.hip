@@ -14,6 +14,10 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#ifndef SPK_MAX_THREADS
+#define SPK_MAX_THREADS 512
+#endif
+
 namespace spk {
 
 // ------------------------------------------------------------------ wave primitives
@@ -56,6 +60,22 @@ __device__ __forceinline__ Tc threadCtx() {
     return c;
 }
 
+// Per-phase wall-clock accounting; mark() is called right after a workgroup barrier so the
+// interval covers the whole phase.
+struct PhaseClock {
+    long last;
+    long acc[NUM_PHASES];
+    __device__ void start() {
+        last = wall_clock64();
+        for (int i = 0; i < NUM_PHASES; i++) acc[i] = 0;
+    }
+    __device__ __forceinline__ void mark(int ph) {
+        const long t = wall_clock64();
+        acc[ph] += t - last;
+        last = t;
+    }
+};
+
 template <typename T, int K>
 __device__ __forceinline__ void zero(T (&a)[K]) {
 #pragma unroll
@@ -63,38 +83,47 @@ __device__ __forceinline__ void zero(T (&a)[K]) {
 }
 
 // v[c] holds element lane + 64 c of a vector of length len (0 beyond)
-template <typename T, typename F>
-__device__ __forceinline__ void loadV(T (&v)[MAXV], int len, int lane, F f) {
+template <typename T, int V, typename F>
+__device__ __forceinline__ void loadV(T (&v)[V], int len, int lane, F f) {
 #pragma unroll
-    for (int c = 0; c < MAXV; c++) {
+    for (int c = 0; c < V; c++) {
         const int k = lane + 64 * c;
         v[c] = (k < len) ? f(k) : T(0);
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void loadPtr(T (&v)[MAXV], const T *p, int len, int lane) {
+template <typename T, int V>
+__device__ __forceinline__ void loadPtr(T (&v)[V], const T *p, int len, int lane) {
     loadV(v, len, lane, [&](int k) { return p[k]; });
 }
 
-// acc[rc] (row lane + 64 rc) += sum_{c < cols} M[row + c * ld] * v(c)
-template <typename T>
-__device__ __forceinline__ void mvAcc(T (&acc)[MAXR], const T *M, int rows, int ld, int cols, const T (&v)[MAXV],
-                                      int lane) {
+// acc[rc] (row lane + 64 rc) += sum_{c < cols} M[row + c * ld] * v(c).
+// All columns of a 64-column chunk (32 per block in FP64) are requested before any is
+// consumed, so a matvec costs one or two memory round trips instead of one per column;
+// out-of-range columns load 0 and v holds 0 beyond its length.
+template <typename T, int R, int V>
+__device__ __forceinline__ void mvAcc(T (&acc)[R], const T *M, int rows, int ld, int cols, const T (&v)[V], int lane) {
+    constexpr int U = (sizeof(T) == 8 ? 32 : 64) / R;
 #pragma unroll
-    for (int vc = 0; vc < MAXV; vc++) {
+    for (int vc = 0; vc < V; vc++) {
         if (64 * vc >= cols) break;
         const int cend = min(64, cols - 64 * vc);
-#pragma unroll 4
-        for (int cc = 0; cc < cend; cc++) {
-            const T xc = bcastLane(v[vc], cc);
-            const T *col = M + (size_t) (64 * vc + cc) * ld;
+        const T *base = M + (size_t) (64 * vc) * ld;
+#pragma unroll 1
+        for (int c0 = 0; c0 < cend; c0 += U) {
+            T m[U][R];
 #pragma unroll
-            for (int rc = 0; rc < MAXR; rc++) {
-                if (64 * rc < rows) {
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int rc = 0; rc < R; rc++) {
                     const int r = lane + 64 * rc;
-                    if (r < rows) acc[rc] += col[r] * xc;
+                    m[u][rc] = (c0 + u < cend && r < rows) ? base[(size_t) (c0 + u) * ld + r] : T(0);
                 }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const T xc = bcastLane(v[vc], (c0 + u) & 63);
+#pragma unroll
+                for (int rc = 0; rc < R; rc++) acc[rc] += m[u][rc] * xc;
             }
         }
     }
@@ -142,23 +171,23 @@ __device__ __forceinline__ T projBox(T v, T lb, T ub) {
 // ------------------------------------------------------------------ L* per node
 // src/operator.cuh:63-82; src/costs.cuh:162-195 (quadratic), :418-458 (quadratic+linear);
 // src/constraints.cuh:171-191 (rectangle), :265-286 (polyhedron).
-template <typename T>
+template <typename T, int R>
 struct AdjOut {
-    T x[MAXR];  // rows nx
-    T u[MAXR];  // rows nu (nonleaf only)
+    T x[R];  // rows nx
+    T u[R];  // rows nu (nonleaf only)
     T y;        // lane < yD (nonleaf only)
     T s, t;     // uniform scalars
 };
 
-template <typename T>
-__device__ __forceinline__ void adjNode(const Inst<T> &P, const T *dual, int n, int lane, AdjOut<T> &o) {
+template <typename T, int R, int V>
+__device__ __forceinline__ void adjNode(const Inst<T> &P, const T *dual, int n, int lane, AdjOut<T, R> &o) {
     const int nx = P.nx, nu = P.nu, nxu = nx + nu, d4 = nxu + 2;
     zero(o.x);
     zero(o.u);
     o.y = T(0);
     o.s = T(0);
     o.t = T(0);
-    T v[MAXV], w[MAXR];
+    T v[V], w[R];
     if (n < P.nl) {
         const T II = dual[P.oII + n];
         o.s = II;
@@ -166,7 +195,7 @@ __device__ __forceinline__ void adjNode(const Inst<T> &P, const T *dual, int n, 
         const long b3 = P.oIII + (long) n * P.con3;
         if (P.ncon == CON_RECT_DEV) {
 #pragma unroll
-            for (int rc = 0; rc < MAXR; rc++) {
+            for (int rc = 0; rc < R; rc++) {
                 const int r = lane + 64 * rc;
                 if (r < nx) o.x[rc] = dual[b3 + r];
                 if (r < nu) o.u[rc] = dual[b3 + nx + r];
@@ -188,33 +217,33 @@ __device__ __forceinline__ void adjNode(const Inst<T> &P, const T *dual, int n, 
             if (P.ncost == COST_QUADLIN_DEV) {
                 const T *qj = P.q.at(j);
 #pragma unroll
-                for (int rc = 0; rc < MAXR; rc++) {
+                for (int rc = 0; rc < R; rc++) {
                     const int r = lane + 64 * rc;
                     if (r < nx) w[rc] = w[rc] - T(0.5) * qj[r] * a - T(0.5) * qj[r] * bb;
                 }
             }
 #pragma unroll
-            for (int rc = 0; rc < MAXR; rc++) o.x[rc] += w[rc];
+            for (int rc = 0; rc < R; rc++) o.x[rc] += w[rc];
             loadPtr(v, iv + nx, nu, lane);
             zero(w);
             mvAcc(w, P.sR.at(j), nu, nu, nu, v, lane);
             if (P.ncost == COST_QUADLIN_DEV) {
                 const T *rj = P.r.at(j);
 #pragma unroll
-                for (int rc = 0; rc < MAXR; rc++) {
+                for (int rc = 0; rc < R; rc++) {
                     const int r = lane + 64 * rc;
                     if (r < nu) w[rc] = w[rc] - T(0.5) * rj[r] * a - T(0.5) * rj[r] * bb;
                 }
             }
 #pragma unroll
-            for (int rc = 0; rc < MAXR; rc++) o.u[rc] += w[rc];
+            for (int rc = 0; rc < R; rc++) o.u[rc] += w[rc];
         }
     } else {
         const int l = n - P.nl;
         const long b5 = P.oV + (long) l * P.con5;
         if (P.lcon == CON_RECT_DEV) {
 #pragma unroll
-            for (int rc = 0; rc < MAXR; rc++) {
+            for (int rc = 0; rc < R; rc++) {
                 const int r = lane + 64 * rc;
                 if (r < nx) o.x[rc] = dual[b5 + r];
             }
@@ -228,11 +257,11 @@ __device__ __forceinline__ void adjNode(const Inst<T> &P, const T *dual, int n, 
         zero(w);
         mvAcc(w, P.lsQ.at(l), nx, nx, nx, v, lane);
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) o.x[rc] += w[rc];
+        for (int rc = 0; rc < R; rc++) o.x[rc] += w[rc];
         if (P.lcost == COST_QUADLIN_DEV) {
             const T *lq = P.lq.at(l);
 #pragma unroll
-            for (int rc = 0; rc < MAXR; rc++) {
+            for (int rc = 0; rc < R; rc++) {
                 const int r = lane + 64 * rc;
                 if (r < nx) o.x[rc] = o.x[rc] - T(0.5) * lq[r] * a - T(0.5) * lq[r] * bb;
             }
@@ -252,10 +281,10 @@ __device__ __forceinline__ void adjNode(const Inst<T> &P, const T *dual, int n, 
 //   blockI(n, yk (lane < yD), IIval)
 //   blockElem(dualIndexBase, vals, rows, lb table entry, ub table entry)  [III and V]
 //   blockSoc(dualIndexBase, xs (rows nx), us (rows nu, may be 0 rows), h, transEntry)  [IV and VI]
-template <typename T, typename PV, typename H>
+template <typename T, int R, int V, typename PV, typename H>
 __device__ __forceinline__ void opNode(const Inst<T> &P, int n, int lane, PV pv, H &h) {
     const int nx = P.nx, nu = P.nu, nxu = nx + nu;
-    T v[MAXV], v2[MAXV], acc[MAXR], acc2[MAXR];
+    T v[V], v2[V], acc[R], acc2[R];
     if (n < P.nl) {
         // I and II
         const T yk = (lane < P.yD) ? pv(P.oY + (long) n * P.yD + lane) : T(0);
@@ -268,7 +297,7 @@ __device__ __forceinline__ void opNode(const Inst<T> &P, int n, int lane, PV pv,
             zero(acc);
             zero(acc2);
 #pragma unroll
-            for (int rc = 0; rc < MAXR; rc++) {
+            for (int rc = 0; rc < R; rc++) {
                 const int r = lane + 64 * rc;
                 if (r < nx) acc[rc] = pv(P.oX + (long) n * nx + r);
                 if (r < nu) acc2[rc] = pv(P.oU + (long) n * nu + r);
@@ -303,7 +332,7 @@ __device__ __forceinline__ void opNode(const Inst<T> &P, int n, int lane, PV pv,
                 const T *rn = P.r.at(n);
                 T pq = T(0), pr = T(0);
 #pragma unroll
-                for (int c = 0; c < MAXV; c++) {
+                for (int c = 0; c < V; c++) {
                     const int k = lane + 64 * c;
                     if (k < nx) pq += qn[k] * v[c];
                     if (k < nu) pr += rn[k] * v2[c];
@@ -321,7 +350,7 @@ __device__ __forceinline__ void opNode(const Inst<T> &P, int n, int lane, PV pv,
         if (P.lcon == CON_RECT_DEV) {
             zero(acc);
 #pragma unroll
-            for (int rc = 0; rc < MAXR; rc++) {
+            for (int rc = 0; rc < R; rc++) {
                 const int r = lane + 64 * rc;
                 if (r < nx) acc[rc] = pv(P.oX + (long) n * nx + r);
             }
@@ -339,7 +368,7 @@ __device__ __forceinline__ void opNode(const Inst<T> &P, int n, int lane, PV pv,
             const T *lq = P.lq.at(l);
             T pq = T(0);
 #pragma unroll
-            for (int c = 0; c < MAXV; c++) {
+            for (int c = 0; c < V; c++) {
                 const int k = lane + 64 * c;
                 if (k < nx) pq += lq[k] * v[c];
             }
@@ -351,7 +380,7 @@ __device__ __forceinline__ void opNode(const Inst<T> &P, int n, int lane, PV pv,
 
 // ---------------------------------------------------------------- L handlers
 // Raw store of L values into out (dual-local indices).
-template <typename T>
+template <typename T, int R>
 struct StoreL {
     const Inst<T> &P;
     T *out;
@@ -360,16 +389,16 @@ struct StoreL {
         if (lane < P.yD) out[P.oI + (long) n * P.yD + lane] = yk;
         if (lane == 0) out[P.oII + n] = IIval;
     }
-    __device__ void blockElem(long base, const T (&a)[MAXR], int rows, const T *, const T *) {
+    __device__ void blockElem(long base, const T (&a)[R], int rows, const T *, const T *) {
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
             if (r < rows) out[base + r] = a[rc];
         }
     }
-    __device__ void blockSoc(long base, const T (&xs)[MAXR], int nxr, const T (&us)[MAXR], int nur, T hv, const T *) {
+    __device__ void blockSoc(long base, const T (&xs)[R], int nxr, const T (&us)[R], int nur, T hv, const T *) {
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
             if (r < nxr) out[base + r] = xs[rc];
             if (r < nur) out[base + nxr + r] = us[rc];
@@ -384,7 +413,7 @@ struct StoreL {
 // Dual prox of the CP step (src/cache.cuh:770-789): a = L*gamma + z_d; w = a/gamma + c;
 // cand_d = gamma (w - Pi(w)).  Pi: masked NNOC (src/risks.cuh:99-106, definitions.cu:430-443),
 // NNOC (src/cones.cuh:134-137), box (src/constraints.cuh:70-74), SOC (src/projections.cuh:102-128).
-template <typename T>
+template <typename T, int R>
 struct ProxL {
     const Inst<T> &P;
     const T *zd;
@@ -406,9 +435,9 @@ struct ProxL {
             cd[k] = (w - p) * P.gamma;
         }
     }
-    __device__ void blockElem(long base, const T (&a)[MAXR], int rows, const T *lb, const T *ub) {
+    __device__ void blockElem(long base, const T (&a)[R], int rows, const T *lb, const T *ub) {
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
             if (r < rows) {
                 const long k = base + r;
@@ -418,12 +447,12 @@ struct ProxL {
             }
         }
     }
-    __device__ void blockSoc(long base, const T (&xs)[MAXR], int nxr, const T (&us)[MAXR], int nur, T hv,
+    __device__ void blockSoc(long base, const T (&xs)[R], int nxr, const T (&us)[R], int nur, T hv,
                              const T *tr) {
-        T wx[MAXR], wu[MAXR];
+        T wx[R], wu[R];
         T ss = T(0);
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
             wx[rc] = T(0);
             wu[rc] = T(0);
@@ -447,7 +476,7 @@ struct ProxL {
         const T sc = (nrm + tl) / (T(2) * nrm + T(1 - i3));
         const T mult = T(1 - i2) * (T(i3) * sc + T(1 - i3));
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
             if (r < nxr) cd[base + r] = (wx[rc] - wx[rc] * mult) * P.gamma;
             if (r < nur) cd[base + nxr + r] = (wu[rc] - wu[rc] * mult) * P.gamma;
@@ -464,7 +493,7 @@ struct ProxL {
 
 // Reduction handler for ell(y) = [L* y_d; L y_p]: accumulates sum x . (y - gamma ell) and
 // max |y / gamma - ell| over the dual part (src/cache.cuh:722-734 and :917-931).
-template <typename T>
+template <typename T, int R>
 struct ReduceL {
     const Inst<T> &P;
     const T *xd, *yd;  // dual parts of the partner and input vectors
@@ -479,16 +508,16 @@ struct ReduceL {
         if (lane < P.yD) acc(P.oI + (long) n * P.yD + lane, yk);
         if (lane == 0) acc(P.oII + n, IIval);
     }
-    __device__ void blockElem(long base, const T (&a)[MAXR], int rows, const T *, const T *) {
+    __device__ void blockElem(long base, const T (&a)[R], int rows, const T *, const T *) {
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
             if (r < rows) acc(base + r, a[rc]);
         }
     }
-    __device__ void blockSoc(long base, const T (&xs)[MAXR], int nxr, const T (&us)[MAXR], int nur, T hv, const T *) {
+    __device__ void blockSoc(long base, const T (&xs)[R], int nxr, const T (&us)[R], int nur, T hv, const T *) {
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
             if (r < nxr) acc(base + r, xs[rc]);
             if (r < nur) acc(base + nxr + r, us[rc]);
@@ -503,16 +532,16 @@ struct ReduceL {
 // ---------------------------------------------------------------- T phases
 // modifyPrimal + proxRootS (src/cache.cuh:748-753, 639-642): cand_p = z_p - gamma L*(z_d); s_0 -= gamma.
 // Leaf tasks also seed the dynamics sweep with q_leaf = -xbar_leaf (src/dynamics.cuh:79-81).
-template <typename T>
-__device__ void phaseAdjAxpy(const Inst<T> &P, const Tc &c, const T *z, T *cand) {
+template <typename T, int R, int V>
+__device__ __noinline__ void phaseAdjAxpy(const Inst<T> &P, const Tc &c, const T *z, T *cand) {
     const T *zd = z + P.sPrim;
     const T g = -P.gamma;
     const int nx = P.nx, nu = P.nu;
     for (int n = c.wave; n < P.N; n += c.nw) {
-        AdjOut<T> o;
-        adjNode(P, zd, n, c.lane, o);
+        AdjOut<T, R> o;
+        adjNode<T, R, V>(P, zd, n, c.lane, o);
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = c.lane + 64 * rc;
             if (r < nx) {
                 const long k = P.oX + (long) n * nx + r;
@@ -539,12 +568,12 @@ __device__ void phaseAdjAxpy(const Inst<T> &P, const Tc &c, const T *z, T *cand)
 }
 
 // Backward sweep step for node i (src/dynamics.cuh:85-139).
-template <typename T>
-__device__ void dynBackwardNode(const Inst<T> &P, T *cand, int i, int stage, int lane) {
+template <typename T, int R, int V>
+__device__ __forceinline__ void dynBackwardNode(const Inst<T> &P, T *cand, int i, int stage, int lane) {
     const int nx = P.nx, nu = P.nu;
     const T *ubar = cand + P.oU + (long) i * nu;
     const T *xbar = cand + P.oX + (long) i * nx;
-    T v[MAXV], w[MAXR], acc[MAXR];
+    T v[V], w[R], acc[R];
     zero(acc);
     const int c0 = P.chFrom[i], nc = P.numCh[i];
     for (int ci = 0; ci < nc; ci++) {
@@ -553,26 +582,36 @@ __device__ void dynBackwardNode(const Inst<T> &P, T *cand, int i, int stage, int
         zero(w);
         mvAcc(w, P.BTr.at(j), nu, nu, nx, v, lane);
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) acc[rc] = (ci == 0) ? w[rc] : acc[rc] + w[rc];
+        for (int rc = 0; rc < R; rc++) acc[rc] = (ci == 0) ? w[rc] : acc[rc] + w[rc];
     }
-    // d = R~^-1 (u - sum B'q) with R~ = L L' (potrs semantics)
+    // d = R~^-1 (u - sum B'q) with R~ = L L': one matvec with the precomputed inverse, or the
+    // two triangular solves of potrs
     T d = (lane < nu) ? (-acc[0] + ubar[lane]) : T(0);
-    const T *L = P.cholL.at(i);
-    for (int k = 0; k < nu; k++) {
-        const T dk = bcastLane(d, k) / L[k + (size_t) k * nu];
-        if (lane == k) d = dk;
-        else if (lane > k && lane < nu) d -= L[lane + (size_t) k * nu] * dk;
-    }
-    const T *U = P.cholU.at(i);
-    for (int k = nu - 1; k >= 0; k--) {
-        const T dk = bcastLane(d, k) / U[k + (size_t) k * nu];
-        if (lane == k) d = dk;
-        else if (lane < k) d -= U[lane + (size_t) k * nu] * dk;
+    if (P.dynInverse) {
+        T dvin[V], dout[R];
+        zero(dvin);
+        dvin[0] = d;
+        zero(dout);
+        mvAcc(dout, P.RtInv.at(i), nu, nu, nu, dvin, lane);
+        d = (lane < nu) ? dout[0] : T(0);
+    } else {
+        const T *L = P.cholL.at(i);
+        for (int k = 0; k < nu; k++) {
+            const T dk = bcastLane(d, k) / L[k + (size_t) k * nu];
+            if (lane == k) d = dk;
+            else if (lane > k && lane < nu) d -= L[lane + (size_t) k * nu] * dk;
+        }
+        const T *U = P.cholU.at(i);
+        for (int k = nu - 1; k >= 0; k--) {
+            const T dk = bcastLane(d, k) / U[k + (size_t) k * nu];
+            if (lane == k) d = dk;
+            else if (lane < k) d -= U[lane + (size_t) k * nu] * dk;
+        }
     }
     if (lane < nu) P.dd[(long) i * nu + lane] = d;
     if (stage == 0) return;
     // q_i = sum_j (APB_j d_i + ABKTr_j q_j) + K_i'(d_i - u_i) - x_i
-    T dv[MAXV];
+    T dv[V];
     zero(dv);
     dv[0] = (lane < nu) ? d : T(0);
     zero(acc);
@@ -583,21 +622,21 @@ __device__ void dynBackwardNode(const Inst<T> &P, T *cand, int i, int stage, int
         loadPtr(v, P.qd + (long) j * nx, nx, lane);
         mvAcc(w, P.ABKTr.at(j), nx, nx, nx, v, lane);
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) acc[rc] = (ci == 0) ? w[rc] : acc[rc] + w[rc];
+        for (int rc = 0; rc < R; rc++) acc[rc] = (ci == 0) ? w[rc] : acc[rc] + w[rc];
     }
     dv[0] = (lane < nu) ? d - ubar[lane] : T(0);
     zero(w);
     mvAcc(w, P.KTr.at(i), nx, nx, nu, dv, lane);
 #pragma unroll
-    for (int rc = 0; rc < MAXR; rc++) {
+    for (int rc = 0; rc < R; rc++) {
         const int r = lane + 64 * rc;
         if (r < nx) P.qd[(long) i * nx + r] = (acc[rc] + w[rc]) - xbar[r];
     }
 }
 
 // Kernel projection for nonleaf node i (src/cache.cuh:650-676); padded child slots enter as 0.
-template <typename T>
-__device__ void kernelNode(const Inst<T> &P, T *cand, int i, int lane) {
+template <typename T, int R, int V>
+__device__ __forceinline__ void kernelNode(const Inst<T> &P, T *cand, int i, int lane) {
     const int yD = P.yD, E = P.E, nc = P.numCh[i], c0 = P.chFrom[i];
     T v = T(0);
     if (lane < yD) v = cand[P.oY + (long) i * yD + lane];
@@ -606,7 +645,7 @@ __device__ void kernelNode(const Inst<T> &P, T *cand, int i, int lane) {
     } else if (lane < yD + 2 * E) {
         if (lane - yD - E < nc) v = cand[P.oS + c0 + (lane - yD - E)];
     }
-    T vin[MAXV], w[MAXR];
+    T vin[V], w[R];
     zero(vin);
     vin[0] = v;
     zero(w);
@@ -620,14 +659,14 @@ __device__ void kernelNode(const Inst<T> &P, T *cand, int i, int lane) {
 }
 
 // Forward rollout for node i (src/dynamics.cuh:144-173).
-template <typename T>
-__device__ void dynForwardNode(const Inst<T> &P, T *cand, int i, int lane) {
+template <typename T, int R, int V>
+__device__ __forceinline__ void dynForwardNode(const Inst<T> &P, T *cand, int i, int lane) {
     const int nx = P.nx, nu = P.nu;
-    T xv[MAXV], uv[MAXV], w[MAXR];
+    T xv[V], uv[V], w[R];
     if (i == 0) {
         loadPtr(xv, P.x0s, nx, lane);
 #pragma unroll
-        for (int c = 0; c < MAXV; c++) {
+        for (int c = 0; c < V; c++) {
             const int k = lane + 64 * c;
             if (k < nx) cand[P.oX + k] = xv[c];
         }
@@ -648,7 +687,7 @@ __device__ void dynForwardNode(const Inst<T> &P, T *cand, int i, int lane) {
         mvAcc(w, AB, nx, nx, nx, xv, lane);
         mvAcc(w, AB + (size_t) nx * nx, nx, nx, nu, uv, lane);
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
             if (r < nx) cand[P.oX + (long) j * nx + r] = w[rc];
         }
@@ -657,45 +696,53 @@ __device__ void dynForwardNode(const Inst<T> &P, T *cand, int i, int lane) {
 
 // Dynamics projection + kernel projection on cand (in place).  The kernel projection touches
 // (y, t, s) only, so it shares the first backward stage's barrier interval.
-template <typename T>
-__device__ void phaseProjections(const Inst<T> &P, const Tc &c, T *cand, bool doDyn, bool doKernel) {
+template <typename T, int R, int V>
+__device__ __noinline__ void phaseProjections(const Inst<T> &P, const Tc &c, T *cand, bool doDyn, bool doKernel) {
     if (doDyn) {
         for (int stage = P.S - 2; stage >= 0; stage--) {
             const int f = P.stageFrom[stage], nStage = P.stageTo[stage] - f + 1;
             const int extra = (doKernel && stage == P.S - 2) ? P.nl : 0;
             for (int task = c.wave; task < nStage + extra; task += c.nw) {
-                if (task < nStage) dynBackwardNode(P, cand, f + task, stage, c.lane);
-                else kernelNode(P, cand, task - nStage, c.lane);
+                if (task < nStage) dynBackwardNode<T, R, V>(P, cand, f + task, stage, c.lane);
+                else kernelNode<T, R, V>(P, cand, task - nStage, c.lane);
             }
             __syncthreads();
         }
         for (int stage = 0; stage <= P.S - 2; stage++) {
             const int f = P.stageFrom[stage], nStage = P.stageTo[stage] - f + 1;
-            for (int task = c.wave; task < nStage; task += c.nw) dynForwardNode(P, cand, f + task, c.lane);
+            for (int task = c.wave; task < nStage; task += c.nw) dynForwardNode<T, R, V>(P, cand, f + task, c.lane);
             __syncthreads();
         }
     } else if (doKernel) {
-        for (int task = c.wave; task < P.nl; task += c.nw) kernelNode(P, cand, task, c.lane);
+        for (int task = c.wave; task < P.nl; task += c.nw) kernelNode<T, R, V>(P, cand, task, c.lane);
         __syncthreads();
     }
 }
 
-// The CP operator T (src/cache.cuh:796-801): cand = T(z).
-template <typename T>
-__device__ void applyT(const Inst<T> &P, const Tc &c, const T *z, T *cand) {
-    phaseAdjAxpy(P, c, z, cand);
-    __syncthreads();
-    phaseProjections(P, c, cand, true, true);
-    // modifyDual + proximalDual on 2 x-bar - z_p
-    ProxL<T> h{P, z + P.sPrim, cand + P.sPrim, c.lane};
+// modifyDual + proximalDual (src/cache.cuh:770-789) on 2 x-bar - z_p.
+template <typename T, int R, int V>
+__device__ __noinline__ void phaseDualProx(const Inst<T> &P, const Tc &c, const T *z, T *cand) {
+    ProxL<T, R> h{P, z + P.sPrim, cand + P.sPrim, c.lane};
     auto pv = [&](long k) { return cand[k] * T(2) - z[k]; };
-    for (int n = c.wave; n < P.N; n += c.nw) opNode(P, n, c.lane, pv, h);
+    for (int n = c.wave; n < P.N; n += c.nw) opNode<T, R, V>(P, n, c.lane, pv, h);
     __syncthreads();
 }
 
+// The CP operator T (src/cache.cuh:796-801): cand = T(z).
+template <typename T, int R, int V>
+__device__ __forceinline__ void applyT(const Inst<T> &P, const Tc &c, const T *z, T *cand, PhaseClock *clk = nullptr) {
+    phaseAdjAxpy<T, R, V>(P, c, z, cand);
+    __syncthreads();
+    if (clk) clk->mark(PH_ADJ);
+    phaseProjections<T, R, V>(P, c, cand, true, true);
+    if (clk) clk->mark(PH_PROJ);
+    phaseDualProx<T, R, V>(P, c, z, cand);
+    if (clk) clk->mark(PH_DUAL);
+}
+
 // sum x.(y - gamma ell(y)) and max |y/gamma - ell(y)| with ell(y) = [L* y_d; L y_p].
-template <typename T>
-__device__ void reduceEll(const Inst<T> &P, const Tc &c, const T *x, const T *y, T *lds, T &dotOut, T &maxOut) {
+template <typename T, int R, int V>
+__device__ __noinline__ void reduceEll(const Inst<T> &P, const Tc &c, const T *x, const T *y, T *lds, T &dotOut, T &maxOut) {
     const int nx = P.nx, nu = P.nu;
     T dot = T(0), mx = T(0);
     auto acc = [&](long k, T ell) {
@@ -704,13 +751,13 @@ __device__ void reduceEll(const Inst<T> &P, const Tc &c, const T *x, const T *y,
         mx = fmax(mx, fabs(yk * P.gammaRecip - ell));
     };
     const T *yd = y + P.sPrim;
-    ReduceL<T> h{P, x + P.sPrim, yd, c.lane, T(0), T(0)};
+    ReduceL<T, R> h{P, x + P.sPrim, yd, c.lane, T(0), T(0)};
     auto pv = [&](long k) { return y[k]; };
     for (int n = c.wave; n < P.N; n += c.nw) {
-        AdjOut<T> o;
-        adjNode(P, yd, n, c.lane, o);
+        AdjOut<T, R> o;
+        adjNode<T, R, V>(P, yd, n, c.lane, o);
 #pragma unroll
-        for (int rc = 0; rc < MAXR; rc++) {
+        for (int rc = 0; rc < R; rc++) {
             const int r = c.lane + 64 * rc;
             if (r < nx) acc(P.oX + (long) n * nx + r, o.x[rc]);
             if (n < P.nl && r < nu) acc(P.oU + (long) n * nu + r, o.u[rc]);
@@ -720,7 +767,7 @@ __device__ void reduceEll(const Inst<T> &P, const Tc &c, const T *x, const T *y,
             acc(P.oS + n, o.s);
             acc(P.oT + n, o.t);
         }
-        opNode(P, n, c.lane, pv, h);
+        opNode<T, R, V>(P, n, c.lane, pv, h);
     }
     T v[1] = {dot + h.dot};
     blockSum(v, lds, c);
@@ -728,33 +775,62 @@ __device__ void reduceEll(const Inst<T> &P, const Tc &c, const T *x, const T *y,
     maxOut = blockMax(fmax(mx, h.mx), lds, c);
 }
 
+// ---------------------------------------------------------------- iterate-wide passes
+// Every thread issues U independent loads before it stores anything, so a single
+// workgroup keeps U * blockDim loads in flight while it streams an iterate.
+template <int U, typename Ld, typename St>
+__device__ __forceinline__ void stream(long n, const Tc &c, Ld ld, St st) {
+    using Vals = decltype(ld(0L));
+    const long nt = c.nt;
+    long k = c.tid;
+    for (; k + (U - 1) * nt < n; k += U * nt) {
+        Vals v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = ld(k + u * nt);
+#pragma unroll
+        for (int u = 0; u < U; u++) st(k + u * nt, v[u]);
+    }
+    for (; k < n; k += nt) st(k, ld(k));
+}
+
+template <typename T>
+struct V2 {
+    T a, b;
+};
+template <typename T>
+struct V4 {
+    T a, b, c, d;
+};
+
 // ---------------------------------------------------------------- Anderson direction
 // src/cache.cuh:861-904 with the least squares argmin ||Y g - r|| solved by Householder QR
 // (geqrf + ormqr + trsm).  Y and S are rings of andBuff columns; logical column 0 is the newest.
 template <typename T>
-__device__ void andersonDirection(const Inst<T> &P, const Tc &c, long idx, int head, const T *r, T *dir, T *lds) {
+__device__ __noinline__ void andersonHouseholder(const Inst<T> &P, const Tc &c, int head, const T *r, T *lds,
+                                                double (&gs)[8]) {
     const long n = P.sIter;
     const int m = P.andBuff;
     auto slot = [&](int lc) { return (head - lc + m) % m; };
-    if (idx < m) {
-        for (long k = c.tid; k < n; k += c.nt) dir[k] = -r[k];
-        __syncthreads();
-        return;
-    }
     T *W = P.W;
     // copy [Y (logical order) | r] and the first column's tail norm
+    const T *Yl[8];
+#pragma unroll
+    for (int lc = 0; lc < 8; lc++) Yl[lc] = P.hY + (long) slot(lc < m ? lc : 0) * n;
     T sig[1] = {T(0)};
-    for (long k = c.tid; k < n; k += c.nt) {
-        for (int lc = 0; lc < m; lc++) W[(long) lc * n + k] = P.hY[(long) slot(lc) * n + k];
-        W[(long) m * n + k] = r[k];
-        if (k >= 1) {
-            const T y0 = P.hY[(long) slot(0) * n + k];
-            sig[0] += y0 * y0;
-        }
-    }
+    stream<4>(
+        n, c, [&](long k) { return k; },
+        [&](long k, long) {
+#pragma unroll
+            for (int lc = 0; lc < 8; lc++)
+                if (lc < m) W[(long) lc * n + k] = Yl[lc][k];
+            W[(long) m * n + k] = r[k];
+            if (k >= 1) sig[0] += Yl[0][k] * Yl[0][k];
+        });
     blockSum(sig, lds, c);
     T sigma = sig[0];
     T Rd[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) Rd[q] = T(1);
     for (int kk = 0; kk < m; kk++) {
         const T alpha = W[(long) kk * n + kk];
         T tau = T(0), beta = alpha, scale = T(0);
@@ -764,30 +840,43 @@ __device__ void andersonDirection(const Inst<T> &P, const Tc &c, long idx, int h
             tau = (beta - alpha) / beta;
             scale = T(1) / (alpha - beta);
         }
-        Rd[kk] = beta;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q == kk) Rd[q] = beta;
         const T *Wk = W + (long) kk * n;
+        const long len = n - kk;
         if (tau != T(0)) {
             T wj[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) wj[j] = T(0);
-            for (long k = kk + c.tid; k < n; k += c.nt) {
-                const T vk = (k == kk) ? T(1) : Wk[k] * scale;
-                for (int j = kk + 1; j <= m; j++) wj[j - kk - 1] += vk * W[(long) j * n + k];
-            }
-            // block reduce the (m - kk) dot products
-            blockSum(wj, lds, c);
+            for (int jj = 0; jj < 8; jj++) wj[jj] = T(0);
+            stream<4>(
+                len, c, [&](long k) { return k; },
+                [&](long k0, long) {
+                    const long k = k0 + kk;
+                    const T vk = (k == kk) ? T(1) : Wk[k] * scale;
+#pragma unroll
+                    for (int jj = 0; jj < 8; jj++)
+                        if (kk + 1 + jj <= m) wj[jj] += vk * W[(long) (kk + 1 + jj) * n + k];
+                });
+            blockSum(wj, lds, c);  // the (m - kk) reflector dot products
             T wt[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) wt[j] = wj[j] * tau;
+            for (int jj = 0; jj < 8; jj++) wt[jj] = wj[jj] * tau;
             T sg[1] = {T(0)};
-            for (long k = kk + c.tid; k < n; k += c.nt) {
-                const T vk = (k == kk) ? T(1) : Wk[k] * scale;
-                for (int j = kk + 1; j <= m; j++) {
-                    T &e = W[(long) j * n + k];
-                    e -= wt[j - kk - 1] * vk;
-                    if (j == kk + 1 && k >= kk + 2) sg[0] += e * e;
-                }
-            }
+            stream<4>(
+                len, c, [&](long k) { return k; },
+                [&](long k0, long) {
+                    const long k = k0 + kk;
+                    const T vk = (k == kk) ? T(1) : Wk[k] * scale;
+#pragma unroll
+                    for (int jj = 0; jj < 8; jj++) {
+                        if (kk + 1 + jj <= m) {
+                            T &e = W[(long) (kk + 1 + jj) * n + k];
+                            e -= wt[jj] * vk;
+                            if (jj == 0 && k >= kk + 2) sg[0] += e * e;
+                        }
+                    }
+                });
             blockSum(sg, lds, c);
             sigma = sg[0];
         } else {
@@ -801,43 +890,204 @@ __device__ void andersonDirection(const Inst<T> &P, const Tc &c, long idx, int h
             sigma = sg[0];
         }
     }
-    // back substitution R g = (Q' r)[0:m]; row kk of R sits in W[j][kk]
+    // back substitution R g = (Q' r)[0:m]; row q of R sits in W[j][q]
     T g[8];
-    for (int kk = m - 1; kk >= 0; kk--) {
-        T v = W[(long) m * n + kk];
-        for (int j = kk + 1; j < m; j++) v -= W[(long) j * n + kk] * g[j];
-        g[kk] = v / Rd[kk];
+#pragma unroll
+    for (int q = 7; q >= 0; q--) {
+        g[q] = T(0);
+        if (q < m) {
+            T v = W[(long) m * n + q];
+#pragma unroll
+            for (int j = q + 1; j < 8; j++)
+                if (j < m) v -= W[(long) j * n + q] * g[j];
+            g[q] = v / Rd[q];
+        }
     }
-    for (long k = c.tid; k < n; k += c.nt) {
-        T sg = T(0);
-        for (int lc = 0; lc < m; lc++) sg += P.hS[(long) slot(lc) * n + k] * g[lc];
-        dir[k] = -sg - r[k];
+    // logical column lc sits in slot (head - lc) mod m
+#pragma unroll
+    for (int lc = 0; lc < 8; lc++)
+        if (lc < m) gs[slot(lc)] = (double) g[lc];
+    __syncthreads();
+}
+
+// New Anderson column (src/cache.cuh:863-868): Y_h = r - r_prev, S_h = (z - z_prev) - Y_h.
+// With the Gram least squares it also accumulates, in FP64, <Y_h, Y_s> for every slot s and
+// <Y_s, r> for every slot, and updates the slot-indexed Gram matrix G held in LDS.
+template <typename T>
+__device__ __noinline__ void andersonColumn(const Inst<T> &P, const Tc &c, int head, const T *Rv, const T *RP,
+                                            const T *Z, const T *ZP, bool gram, double *G, double *gb, double *ldsd) {
+    const long n = P.sIter;
+    const int m = P.andBuff;
+    T *Yh = P.hY + (long) head * n, *Sh = P.hS + (long) head * n;
+    if (!gram) {
+        stream<8>(
+            n, c, [&](long k) { return V4<T>{Rv[k], RP[k], Z[k], ZP[k]}; },
+            [&](long k, const V4<T> &v) {
+                const T dr = v.a - v.b;
+                Yh[k] = dr;
+                Sh[k] = (v.c - v.d) - dr;
+            });
+        __syncthreads();
+        return;
+    }
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = 0.;
+    const T *Ys[8];
+#pragma unroll
+    for (int s2 = 0; s2 < 8; s2++) Ys[s2] = P.hY + (long) (s2 < m ? s2 : 0) * n;
+    stream<4>(
+        n, c, [&](long k) { return V4<T>{Rv[k], RP[k], Z[k], ZP[k]}; },
+        [&](long k, const V4<T> &v) {
+            const T dr = v.a - v.b;
+            Yh[k] = dr;
+            Sh[k] = (v.c - v.d) - dr;
+            const double drd = (double) dr, rd = (double) v.a;
+#pragma unroll
+            for (int s2 = 0; s2 < 8; s2++) {
+                if (s2 < m) {
+                    const double ys = (s2 == head) ? drd : (double) Ys[s2][k];
+                    acc[s2] += drd * ys;
+                    acc[8 + s2] += ys * rd;
+                }
+            }
+        });
+    blockSum(acc, ldsd, c);
+    if (c.tid == 0) {
+        for (int s2 = 0; s2 < m; s2++) {
+            G[head * 8 + s2] = acc[s2];
+            G[s2 * 8 + head] = acc[s2];
+            gb[s2] = acc[8 + s2];
+        }
+    }
+    __syncthreads();
+}
+
+// Least squares argmin ||Y g - r|| from the Gram matrix: Cholesky of G + eps * max(diag) I in
+// FP64 (uniform in every thread).  Returns false when the factorisation breaks down.
+__device__ __forceinline__ bool gramSolve(const double *G, const double *gb, int m, double (&g)[8]) {
+    double Lm[8][8];
+    double dmax = 0.;
+    for (int i = 0; i < m; i++) dmax = fmax(dmax, G[i * 8 + i]);
+    if (!(dmax > 0.)) return false;
+    const double ridge = 1e-12 * dmax;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (i < m && j <= i) {
+                double v = G[i * 8 + j] + (i == j ? ridge : 0.);
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (k < j) v -= Lm[i][k] * Lm[j][k];
+                if (i == j) {
+                    if (!(v > 0.)) return false;
+                    Lm[i][i] = sqrt(v);
+                } else {
+                    Lm[i][j] = v / Lm[j][j];
+                }
+            }
+        }
+    }
+    double y[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        y[i] = 0.;
+        if (i < m) {
+            double v = gb[i];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (k < i) v -= Lm[i][k] * y[k];
+            y[i] = v / Lm[i][i];
+        }
+    }
+#pragma unroll
+    for (int i = 7; i >= 0; i--) {
+        g[i] = 0.;
+        if (i < m) {
+            double v = y[i];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (k > i && k < m) v -= Lm[k][i] * g[k];
+            g[i] = v / Lm[i][i];
+        }
+    }
+    return true;
+}
+
+// d = -S g - r (src/cache.cuh:897-898); with toCand the K0 update cand = z_prev + d is written
+// directly (src/cache.cuh:1055-1057) and d itself is not stored.
+template <typename T>
+__device__ __noinline__ void andersonApply(const Inst<T> &P, const Tc &c, const double (&gs)[8], bool useLS,
+                                           const T *Rv, const T *ZP, T *out, bool toCand) {
+    const long n = P.sIter;
+    const int m = P.andBuff;
+    T gg[8];
+    const T *Ss[8];
+#pragma unroll
+    for (int s2 = 0; s2 < 8; s2++) {
+        gg[s2] = (useLS && s2 < m) ? (T) gs[s2] : T(0);
+        Ss[s2] = P.hS + (long) (s2 < m ? s2 : 0) * n;
+    }
+    if (!useLS) {
+        stream<8>(
+            n, c, [&](long k) { return V2<T>{Rv[k], ZP[k]}; },
+            [&](long k, const V2<T> &v) {
+                const T d = -v.a;
+                out[k] = toCand ? v.b + d : d;
+            });
+    } else {
+        stream<4>(
+            n, c, [&](long k) { return V2<T>{Rv[k], ZP[k]}; },
+            [&](long k, const V2<T> &v) {
+                T sg = T(0);
+#pragma unroll
+                for (int s2 = 0; s2 < 8; s2++)
+                    if (s2 < m) sg += Ss[s2][k] * gg[s2];
+                const T d = -sg - v.a;
+                out[k] = toCand ? v.b + d : d;
+            });
     }
     __syncthreads();
 }
 
 // ---------------------------------------------------------------- the solve
-template <typename T>
-__device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double tickRate) {
+// SuperMann with Anderson directions (src/cache.cuh:1008-1132) and runCp (:967-1002).
+// The six iterate buffers rotate by pointer instead of being copied: acceptCandidate,
+// backup/restore and saveToPrev are pointer moves.
+template <typename T, int R, int V>
+__device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, double tickRate) {
     const Tc c = threadCtx();
     const long n = P.sIter;
-    T *z = P.z, *cand = P.cand;
     const int m = P.andBuff;
+    const bool gram = P.lsMethod == 1;
+    __shared__ double G[64], gb[8];
+    T *Z = P.z, *C = P.cand, *ZP = P.zPrev, *RR = P.res, *RP = P.resPrev, *BK = P.backup;
+    T *D = P.dir, *DS = P.dirS;
     int head = m - 1;
     const long t0 = wall_clock64();
     long nT = 0, nEll = 0;
+    PhaseClock clk;
+    clk.start();
     // initialisePrev / initialiseIter (src/cache.cuh:611-632)
     if (P.method == METHOD_SPOCK) {
-        for (long k = c.tid; k < n; k += c.nt) {
-            if (P.warm) {
-                P.zPrev[k] = z[k];
-                P.resPrev[k] = P.res[k];
-            } else {
-                P.zPrev[k] = T(1);
-                P.resPrev[k] = T(1);
-            }
+        if (P.warm) {
+            stream<8>(
+                n, c, [&](long k) { return V2<T>{Z[k], RR[k]}; },
+                [&](long k, const V2<T> &v) {
+                    ZP[k] = v.a;
+                    RP[k] = v.b;
+                });
+        } else {
+            stream<8>(
+                n, c, [&](long k) { return 0; },
+                [&](long k, int) {
+                    ZP[k] = T(1);
+                    RP[k] = T(1);
+                });
         }
     }
+    if (c.tid < 64) G[c.tid] = 0.;
     __syncthreads();
     bool initFlag = true;
     int status = ST_NOTRUN;
@@ -851,26 +1101,26 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double tickRat
     long it = 0;
     for (it = 0; it < hardCap; it++) {
         if (it != 0 && P.method == METHOD_SPOCK) {  // acceptCandidate: z <- cand
-            T *tmp = z;
-            z = cand;
-            cand = tmp;
+            T *tmp = Z;
+            Z = C;
+            C = tmp;
         }
-        applyT(P, c, z, cand);
+        clk.mark(PH_OTHER);
+        applyT<T, R, V>(P, c, Z, C, &clk);
         nT++;
-        // computeError (src/cache.cuh:910-961): r = z - cand, backup = cand
-        for (long k = c.tid; k < n; k += c.nt) {
-            const T cv = cand[k];
-            P.res[k] = z[k] - cv;
-            P.backup[k] = cv;
-        }
+        // computeError (src/cache.cuh:910-961): r = z - cand
+        stream<8>(
+            n, c, [&](long k) { return V2<T>{Z[k], C[k]}; }, [&](long k, const V2<T> &v) { RR[k] = v.a - v.b; });
         __syncthreads();
+        clk.mark(PH_RES);
         bool haveDot = false;
         T dotRR = T(0);
         if (it % P.checkPeriod == 0) {
             T mx;
-            reduceEll(P, c, P.res, P.res, lds, dotRR, mx);
+            reduceEll<T, R, V>(P, c, RR, RR, lds, dotRR, mx);
             nEll++;
             haveDot = true;
+            clk.mark(PH_ELL);
             if (initFlag) {
                 tol = fmax(P.tolAbs, P.tolRel * mx);
                 initFlag = false;
@@ -879,90 +1129,113 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double tickRat
                 int code = ST_NOTRUN;
                 if (mx <= tol) code = ST_CONVERGED;
                 if (P.maxIters && it >= P.maxIters) code = ST_OUTOFITER;
-                if (P.maxTicks > 0 && it % 1000 == 0 && (double) (wall_clock64() - t0) >= P.maxTicks) code = ST_OUTOFTIME;
+                if (P.maxTicks > 0 && it % 1000 == 0 && (double) (wall_clock64() - t0) >= P.maxTicks)
+                    code = ST_OUTOFTIME;
                 if (code != ST_NOTRUN) {
                     status = code;
                     iters = it;
                 }
             }
         }
-        if (status != ST_NOTRUN) {  // acceptCandidate and stop
-            T *tmp = z;
-            z = cand;
-            cand = tmp;
-            break;
-        }
-        if (P.method == METHOD_CP) {  // runCp: accept every candidate (src/cache.cuh:967-1002)
-            T *tmp = z;
-            z = cand;
-            cand = tmp;
+        if (status != ST_NOTRUN || P.method == METHOD_CP) {  // acceptCandidate (and stop)
+            T *tmp = Z;
+            Z = C;
+            C = tmp;
+            if (status != ST_NOTRUN) break;
             continue;
+        }
+        {  // backup (src/cache.cuh:804-806): keep T(z) aside, continue in a fresh buffer
+            T *tmp = BK;
+            BK = C;
+            C = tmp;
         }
         // normM(r) (src/cache.cuh:740-742)
         if (!haveDot) {
             T mx;
-            reduceEll(P, c, P.res, P.res, lds, dotRR, mx);
+            reduceEll<T, R, V>(P, c, RR, RR, lds, dotRR, mx);
             nEll++;
+            clk.mark(PH_ELL);
         }
         w = sqrt(dotRR);
         wLast = w;
-        // delta iterate/residual, saveToPrev and the new Anderson column
+        // delta iterate/residual, new Anderson column, then saveToPrev as pointer moves
         head = (head + 1) % m;
+        andersonColumn<T>(P, c, head, RR, RP, Z, ZP, gram, G, gb, ldsd);
+        clk.mark(PH_ACOL);
         {
-            T *Yc = P.hY + (long) head * n, *Sc = P.hS + (long) head * n;
-            for (long k = c.tid; k < n; k += c.nt) {
-                const T rk = P.res[k], zk = z[k];
-                const T dr = rk - P.resPrev[k];
-                const T dz = zk - P.zPrev[k];
-                Yc[k] = dr;
-                Sc[k] = dz - dr;
-                P.zPrev[k] = zk;
-                P.resPrev[k] = rk;
+            T *tmp = ZP;
+            ZP = Z;
+            Z = tmp;
+            tmp = RP;
+            RP = RR;
+            RR = tmp;
+        }
+        // updateDirection (src/cache.cuh:861-904); r now lives in RP
+        double gs[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) gs[q] = 0.;
+        bool useLS = false;
+        if (it >= m) {
+            if (gram) {
+                useLS = gramSolve(G, gb, m, gs);
+            } else {
+                andersonHouseholder<T>(P, c, head, RP, lds, gs);
+                useLS = true;
             }
         }
-        __syncthreads();
-        andersonDirection(P, c, it, head, P.res, P.dir, lds);
-        if (w <= c0 * zeta) {  // K0 blind update (src/cache.cuh:1055-1061)
-            for (long k = c.tid; k < n; k += c.nt) cand[k] = P.zPrev[k] + P.dir[k];
-            __syncthreads();
+        clk.mark(PH_ALS);
+        const bool blind = w <= c0 * zeta;
+        andersonApply<T>(P, c, gs, useLS, RP, ZP, blind ? C : D, blind);
+        clk.mark(PH_AAPPLY);
+        if (blind) {  // K0 blind update (src/cache.cuh:1055-1061)
             zeta = w;
             k0++;
             continue;
         }
         T tau = T(1);
         for (int in = 0; in < P.maxInner; in++) {  // line search (src/cache.cuh:1063-1099)
-            for (long k = c.tid; k < n; k += c.nt) {
-                const T ds = P.dir[k] * tau;
-                P.dirS[k] = ds;
-                z[k] = P.zPrev[k] + ds;
-            }
+            stream<8>(
+                n, c, [&](long k) { return V2<T>{D[k], ZP[k]}; },
+                [&](long k, const V2<T> &v) {
+                    const T ds = v.a * tau;
+                    DS[k] = ds;
+                    Z[k] = v.b + ds;
+                });
             __syncthreads();
-            applyT(P, c, z, cand);
+            clk.mark(PH_LINE);
+            applyT<T, R, V>(P, c, Z, C, &clk);
             nT++;
-            for (long k = c.tid; k < n; k += c.nt) P.res[k] = z[k] - cand[k];
+            stream<8>(
+                n, c, [&](long k) { return V2<T>{Z[k], C[k]}; }, [&](long k, const V2<T> &v) { RR[k] = v.a - v.b; });
             __syncthreads();
+            clk.mark(PH_RES);
             T dot, mx;
-            reduceEll(P, c, P.res, P.res, lds, dot, mx);
+            reduceEll<T, R, V>(P, c, RR, RR, lds, dot, mx);
             nEll++;
+            clk.mark(PH_ELL);
             const T wTilde = sqrt(dot);
-            if (w <= wSafe && wTilde <= c1 * w) {  // K1
-                for (long k = c.tid; k < n; k += c.nt) cand[k] = z[k];
-                __syncthreads();
+            if (w <= wSafe && wTilde <= c1 * w) {  // K1: cand <- z
+                T *tmp = C;
+                C = Z;
+                Z = tmp;
                 wSafe = wTilde + pow(c2, T(it));
                 k1++;
                 break;
             }
             T dotRD;
-            reduceEll(P, c, P.res, P.dirS, lds, dotRD, mx);
+            reduceEll<T, R, V>(P, c, RR, DS, lds, dotRD, mx);
             nEll++;
+            clk.mark(PH_ELL);
             const T rho = wTilde * wTilde - dotRD;
             if (rho >= sigma * wTilde * w) {  // K2
                 const T f = lambda * rho / (wTilde * wTilde);
-                for (long k = c.tid; k < n; k += c.nt) {
-                    const T rk = P.res[k] * f;
-                    P.res[k] = rk;
-                    cand[k] = P.zPrev[k] - rk;
-                }
+                stream<8>(
+                    n, c, [&](long k) { return V2<T>{RR[k], ZP[k]}; },
+                    [&](long k, const V2<T> &v) {
+                        const T rk = v.a * f;
+                        RR[k] = rk;
+                        C[k] = v.b - rk;
+                    });
                 __syncthreads();
                 k2++;
                 break;
@@ -971,8 +1244,9 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double tickRat
                 k2bt++;
             }
             if (in >= P.maxInner - 1) {  // K3: restore the plain CP step
-                for (long k = c.tid; k < n; k += c.nt) cand[k] = P.backup[k];
-                __syncthreads();
+                T *tmp = C;
+                C = BK;
+                BK = tmp;
                 k3++;
             }
         }
@@ -981,9 +1255,16 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double tickRat
         status = ST_OUTOFITER;
         iters = it;
     }
-    // the accepted iterate must end in P.z
-    if (z != P.z) {
-        for (long k = c.tid; k < n; k += c.nt) P.z[k] = z[k];
+    // the accepted iterate must end in P.z and the last residual in P.res; both sources are
+    // read before either destination is written, so rotated roles cannot clobber each other
+    if (Z != P.z || RR != P.res) {
+        T *dz = P.z, *dr = P.res;
+        stream<8>(
+            n, c, [&](long k) { return V2<T>{Z[k], RR[k]}; },
+            [&](long k, const V2<T> &v) {
+                dz[k] = v.a;
+                dr[k] = v.b;
+            });
     }
     __syncthreads();
     if (c.tid == 0) {
@@ -999,40 +1280,105 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double tickRat
         out.k3 = k3;
         out.nT = nT;
         out.nEll = nEll;
+        clk.mark(PH_OTHER);
+        for (int i = 0; i < NUM_PHASES; i++) out.ticks[i] = clk.acc[i];
     }
 }
 
+// Copies the instance descriptor into LDS and, when they fit in the dynamic LDS budget,
+// its per-node index arrays and tree arrays, repointing the copy at the LDS arrays.
 template <typename T>
-__global__ void __launch_bounds__(1024) k_solve(const Inst<T> *__restrict__ insts, Out *outs, int count,
-                                                double tickRate) {
+__device__ const Inst<T> &stageInstance(const Inst<T> &G, Inst<T> &L, int *pool, int poolInts) {
+    const Tc c = threadCtx();
+    if (c.tid == 0) L = G;
+    __syncthreads();
+    if (poolInts <= 0) return L;
+    int used = 0;
+    auto take = [&](const int *&p, int count) {
+        if (!p || count <= 0 || used + count > poolInts) return;
+        int *dst = pool + used;
+        for (int i = c.tid; i < count; i += c.nt) dst[i] = p[i];
+        used += count;
+        __syncthreads();
+        if (c.tid == 0) p = dst;
+        __syncthreads();
+    };
+    const int N = G.N, nl = G.nl, nlf = G.nlf;
+    take(L.anc, N);
+    take(L.chFrom, nl);
+    take(L.numCh, nl);
+    take(L.stageFrom, G.S);
+    take(L.stageTo, G.S);
+    take(L.BTr.idx, N);
+    take(L.AB.idx, N);
+    take(L.ABKTr.idx, N);
+    take(L.APB.idx, N);
+    take(L.sQ.idx, N);
+    take(L.sR.idx, N);
+    take(L.trans.idx, N);
+    take(L.q.idx, N);
+    take(L.r.idx, N);
+    take(L.K.idx, nl);
+    take(L.KTr.idx, nl);
+    take(L.RtInv.idx, nl);
+    take(L.cholL.idx, nl);
+    take(L.cholU.idx, nl);
+    take(L.NN.idx, nl);
+    take(L.b.idx, nl);
+    take(L.nLB.idx, nl);
+    take(L.nUB.idx, nl);
+    take(L.nG.idx, nl);
+    take(L.nGT.idx, nl);
+    take(L.lsQ.idx, nlf);
+    take(L.lq.idx, nlf);
+    take(L.ltrans.idx, nlf);
+    take(L.lLB.idx, nlf);
+    take(L.lUB.idx, nlf);
+    take(L.lG.idx, nlf);
+    take(L.lGT.idx, nlf);
+    return L;
+}
+
+template <typename T, int R, int V>
+__global__ void __launch_bounds__(SPK_MAX_THREADS) k_solve(const Inst<T> *__restrict__ insts, Out *outs, int count,
+                                                double tickRate, int poolInts) {
     __shared__ T lds[8 * 32];
+    __shared__ double ldsd[16 * 32];
+    __shared__ __align__(16) char localBuf[sizeof(Inst<T>)];
+    Inst<T> &local = *reinterpret_cast<Inst<T> *>(localBuf);
+    extern __shared__ int pool[];
     const int k = blockIdx.x;
     if (k >= count) return;
-    solveInstance(insts[k], outs[k], lds, tickRate);
+    const Inst<T> &P = stageInstance(insts[k], local, pool, poolInts);
+    solveInstance<T, R, V>(P, outs[k], lds, ldsd, tickRate);
 }
 
 // ---------------------------------------------------------------- single-operator kernels
 // Debug entry points used by the parity tests: they read P.z and write P.cand.
 enum DebugOp { OP_L = 0, OP_LADJ = 1, OP_DYN = 2, OP_KERNEL = 3, OP_T = 4, OP_ELL = 5 };
 
-template <typename T>
-__global__ void __launch_bounds__(1024) k_debug(const Inst<T> *__restrict__ insts, Out *outs, int count, int op) {
+template <typename T, int R, int V>
+__global__ void __launch_bounds__(SPK_MAX_THREADS) k_debug(const Inst<T> *__restrict__ insts, Out *outs, int count, int op,
+                                                int poolInts) {
     __shared__ T lds[8 * 32];
+    __shared__ __align__(16) char localBuf[sizeof(Inst<T>)];
+    Inst<T> &local = *reinterpret_cast<Inst<T> *>(localBuf);
+    extern __shared__ int pool[];
     const int kInst = blockIdx.x;
     if (kInst >= count) return;
-    const Inst<T> &P = insts[kInst];
+    const Inst<T> &P = stageInstance(insts[kInst], local, pool, poolInts);
     const Tc c = threadCtx();
     T *z = P.z, *cand = P.cand;
     if (op == OP_L) {
-        StoreL<T> h{P, cand + P.sPrim, c.lane};
+        StoreL<T, R> h{P, cand + P.sPrim, c.lane};
         auto pv = [&](long k) { return z[k]; };
-        for (int n = c.wave; n < P.N; n += c.nw) opNode(P, n, c.lane, pv, h);
+        for (int n = c.wave; n < P.N; n += c.nw) opNode<T, R, V>(P, n, c.lane, pv, h);
     } else if (op == OP_LADJ) {
         const T *zd = z + P.sPrim;
         for (int n = c.wave; n < P.N; n += c.nw) {
-            AdjOut<T> o;
-            adjNode(P, zd, n, c.lane, o);
-            for (int rc = 0; rc < MAXR; rc++) {
+            AdjOut<T, R> o;
+            adjNode<T, R, V>(P, zd, n, c.lane, o);
+            for (int rc = 0; rc < R; rc++) {
                 const int r = c.lane + 64 * rc;
                 if (r < P.nx) cand[P.oX + (long) n * P.nx + r] = o.x[rc];
                 if (n < P.nl && r < P.nu) cand[P.oU + (long) n * P.nu + r] = o.u[rc];
@@ -1051,12 +1397,12 @@ __global__ void __launch_bounds__(1024) k_debug(const Inst<T> *__restrict__ inst
                 if (n >= P.nl)
                     for (int r = c.lane; r < P.nx; r += 64) P.qd[(long) n * P.nx + r] = -cand[P.oX + (long) n * P.nx + r];
         __syncthreads();
-        phaseProjections(P, c, cand, op == OP_DYN, op == OP_KERNEL);
+        phaseProjections<T, R, V>(P, c, cand, op == OP_DYN, op == OP_KERNEL);
     } else if (op == OP_T) {
-        applyT(P, c, z, cand);
+        applyT<T, R, V>(P, c, z, cand);
     } else if (op == OP_ELL) {
         T dot, mx;
-        reduceEll(P, c, z, cand, lds, dot, mx);
+        reduceEll<T, R, V>(P, c, z, cand, lds, dot, mx);
         if (c.tid == 0) {
             outs[kInst].w = (double) dot;
             outs[kInst].err = (double) mx;
@@ -1078,11 +1424,18 @@ __global__ void k_fill(T *p, T v, long n) {
     for (long k = blockIdx.x * (long) blockDim.x + threadIdx.x; k < n; k += (long) gridDim.x * blockDim.x) p[k] = v;
 }
 
-// explicit instantiations
-template __global__ void k_solve<double>(const Inst<double> *, Out *, int, double);
-template __global__ void k_solve<float>(const Inst<float> *, Out *, int, double);
-template __global__ void k_debug<double>(const Inst<double> *, Out *, int, int);
-template __global__ void k_debug<float>(const Inst<float> *, Out *, int, int);
+// explicit instantiations: register chunk classes (R, V) = (1,1) for dims <= 64,
+// and (4,4) for dims <= 256
+#define SPK_INST(T, R, V)                                                                  \
+    template __global__ void k_solve<T, R, V>(const Inst<T> *, Out *, int, double, int); \
+    template __global__ void k_debug<T, R, V>(const Inst<T> *, Out *, int, int, int);
+SPK_INST(double, 1, 1)
+#ifndef SPK_FAST_BUILD
+SPK_INST(double, 4, 4)
+SPK_INST(float, 1, 1)
+SPK_INST(float, 4, 4)
+#endif
+#undef SPK_INST
 template __global__ void k_fill<double>(double *, double, long);
 template __global__ void k_fill<float>(float *, float, long);
 

@@ -132,6 +132,38 @@ Table single(const std::vector<double> &m, int rows, int cols, int count) {
     return t;
 }
 
+// (L L')^-1 for every distinct lower Cholesky factor, in FP64: L^-1 by forward substitution,
+// then L^-T L^-1.  Used by the dynamics sweep as one matvec instead of two triangular solves.
+Table cholInverseTable(const Table &L) {
+    Table t;
+    t.rows = L.rows;
+    t.cols = L.cols;
+    t.index = L.index;
+    const int n = L.rows;
+    const int c = L.count();
+    t.data.assign((size_t) c * n * n, 0.);
+    std::vector<double> Li((size_t) n * n);
+    for (int k = 0; k < c; k++) {
+        const double *Lk = L.data.data() + (size_t) k * n * n;
+        std::fill(Li.begin(), Li.end(), 0.);
+        for (int col = 0; col < n; col++) {  // solve L x = e_col
+            for (int i = 0; i < n; i++) {
+                double v = (i == col) ? 1. : 0.;
+                for (int j = 0; j < i; j++) v -= Lk[i + (size_t) j * n] * Li[j + (size_t) col * n];
+                Li[i + (size_t) col * n] = v / Lk[i + (size_t) i * n];
+            }
+        }
+        double *out = t.data.data() + (size_t) k * n * n;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double v = 0.;
+                for (int r = 0; r < n; r++) v += Li[r + (size_t) i * n] * Li[r + (size_t) j * n];
+                out[i + (size_t) j * n] = v;
+            }
+    }
+    return t;
+}
+
 Table transposeTable(const Table &in) {
     Table t;
     t.rows = in.cols;
@@ -281,6 +313,7 @@ HostProblem loadProblem(const std::string &folderIn) {
     p.APB = stackTable("dynamics_APB", nx, nu, N);
     p.cholL = stackTable("dynamics_lowCholesky", nu, nu, nl);
     p.cholU = transposeTable(p.cholL);
+    p.RtInv = cholInverseTable(p.cholL);
     // risk (src/risks.cuh:26-33)
     p.NN = stackTable("risk_NNtr", p.nD, p.nD, nl);
     p.b = stackTable("risk_b", p.yD, 1, nl);

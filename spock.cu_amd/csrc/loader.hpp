@@ -39,7 +39,7 @@ struct HostProblem {
     std::vector<int> anc, chFrom, numCh, stageFrom, stageTo, events;
     std::vector<double> prob, condProb;
     // dynamics factors
-    Table BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, P;
+    Table BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, RtInv, P;  // RtInv = (L L')^-1
     // risk
     Table NN, b;
     // costs (nonleaf tables are indexed per node 0..N-1, leaf tables per leaf)

@@ -9,9 +9,10 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libspock_amd.so")
+LIB_PATH = os.environ.get("SPK_LIB") or os.path.join(HERE, "lib", "libspock_amd.so")
 
 F64, F32 = 0, 1
+LS_HOUSEHOLDER, LS_GRAM = 0, 1
 SPOCK, CP = 0, 1
 OP_L, OP_LADJ, OP_DYN, OP_KERNEL, OP_T, OP_ELL = range(6)
 STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
@@ -29,16 +30,23 @@ EXPORTS = [
 class Options(C.Structure):
     _fields_ = [("tol_abs", C.c_double), ("tol_rel", C.c_double), ("max_time_secs", C.c_double),
                 ("max_iters", C.c_long), ("max_inner", C.c_int), ("anderson_buffer", C.c_int),
-                ("check_period", C.c_int), ("method", C.c_int), ("block_size", C.c_int)]
+                ("check_period", C.c_int), ("method", C.c_int), ("block_size", C.c_int),
+                ("ls_method", C.c_int), ("dyn_inverse", C.c_int)]
 
 
 class Result(C.Structure):
     _fields_ = [("status", C.c_int), ("iters", C.c_long), ("err", C.c_double), ("tol", C.c_double),
                 ("w", C.c_double), ("k0", C.c_long), ("k1", C.c_long), ("k2", C.c_long), ("k2bt", C.c_long),
-                ("k3", C.c_long), ("num_T", C.c_long), ("num_ell", C.c_long)]
+                ("k3", C.c_long), ("num_T", C.c_long), ("num_ell", C.c_long),
+                ("phase_seconds", C.c_double * 10)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["phase_seconds"] = dict(zip(PHASES, list(self.phase_seconds)))
+        return d
+
+
+PHASES = ["adj", "proj", "dual", "res", "ell", "acol", "als", "aapply", "line", "other"]
 
 
 _dp = C.POINTER(C.c_double)

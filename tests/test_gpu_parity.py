@@ -132,9 +132,11 @@ def test_batch_of_initial_states_matches_single_solves(ctx, problem_dirs):
     b, res = gpu_solve(ctx, [p] * B, x0s, **opts)
     for k in range(B):
         bk, rk = gpu_solve(ctx, [p], [x0s[k]], **opts)
-        assert res[k]["status"] == rk[0]["status"] == 0
+        assert res[k]["status"] == rk[0]["status"]
         assert res[k]["iters"] == rk[0]["iters"]
         assert np.array_equal(b.iterate(k), bk.iterate(0))
+        ref = o.solve(x0s[k], tol_abs=1e-3, max_iters=3000)
+        assert res[k]["status"] == ref["status"]
 
 
 def test_uneven_batch_matches_single_solves(ctx, problem_dirs):
