@@ -99,19 +99,38 @@ int spk_problem_scaling(const spk_problem *p, double *out);
 double spk_problem_step_size(const spk_problem *p);
 
 int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int dtype, spk_batch **out);
+/* `count` instances of ONE problem (e.g. many initial states of one tree) in the
+ * block-interleaved layout of the lane engine: instances form blocks of 64 and element e of
+ * instance k is stored at ((k / 64) * n + e) * 64 + k % 64 (n = size_iterate); a wave's 64
+ * lanes are the 64 instances of one block and the factor tables are read once per wave.  Every spk_batch_* call below accepts such a batch; the
+ * Anderson least squares is the FP64 Gram solve (ls_method is ignored), the dynamics sweep
+ * applies the precomputed (L L')^-1 (dyn_inverse is ignored), anderson_buffer <= 4,
+ * numInputs <= 32, numEvents <= 7.  block_size is the workgroup size (64..1024, default
+ * 1024 when <= 0); one workgroup serves 64 instances.  No counterpart in the reference, which
+ * solves one instance per Cache. */
+int spk_batch_create_shared(spk_ctx *ctx, spk_problem *problem, int count, int dtype, spk_batch **out);
+int spk_batch_is_shared(const spk_batch *b);
 int spk_batch_destroy(spk_batch *b);
 int spk_batch_count(const spk_batch *b);
-/* zero every iterate and workspace (Cache::reset) */
+/* zero every iterate and workspace (Cache::reset, src/cache.cuh:436-479); the initial states set by
+ * spk_batch_set_x0 are inputs and stay, where the reference re-uploads them on every runSpock */
 int spk_batch_reset(spk_batch *b);
 /* initial states, unscaled, concatenated over instances (nx_k values each) */
 int spk_batch_set_x0(spk_batch *b, const double *x0);
+/* per-instance acceptance tolerances in the solver's scaled units (spk_result.tol of an earlier
+ * solve): when set (> 0), a solve tests convergence from its first error check on against this
+ * value instead of calibrating max(tol_abs, tol_rel * err0) there (src/cache.cuh:932-936).  Used for
+ * warm starts and FP64 polish so they accept against the cold solve's threshold.  tol == NULL
+ * clears; 0 entries calibrate as the reference. */
+int spk_batch_set_tolerances(spk_batch *b, const double *tol);
 /* host iterate (size_iterate_k values) of instance k, used as the warm start of the next solve */
 int spk_batch_set_iterate(spk_batch *b, int k, const double *z);
 int spk_batch_get_iterate(spk_batch *b, int k, double *z);
 /* all iterates at once: host arrays concatenated over instances */
 int spk_batch_set_iterates(spk_batch *b, const double *z);
 int spk_batch_get_iterates(spk_batch *b, double *z);
-/* device pointer to instance k's accepted iterate (dtype of the batch) */
+/* device pointer to instance k's accepted iterate (dtype of the batch); contiguous for
+ * per-instance batches, stride 64 elements for shared batches */
 void *spk_batch_iterate_ptr(spk_batch *b, int k);
 /* copy iterates between batches of the same problems (converting dtype) on the device */
 int spk_batch_copy_iterates(spk_batch *dst, const spk_batch *src);

@@ -45,6 +45,8 @@ def _lib():
     lib.orc_dotM.restype = C.c_double
     lib.orc_dotM.argtypes = [C.c_void_p, _dp, _dp]
     lib.orc_lstsq.argtypes = [_dp, C.c_long, C.c_int, _dp, _dp]
+    lib.orc_lstsq_gram.restype = C.c_int
+    lib.orc_lstsq_gram.argtypes = [_dp, C.c_long, C.c_int, _dp, _dp]
     lib.orc_solve.restype = C.c_int
     lib.orc_solve.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _lp, _dp, _lp, _dp, _ip, _dp, C.c_long]
     return lib
@@ -147,13 +149,13 @@ class Oracle:
         return lib().orc_dotM(self.h, _d(x), _d(y))
 
     def solve(self, x0, method="spock", warm=None, tol_abs=1e-3, tol_rel=0.0, max_time=0.0, max_iters=0,
-              max_inner=8, anderson=3, check_period=25, trace_len=0):
+              max_inner=8, anderson=3, check_period=25, trace_len=0, ls="householder", tol_fixed=0.0):
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         z = np.zeros(self.n_iter)
         out = np.zeros(7, dtype=np.int64)
         outd = np.zeros(2)
-        dopts = np.array([tol_abs, tol_rel, max_time])
-        iopts = np.array([max_iters, max_inner, anderson, check_period], dtype=np.int64)
+        dopts = np.array([tol_abs, tol_rel, max_time, tol_fixed])
+        iopts = np.array([max_iters, max_inner, anderson, check_period, 1 if ls == "gram" else 0], dtype=np.int64)
         wp = _d(np.ascontiguousarray(warm, dtype=np.float64)) if warm is not None else None
         tb = np.zeros(trace_len, dtype=np.int32) if trace_len else None
         tw = np.zeros(trace_len) if trace_len else None
@@ -174,6 +176,15 @@ def lstsq(Y, r):
     g = np.zeros(Y.shape[1])
     lib().orc_lstsq(_d(Y), Y.shape[0], Y.shape[1], _d(r), _d(g))
     return g
+
+
+def lstsq_gram(Y, r):
+    """Normal-equation least squares as the GPU engines' Gram option; None when it breaks down."""
+    Y = np.asfortranarray(Y, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    g = np.zeros(Y.shape[1])
+    ok = lib().orc_lstsq_gram(_d(Y), Y.shape[0], Y.shape[1], _d(r), _d(g))
+    return g if ok else None
 
 
 def soc(v):

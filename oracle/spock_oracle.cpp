@@ -537,10 +537,55 @@ bool lstsqHouseholder(const double *Yin, size_t n, int m, const double *rin, dou
     return true;
 }
 
+// argmin ||Y g - r|| from the normal equations: Cholesky of G = Y'Y + 1e-12 max(diag G) I.  The
+// GPU engines' alternative to the Householder solve (their Gram entries are accumulated as the
+// columns arrive); returns false when the factorisation breaks down, and the direction is then -r.
+bool lstsqGram(const double *Y, size_t n, int m, const double *r, double *g) {
+    std::vector<double> G((size_t) m * m, 0.), b(m, 0.), L((size_t) m * m, 0.), y(m, 0.);
+    for (int i = 0; i < m; i++) {
+        for (int j = 0; j <= i; j++) {
+            double acc = 0.;
+            for (size_t k = 0; k < n; k++) acc += Y[(size_t) i * n + k] * Y[(size_t) j * n + k];
+            G[(size_t) i * m + j] = G[(size_t) j * m + i] = acc;
+        }
+        double acc = 0.;
+        for (size_t k = 0; k < n; k++) acc += Y[(size_t) i * n + k] * r[k];
+        b[i] = acc;
+    }
+    double dmax = 0.;
+    for (int i = 0; i < m; i++) dmax = std::max(dmax, G[(size_t) i * m + i]);
+    if (!(dmax > 0.)) return false;
+    const double ridge = 1e-12 * dmax;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j <= i; j++) {
+            double v = G[(size_t) i * m + j] + (i == j ? ridge : 0.);
+            for (int k = 0; k < j; k++) v -= L[(size_t) i * m + k] * L[(size_t) j * m + k];
+            if (i == j) {
+                if (!(v > 0.)) return false;
+                L[(size_t) i * m + i] = std::sqrt(v);
+            } else {
+                L[(size_t) i * m + j] = v / L[(size_t) j * m + j];
+            }
+        }
+    for (int i = 0; i < m; i++) {
+        double v = b[i];
+        for (int k = 0; k < i; k++) v -= L[(size_t) i * m + k] * y[k];
+        y[i] = v / L[(size_t) i * m + i];
+    }
+    for (int i = m - 1; i >= 0; i--) {
+        double v = y[i];
+        for (int k = i + 1; k < m; k++) v -= L[(size_t) k * m + i] * g[k];
+        g[i] = v / L[(size_t) i * m + i];
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------ solver
 struct Opts {
     double tolAbs, tolRel, maxTime;
     long maxIters, maxInner, andBuff, checkPeriod;
+    long lsGram = 0;        // 1: Gram/Cholesky least squares instead of Householder
+    double tolFixed = 0.;   // > 0: accept against this tolerance from the first check on
 };
 
 struct Trace {  // per outer iteration: branch taken and the residual norm
@@ -584,6 +629,10 @@ struct Solver {
         ell(p, r.data(), ellR.data());
         double mx = 0.;
         for (size_t k = 0; k < n; k++) mx = std::max(mx, std::fabs(r[k] * p.gammaRecip - ellR[k]));
+        if (initFlag && o.tolFixed > 0.) {
+            tol = o.tolFixed;
+            initFlag = false;
+        }
         if (initFlag) {
             tol = std::max(tolAbs, tolRel * mx);
             initFlag = false;
@@ -615,9 +664,10 @@ struct Solver {
             Y[k] = dr;
             S[k] = dz - dr;
         }
-        if (idx >= m) {
-            std::vector<double> g(m);
-            lstsqHouseholder(Y.data(), n, m, r.data(), g.data());
+        std::vector<double> g(m);
+        const bool useLS = idx >= m && (o.lsGram ? lstsqGram(Y.data(), n, m, r.data(), g.data())
+                                                 : lstsqHouseholder(Y.data(), n, m, r.data(), g.data()));
+        if (useLS) {
             for (size_t k = 0; k < n; k++) {
                 double sg = 0.;
                 for (int c = 0; c < m; c++) sg += S[(size_t) c * n + k] * g[c];
@@ -769,14 +819,19 @@ double orc_dotM(void *h, const double *x, const double *y) { return dotM(*static
 
 void orc_lstsq(const double *Y, long n, int m, const double *r, double *g) { lstsqHouseholder(Y, (size_t) n, m, r, g); }
 
-// method 0 = SPOCK (runSpock), 1 = CP (runCp).  dopts = {tolAbs, tolRel, maxTime};
-// iopts = {maxIters, maxInner, andBuff, checkPeriod}.  out = {iters, status, k0, k1, k2, k2bt, k3}
+int orc_lstsq_gram(const double *Y, long n, int m, const double *r, double *g) {
+    return lstsqGram(Y, (size_t) n, m, r, g) ? 1 : 0;
+}
+
+// method 0 = SPOCK (runSpock), 1 = CP (runCp).  dopts = {tolAbs, tolRel, maxTime, tolFixed};
+// iopts = {maxIters, maxInner, andBuff, checkPeriod, lsGram}.  out = {iters, status, k0, k1, k2, k2bt, k3}
 // outD = {lastErr, tol}.  z_out receives the final iterate; trace arrays (nullable) get per-iteration
 // branch codes and residual norms up to traceLen entries.
 int orc_solve(void *h, int method, const double *x0, const double *warm, const double *dopts, const long *iopts,
               double *zOut, long *out, double *outD, int *traceBranch, double *traceW, long traceLen) {
     auto &p = *static_cast<Problem *>(h);
-    Opts o{dopts[0], dopts[1], dopts[2], iopts[0], iopts[1], iopts[2], iopts[3] > 0 ? iopts[3] : 25};
+    Opts o{dopts[0], dopts[1], dopts[2], iopts[0], iopts[1], iopts[2], iopts[3] > 0 ? iopts[3] : 25, iopts[4],
+           dopts[3]};
     Solver s(p, o);
     bool tr = traceBranch || traceW;
     int st = method == 0 ? s.runSpock(x0, warm, tr) : s.runCp(x0, warm);

@@ -24,6 +24,14 @@ __global__ void k_f32_to_f64(const float *, double *, long);
 __global__ void k_f64_to_f32(const double *, float *, long);
 template <typename T>
 __global__ void k_fill(T *, T, long);
+template <typename T>
+__global__ void k_lane_solve(const LaneInst<T> *, Out *);
+template <typename T>
+__global__ void k_lane_debug(const LaneInst<T> *, Out *, int);
+template <typename S, typename D>
+__global__ void k_interleave(const S *, long, D *, long, int, long);
+template <typename S, typename D>
+__global__ void k_deinterleave(const S *, long, D *, long, long, int);
 }  // namespace spk
 
 using spk::HostProblem;
@@ -88,11 +96,18 @@ struct spk_batch {
     int dtype = SPK_F64;
     int count = 0;
     std::vector<spk_problem *> probs;
-    std::vector<size_t> coreOff, andOff;  // element offsets of each instance's regions
-    size_t coreElems = 0, andElems = 0;
+    std::vector<size_t> coreOff, andOff, x0Off;  // element offsets of each instance's regions
+    size_t coreElems = 0, andElems = 0, x0Elems = 0;
     int mAlloc = 0;
-    DevBuf core, anders, dInst, dOut;
+    DevBuf core, anders, x0buf, dInst, dOut;  // x0buf: scaled initial states (inputs, kept by reset)
     std::vector<char> hInst;  // array of Inst<T>
+    std::vector<double> tolFixed;  // per-instance given tolerance (0 = calibrate at the first check)
+    // shared-problem batch in the instance-interleaved layout (lane engine)
+    bool lanes = false;
+    long Bpad = 0;     // count rounded up to 64
+    DevBuf lbuf, ltol; // interleaved state; per-instance given tolerances
+    size_t lRegion[16] = {};  // element offsets: work[0..8], zio, rio, hS, hY, qd, dd, x0
+    static constexpr int LANE_MAXM = 4;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool pending = false;
     size_t elSize() const { return dtype == SPK_F64 ? 8 : 4; }
@@ -118,9 +133,11 @@ void uploadTable(spk_problem *p, const std::string &name, const Table &t) {
     s.rows = t.rows;
     s.cols = t.cols;
     std::vector<T> d(t.data.begin(), t.data.end());
+    // 64 zero elements of padding: the lane engine reads fixed row blocks past a table's end
+    d.resize(d.size() + 64, T(0));
     s.data = std::make_unique<DevBuf>();
     s.data->alloc(d.size() * sizeof(T));
-    if (!d.empty()) check(hipMemcpy(s.data->p, d.data(), d.size() * sizeof(T), hipMemcpyHostToDevice), "upload table");
+    check(hipMemcpy(s.data->p, d.data(), d.size() * sizeof(T), hipMemcpyHostToDevice), "upload table");
     s.idx = std::make_unique<DevBuf>();
     s.idx->alloc(t.index.size() * sizeof(int));
     if (!t.index.empty())
@@ -160,9 +177,35 @@ size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Per-instance element counts of the two state regions.
 size_t coreCount(const HostProblem &h) {
-    return alignUp(8 * (size_t) h.sIter + (size_t) h.nx * h.N + (size_t) h.nu * h.nl + (size_t) h.nx, 64);
+    return alignUp(8 * (size_t) h.sIter + (size_t) h.nx * h.N + (size_t) h.nu * h.nl, 64);
 }
 size_t andCount(const HostProblem &h, int m) { return alignUp((size_t) (3 * m + 1) * h.sIter, 64); }
+
+// Problem sizes, offsets, tree arrays and device tables shared by both instance descriptors.
+template <typename T, typename I>
+void fillProblem(I &I_, spk_problem *p) {
+    const HostProblem &h = p->hp;
+    I_.N = h.N; I_.nl = h.nl; I_.nlf = h.nlf; I_.S = h.S; I_.E = h.E; I_.nx = h.nx; I_.nu = h.nu;
+    I_.yD = h.yD; I_.nD = h.nD; I_.ncost = h.ncost; I_.lcost = h.lcost; I_.ncon = h.ncon; I_.lcon = h.lcon;
+    I_.con3 = h.con3; I_.con5 = h.con5;
+    I_.oU = 0; I_.oX = h.sU; I_.oY = I_.oX + h.sX; I_.oT = I_.oY + h.sY; I_.oS = I_.oT + h.sT; I_.sPrim = h.sPrim;
+    I_.oI = 0; I_.oII = h.sI; I_.oIII = I_.oII + h.sII; I_.oIV = I_.oIII + h.sIII; I_.oV = I_.oIV + h.sIV;
+    I_.oVI = I_.oV + h.sV; I_.sDual = h.sDual; I_.sIter = h.sIter;
+    I_.gamma = (T) h.stepSize;
+    I_.gammaRecip = (T) (1. / (double) I_.gamma);
+    I_.anc = static_cast<const int *>(p->anc.p);
+    I_.chFrom = static_cast<const int *>(p->chFrom.p);
+    I_.numCh = static_cast<const int *>(p->numCh.p);
+    I_.stageFrom = static_cast<const int *>(p->stageFrom.p);
+    I_.stageTo = static_cast<const int *>(p->stageTo.p);
+#define DT(name) I_.name = dtab<T>(p, #name);
+    SPK_TABLES(DT)
+#undef DT
+}
+
+double tolScaleOf(const HostProblem &h) {
+    return h.precond ? *std::min_element(h.scaling.begin(), h.scaling.end()) : 1.;
+}
 
 template <typename T>
 void buildInsts(spk_batch *b, const spk_options &o, int warm) {
@@ -196,7 +239,7 @@ void buildInsts(spk_batch *b, const spk_options &o, int warm) {
         I.resPrev = c + 5 * n; I.dir = c + 6 * n; I.dirS = c + 7 * n;
         I.qd = c + 8 * n;
         I.dd = I.qd + (size_t) h.nx * h.N;
-        I.x0s = I.dd + (size_t) h.nu * h.nl;
+        I.x0s = static_cast<const T *>(b->x0buf.p) + b->x0Off[k];
         T *a = and_ + b->andOff[k];
         const int m = o.anderson_buffer;
         I.hS = a; I.hY = a + (size_t) m * n; I.W = a + (size_t) 2 * m * n;
@@ -204,6 +247,7 @@ void buildInsts(spk_batch *b, const spk_options &o, int warm) {
         if (h.precond) tolScale = *std::min_element(h.scaling.begin(), h.scaling.end());
         I.tolAbs = (T) (o.tol_abs * tolScale);
         I.tolRel = (T) (o.tol_rel * tolScale);
+        I.tolFixed = b->tolFixed.empty() ? T(0) : (T) b->tolFixed[k];
         I.maxIters = o.max_iters;
         I.maxInner = o.max_inner;
         I.andBuff = m;
@@ -316,6 +360,157 @@ void launchSolve(spk_batch *b, const spk_options &o, int warm) {
     check(hipGetLastError(), "launch k_solve");
     check(hipEventRecord(b->ev1, b->ctx->stream), "event");
     b->pending = true;
+}
+
+
+// ---------------------------------------------------------------- shared-problem (lane) batches
+// Regions of the interleaved state, each (rows x Bpad): work[0..8] and zio, rio (sIter rows),
+// the Anderson rings hS, hY (LANE_MAXM slots of sIter rows), qd (nx*N), dd (nu*nl), x0 (nx).
+enum LaneRegion { LR_ZIO = 9, LR_RIO = 10, LR_HS = 11, LR_HY = 12, LR_QD = 13, LR_DD = 14, LR_X0 = 15, LR_END = 16 };
+
+size_t laneLayout(spk_batch *b) {
+    const HostProblem &h = b->probs[0]->hp;
+    const size_t n = h.sIter, B = b->Bpad, m = spk_batch::LANE_MAXM;
+    size_t off = 0;
+    for (int r = 0; r <= LR_RIO; r++) {
+        b->lRegion[r] = off;
+        off += n * B;
+    }
+    b->lRegion[LR_HS] = off;
+    off += m * n * B;
+    b->lRegion[LR_HY] = off;
+    off += m * n * B;
+    b->lRegion[LR_QD] = off;
+    off += (size_t) h.nx * h.N * B;
+    b->lRegion[LR_DD] = off;
+    off += (size_t) h.nu * h.nl * B;
+    b->lRegion[LR_X0] = off;
+    off += (size_t) h.nx * B;
+    return off;
+}
+
+template <typename T>
+T *lanePtr(const spk_batch *b, int region) {
+    return static_cast<T *>(b->lbuf.p) + b->lRegion[region];
+}
+
+void validateLanes(const HostProblem &h, int) {
+    // d = R~^-1 e and the kernel projection are computed in place from up to 32 register rows
+    if (h.nu > 32) throw std::runtime_error("shared batches support numInputs <= 32");
+    if (h.nD > 32) throw std::runtime_error("shared batches support numEvents <= 7");
+}
+
+template <typename T>
+void buildLaneInst(spk_batch *b, const spk_options &o, int warm) {
+    spk::LaneInst<T> L;
+    std::memset(static_cast<void *>(&L), 0, sizeof(L));
+    spk_problem *p = b->probs[0];
+    const HostProblem &h = p->hp;
+    fillProblem<T>(L, p);
+    L.B = b->Bpad;
+    L.count = b->count;
+    for (int r = 0; r < 9; r++) L.work[r] = lanePtr<T>(b, r);
+    L.zio = lanePtr<T>(b, LR_ZIO);
+    L.rio = lanePtr<T>(b, LR_RIO);
+    L.hS = lanePtr<T>(b, LR_HS);
+    L.hY = lanePtr<T>(b, LR_HY);
+    L.qd = lanePtr<T>(b, LR_QD);
+    L.dd = lanePtr<T>(b, LR_DD);
+    L.x0s = lanePtr<T>(b, LR_X0);
+    L.tolFixed = b->tolFixed.empty() ? nullptr : static_cast<const T *>(b->ltol.p);
+    const double tolScale = tolScaleOf(h);
+    L.tolAbs = (T) (o.tol_abs * tolScale);
+    L.tolRel = (T) (o.tol_rel * tolScale);
+    L.maxIters = o.max_iters;
+    L.maxInner = o.max_inner;
+    L.andBuff = o.anderson_buffer;
+    L.checkPeriod = o.check_period > 0 ? o.check_period : 25;
+    L.method = o.method;
+    L.warm = warm;
+    L.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
+    b->hInst.assign(sizeof(L), 0);
+    std::memcpy(b->hInst.data(), &L, sizeof(L));
+    b->dInst.alloc(sizeof(L));
+    check(hipMemcpyAsync(b->dInst.p, b->hInst.data(), sizeof(L), hipMemcpyHostToDevice, b->ctx->stream),
+          "upload lane instance");
+}
+
+int laneBlock(const spk_options &o) {
+    const int bs = o.block_size > 0 ? o.block_size : 1024;
+    return std::max(64, std::min(1024, bs / 64 * 64));
+}
+
+template <typename T>
+void launchLaneSolve(spk_batch *b, const spk_options &o, int warm) {
+    buildLaneInst<T>(b, o, warm);
+    b->dOut.alloc(sizeof(spk::Out) * b->count);
+    check(hipEventRecord(b->ev0, b->ctx->stream), "event");
+    hipLaunchKernelGGL(spk::k_lane_solve<T>, dim3((unsigned) (b->Bpad / 64)), dim3(laneBlock(o)), 0, b->ctx->stream,
+                       static_cast<const spk::LaneInst<T> *>(b->dInst.p), static_cast<spk::Out *>(b->dOut.p));
+    check(hipGetLastError(), "launch k_lane_solve");
+    check(hipEventRecord(b->ev1, b->ctx->stream), "event");
+    b->pending = true;
+}
+
+// Host rows (instance-major, n values each) <-> one interleaved region.
+template <typename T>
+void laneUpload(spk_batch *b, int region, const double *rows, size_t n) {
+    std::vector<T> tmp(n * b->Bpad, T(0));
+    for (int k = 0; k < b->count; k++)
+        for (size_t e = 0; e < n; e++) tmp[spk::laneIndex((long) n, (long) e, k)] = (T) rows[(size_t) k * n + e];
+    check(hipMemcpy(lanePtr<T>(b, region), tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+}
+
+template <typename T>
+void laneDownload(spk_batch *b, int region, double *rows, size_t n) {
+    std::vector<T> tmp(n * b->Bpad);
+    check(hipMemcpy(tmp.data(), lanePtr<T>(b, region), tmp.size() * sizeof(T), hipMemcpyDeviceToHost), "download");
+    for (int k = 0; k < b->count; k++)
+        for (size_t e = 0; e < n; e++) rows[(size_t) k * n + e] = (double) tmp[spk::laneIndex((long) n, (long) e, k)];
+}
+
+template <typename T>
+void laneColumn(spk_batch *b, int region, int k, double *z, const double *zin, size_t n) {
+    const size_t es = sizeof(T), pitch = 64 * es;
+    T *base = lanePtr<T>(b, region) + spk::laneIndex((long) n, 0, k);
+    std::vector<T> tmp(n);
+    if (zin) {
+        for (size_t e = 0; e < n; e++) tmp[e] = (T) zin[e];
+        check(hipMemcpy2D(base, pitch, tmp.data(), es, es, n, hipMemcpyHostToDevice), "set column");
+    } else {
+        check(hipMemcpy2D(tmp.data(), es, base, pitch, es, n, hipMemcpyDeviceToHost), "get column");
+        for (size_t e = 0; e < n; e++) z[e] = (double) tmp[e];
+    }
+}
+
+// Uniform row stride of a per-instance batch whose instances all share one problem's sizes.
+size_t uniformCoreStride(const spk_batch *b) {
+    const size_t stride = b->count > 1 ? b->coreOff[1] - b->coreOff[0] : coreCount(b->probs[0]->hp);
+    for (int k = 0; k < b->count; k++)
+        if (b->coreOff[k] != (size_t) k * stride || b->probs[k]->hp.sIter != b->probs[0]->hp.sIter)
+            throw std::runtime_error("copy between layouts needs instances of one problem size");
+    return stride;
+}
+
+template <typename S, typename D>
+void launchInterleave(const spk_batch *src, spk_batch *dst, size_t n) {
+    const size_t stride = uniformCoreStride(src);
+    hipLaunchKernelGGL((spk::k_interleave<S, D>), dim3(1024), dim3(256), 0, dst->ctx->stream,
+                       static_cast<const S *>(src->core.p), (long) stride, lanePtr<D>(dst, LR_ZIO), (long) n,
+                       dst->count, dst->Bpad);
+}
+
+template <typename S, typename D>
+void launchDeinterleave(const spk_batch *src, spk_batch *dst, size_t n) {
+    const size_t stride = uniformCoreStride(dst);
+    hipLaunchKernelGGL((spk::k_deinterleave<S, D>), dim3(1024), dim3(256), 0, dst->ctx->stream,
+                       lanePtr<S>(src, LR_ZIO), src->Bpad, static_cast<D *>(dst->core.p), (long) stride, (long) n,
+                       dst->count);
+}
+
+template <typename T>
+void fillDevice(spk_batch *b, T *p, size_t n) {
+    hipLaunchKernelGGL(spk::k_fill<T>, dim3(1024), dim3(256), 0, b->ctx->stream, p, T(0), (long) n);
 }
 
 }  // namespace
@@ -442,6 +637,15 @@ int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int 
         b->coreElems = off;
         b->core.alloc(off * b->elSize());
         check(hipMemset(b->core.p, 0, off * b->elSize()), "zero state");
+        b->x0Off.resize(count);
+        off = 0;
+        for (int k = 0; k < count; k++) {
+            b->x0Off[k] = off;
+            off += b->probs[k]->hp.nx;
+        }
+        b->x0Elems = off;
+        b->x0buf.alloc(off * b->elSize());
+        check(hipMemset(b->x0buf.p, 0, off * b->elSize()), "zero x0");
         check(hipEventCreate(&b->ev0), "event");
         check(hipEventCreate(&b->ev1), "event");
         ensureAnderson(b.get(), 3);
@@ -452,6 +656,36 @@ int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int 
         return fail(ctx, e);
     }
 }
+
+int spk_batch_create_shared(spk_ctx *ctx, spk_problem *problem, int count, int dtype, spk_batch **out) {
+    try {
+        if (count <= 0) throw std::runtime_error("batch count must be positive");
+        if (!problem) throw std::runtime_error("null problem");
+        if (dtype != SPK_F64 && dtype != SPK_F32) throw std::runtime_error("dtype must be SPK_F64 or SPK_F32");
+        validateLanes(problem->hp, dtype);
+        auto b = std::make_unique<spk_batch>();
+        b->ctx = ctx;
+        b->dtype = dtype;
+        b->count = count;
+        b->lanes = true;
+        b->Bpad = ((long) count + 63) / 64 * 64;
+        b->probs.assign(count, problem);
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (dtype == SPK_F64) ensureUploaded<double>(problem);
+        else ensureUploaded<float>(problem);
+        const size_t total = laneLayout(b.get());
+        b->lbuf.alloc(total * b->elSize());
+        check(hipMemset(b->lbuf.p, 0, total * b->elSize()), "zero state");
+        check(hipEventCreate(&b->ev0), "event");
+        check(hipEventCreate(&b->ev1), "event");
+        *out = b.release();
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(ctx, e);
+    }
+}
+
+int spk_batch_is_shared(const spk_batch *b) { return b && b->lanes ? 1 : 0; }
 
 int spk_batch_destroy(spk_batch *b) {
     if (!b) return 0;
@@ -467,9 +701,14 @@ int spk_batch_count(const spk_batch *b) { return b->count; }
 int spk_batch_reset(spk_batch *b) {
     try {
         check(hipStreamSynchronize(b->ctx->stream), "sync");
+        if (b->lanes) {  // everything before the initial states
+            check(hipMemset(b->lbuf.p, 0, b->lRegion[LR_X0] * b->elSize()), "reset");
+            return 0;
+        }
         check(hipMemset(b->core.p, 0, b->coreElems * b->elSize()), "reset");
         if (b->anders.p) check(hipMemset(b->anders.p, 0, b->andElems * b->elSize()), "reset");
-        // x0 is part of the core region; the reference's reset also zeroes the initial state
+        // the initial states are inputs of the next solve (spk_batch_set_x0) and stay; the
+        // reference zeroes its copy because runSpock(x0) uploads x0 again (src/cache.cuh:448,1011)
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);
@@ -479,16 +718,46 @@ int spk_batch_reset(spk_batch *b) {
 int spk_batch_set_x0(spk_batch *b, const double *x0) {
     try {
         check(hipStreamSynchronize(b->ctx->stream), "sync");
-        size_t pos = 0;
+        if (b->lanes) {
+            const HostProblem &h = b->probs[0]->hp;
+            std::vector<double> s(x0, x0 + (size_t) b->count * h.nx);
+            if (h.precond)
+                for (int k = 0; k < b->count; k++)
+                    for (int i = 0; i < h.nx; i++) s[(size_t) k * h.nx + i] *= h.scaling[i];  // src/cache.cuh:599-601
+            if (b->dtype == SPK_F64) laneUpload<double>(b, LR_X0, s.data(), h.nx);
+            else laneUpload<float>(b, LR_X0, s.data(), h.nx);
+            return 0;
+        }
+        std::vector<double> s(x0, x0 + b->x0Elems);
         for (int k = 0; k < b->count; k++) {
             const HostProblem &h = b->probs[k]->hp;
-            std::vector<double> s(x0 + pos, x0 + pos + h.nx);
-            pos += h.nx;
             if (h.precond)
-                for (int i = 0; i < h.nx; i++) s[i] = s[i] * h.scaling[i];  // src/cache.cuh:599-601
-            const size_t o = 8 * (size_t) h.sIter + (size_t) h.nx * h.N + (size_t) h.nu * h.nl;
-            if (b->dtype == SPK_F64) toDevice<double>(b, corePtr<double>(b, k) + o, s.data(), h.nx);
-            else toDevice<float>(b, corePtr<float>(b, k) + o, s.data(), h.nx);
+                for (int i = 0; i < h.nx; i++) s[b->x0Off[k] + i] *= h.scaling[i];  // src/cache.cuh:599-601
+        }
+        if (b->dtype == SPK_F64) toDevice<double>(b, static_cast<double *>(b->x0buf.p), s.data(), s.size());
+        else toDevice<float>(b, static_cast<float *>(b->x0buf.p), s.data(), s.size());
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_set_tolerances(spk_batch *b, const double *tol) {
+    try {
+        if (!tol) {
+            b->tolFixed.clear();
+        } else {
+            b->tolFixed.assign(tol, tol + b->count);
+            for (double t : b->tolFixed)
+                if (!(t >= 0.)) throw std::runtime_error("tolerances must be >= 0");
+            if (b->lanes) {
+                check(hipStreamSynchronize(b->ctx->stream), "sync");
+                b->ltol.alloc(b->Bpad * b->elSize());
+                std::vector<double> t(b->tolFixed);
+                t.resize(b->Bpad, 0.);
+                if (b->dtype == SPK_F64) toDevice<double>(b, static_cast<double *>(b->ltol.p), t.data(), t.size());
+                else toDevice<float>(b, static_cast<float *>(b->ltol.p), t.data(), t.size());
+            }
         }
         return 0;
     } catch (const std::exception &e) {
@@ -501,6 +770,11 @@ int spk_batch_set_iterate(spk_batch *b, int k, const double *z) {
         if (k < 0 || k >= b->count) throw std::runtime_error("instance index out of range");
         check(hipStreamSynchronize(b->ctx->stream), "sync");
         const size_t n = b->probs[k]->hp.sIter;
+        if (b->lanes) {
+            if (b->dtype == SPK_F64) laneColumn<double>(b, LR_ZIO, k, nullptr, z, n);
+            else laneColumn<float>(b, LR_ZIO, k, nullptr, z, n);
+            return 0;
+        }
         if (b->dtype == SPK_F64) toDevice<double>(b, corePtr<double>(b, k), z, n);
         else toDevice<float>(b, corePtr<float>(b, k), z, n);
         return 0;
@@ -514,6 +788,11 @@ int spk_batch_get_iterate(spk_batch *b, int k, double *z) {
         if (k < 0 || k >= b->count) throw std::runtime_error("instance index out of range");
         check(hipStreamSynchronize(b->ctx->stream), "sync");
         const size_t n = b->probs[k]->hp.sIter;
+        if (b->lanes) {
+            if (b->dtype == SPK_F64) laneColumn<double>(b, LR_ZIO, k, z, nullptr, n);
+            else laneColumn<float>(b, LR_ZIO, k, z, nullptr, n);
+            return 0;
+        }
         if (b->dtype == SPK_F64) toHost<double>(b, z, corePtr<double>(b, k), n);
         else toHost<float>(b, z, corePtr<float>(b, k), n);
         return 0;
@@ -523,6 +802,16 @@ int spk_batch_get_iterate(spk_batch *b, int k, double *z) {
 }
 
 int spk_batch_set_iterates(spk_batch *b, const double *z) {
+    if (b->lanes) {
+        try {
+            check(hipStreamSynchronize(b->ctx->stream), "sync");
+            if (b->dtype == SPK_F64) laneUpload<double>(b, LR_ZIO, z, b->probs[0]->hp.sIter);
+            else laneUpload<float>(b, LR_ZIO, z, b->probs[0]->hp.sIter);
+            return 0;
+        } catch (const std::exception &e) {
+            return fail(b->ctx, e);
+        }
+    }
     size_t pos = 0;
     for (int k = 0; k < b->count; k++) {
         if (spk_batch_set_iterate(b, k, z + pos)) return -1;
@@ -532,6 +821,16 @@ int spk_batch_set_iterates(spk_batch *b, const double *z) {
 }
 
 int spk_batch_get_iterates(spk_batch *b, double *z) {
+    if (b->lanes) {
+        try {
+            check(hipStreamSynchronize(b->ctx->stream), "sync");
+            if (b->dtype == SPK_F64) laneDownload<double>(b, LR_ZIO, z, b->probs[0]->hp.sIter);
+            else laneDownload<float>(b, LR_ZIO, z, b->probs[0]->hp.sIter);
+            return 0;
+        } catch (const std::exception &e) {
+            return fail(b->ctx, e);
+        }
+    }
     size_t pos = 0;
     for (int k = 0; k < b->count; k++) {
         if (spk_batch_get_iterate(b, k, z + pos)) return -1;
@@ -542,6 +841,9 @@ int spk_batch_get_iterates(spk_batch *b, double *z) {
 
 void *spk_batch_iterate_ptr(spk_batch *b, int k) {
     if (k < 0 || k >= b->count) return nullptr;
+    if (b->lanes)
+        return static_cast<char *>(b->lbuf.p) +
+               (b->lRegion[LR_ZIO] + spk::laneIndex(b->probs[0]->hp.sIter, 0, k)) * b->elSize();
     return static_cast<char *>(b->core.p) + b->coreOff[k] * b->elSize();
 }
 
@@ -549,6 +851,50 @@ int spk_batch_copy_iterates(spk_batch *dst, const spk_batch *src) {
     try {
         if (dst->count != src->count) throw std::runtime_error("batch sizes differ");
         check(hipStreamSynchronize(src->ctx->stream), "sync");
+        if (src->lanes || dst->lanes) {
+            const size_t n = dst->probs[0]->hp.sIter;
+            for (int k = 0; k < dst->count; k++)
+                if ((size_t) src->probs[k]->hp.sIter != n || (size_t) dst->probs[k]->hp.sIter != n)
+                    throw std::runtime_error("instance sizes differ");
+            const bool s64 = src->dtype == SPK_F64, d64 = dst->dtype == SPK_F64;
+            if (src->lanes && dst->lanes) {
+                if (src->Bpad != dst->Bpad) throw std::runtime_error("batch sizes differ");
+                const long len = (long) (n * dst->Bpad);
+                if (s64 == d64)
+                    check(hipMemcpyAsync(static_cast<char *>(dst->lbuf.p) + dst->lRegion[LR_ZIO] * dst->elSize(),
+                                         static_cast<const char *>(src->lbuf.p) + src->lRegion[LR_ZIO] * src->elSize(),
+                                         len * dst->elSize(), hipMemcpyDeviceToDevice, dst->ctx->stream), "copy");
+                else if (!s64)
+                    hipLaunchKernelGGL(spk::k_f32_to_f64, dim3(1024), dim3(256), 0, dst->ctx->stream,
+                                       lanePtr<float>(src, LR_ZIO), lanePtr<double>(dst, LR_ZIO), len);
+                else
+                    hipLaunchKernelGGL(spk::k_f64_to_f32, dim3(1024), dim3(256), 0, dst->ctx->stream,
+                                       lanePtr<double>(src, LR_ZIO), lanePtr<float>(dst, LR_ZIO), len);
+            } else if (dst->lanes) {
+                if (s64 && d64) launchInterleave<double, double>(src, dst, n);
+                else if (s64) launchInterleave<double, float>(src, dst, n);
+                else if (d64) launchInterleave<float, double>(src, dst, n);
+                else launchInterleave<float, float>(src, dst, n);
+            } else {
+                if (s64 && d64) launchDeinterleave<double, double>(src, dst, n);
+                else if (s64) launchDeinterleave<double, float>(src, dst, n);
+                else if (d64) launchDeinterleave<float, double>(src, dst, n);
+                else launchDeinterleave<float, float>(src, dst, n);
+            }
+            // fresh residual history for the continued solve
+            if (dst->lanes) {
+                if (d64) fillDevice<double>(dst, lanePtr<double>(dst, LR_RIO), n * dst->Bpad);
+                else fillDevice<float>(dst, lanePtr<float>(dst, LR_RIO), n * dst->Bpad);
+            } else {
+                for (int k = 0; k < dst->count; k++) {
+                    if (d64) fillDevice<double>(dst, corePtr<double>(dst, k) + 4 * n, n);
+                    else fillDevice<float>(dst, corePtr<float>(dst, k) + 4 * n, n);
+                }
+            }
+            check(hipGetLastError(), "copy kernels");
+            check(hipStreamSynchronize(dst->ctx->stream), "sync");
+            return 0;
+        }
         for (int k = 0; k < dst->count; k++) {
             const size_t n = dst->probs[k]->hp.sIter;
             if (src->probs[k]->hp.sIter != (long) n) throw std::runtime_error("instance sizes differ");
@@ -590,6 +936,13 @@ int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm) {
         if (o.ls_method != SPK_LS_HOUSEHOLDER && o.ls_method != SPK_LS_GRAM)
             throw std::runtime_error("ls_method must be SPK_LS_HOUSEHOLDER or SPK_LS_GRAM");
         check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        if (b->lanes) {
+            if (o.anderson_buffer > spk_batch::LANE_MAXM)
+                throw std::runtime_error("shared batches support anderson_buffer <= 4");
+            if (b->dtype == SPK_F64) launchLaneSolve<double>(b, o, warm);
+            else launchLaneSolve<float>(b, o, warm);
+            return 0;
+        }
         ensureAnderson(b, o.anderson_buffer);
         if (b->dtype == SPK_F64) launchSolve<double>(b, o, warm);
         else launchSolve<float>(b, o, warm);
@@ -639,6 +992,45 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
         if (op < SPK_OP_L || op > SPK_OP_ELL) throw std::runtime_error("unknown debug op");
         check(hipSetDevice(b->ctx->device), "hipSetDevice");
         check(hipStreamSynchronize(b->ctx->stream), "sync");
+        if (b->lanes) {
+            const size_t n = b->probs[0]->hp.sIter;
+            const bool f64 = b->dtype == SPK_F64;
+            if (f64) {
+                laneUpload<double>(b, LR_ZIO, in, n);
+                laneUpload<double>(b, 1, out, n);
+            } else {
+                laneUpload<float>(b, LR_ZIO, in, n);
+                laneUpload<float>(b, 1, out, n);
+            }
+            spk_options o;
+            spk_default_options(&o);
+            b->dOut.alloc(sizeof(spk::Out) * b->count);
+            check(hipMemset(b->dOut.p, 0, sizeof(spk::Out) * b->count), "zero out");
+            const dim3 grid((unsigned) (b->Bpad / 64)), block(laneBlock(o));
+            if (f64) {
+                buildLaneInst<double>(b, o, 0);
+                hipLaunchKernelGGL(spk::k_lane_debug<double>, grid, block, 0, b->ctx->stream,
+                                   static_cast<const spk::LaneInst<double> *>(b->dInst.p),
+                                   static_cast<spk::Out *>(b->dOut.p), op);
+            } else {
+                buildLaneInst<float>(b, o, 0);
+                hipLaunchKernelGGL(spk::k_lane_debug<float>, grid, block, 0, b->ctx->stream,
+                                   static_cast<const spk::LaneInst<float> *>(b->dInst.p),
+                                   static_cast<spk::Out *>(b->dOut.p), op);
+            }
+            check(hipGetLastError(), "launch k_lane_debug");
+            check(hipStreamSynchronize(b->ctx->stream), "sync debug op");
+            if (f64) laneDownload<double>(b, 1, out, n);
+            else laneDownload<float>(b, 1, out, n);
+            std::vector<spk::Out> res(b->count);
+            check(hipMemcpy(res.data(), b->dOut.p, sizeof(spk::Out) * b->count, hipMemcpyDeviceToHost), "results");
+            if (scalars)
+                for (int k = 0; k < b->count; k++) {
+                    scalars[2 * k] = res[k].w;
+                    scalars[2 * k + 1] = res[k].err;
+                }
+            return 0;
+        }
         size_t pos = 0;
         for (int k = 0; k < b->count; k++) {
             const size_t n = b->probs[k]->hp.sIter;

@@ -30,9 +30,43 @@ struct Inst {
     const T *x0s;  // scaled initial state (nx)
     // options
     T tolAbs, tolRel;  // already multiplied by min(scaling) when preconditioned
+    T tolFixed;        // > 0: accept when the error is <= tolFixed from the first check on
     long maxIters;
     int maxInner, andBuff, checkPeriod, method, warm, lsMethod, dynInverse;
     double maxTicks;  // wall-clock cap in device ticks (0 = none)
+};
+
+// Block-interleaved layout of a region of `rows` rows: instances are grouped in blocks of 64 and
+// each block stores rows x 64 words contiguously, element e of instance b at
+// ((b / 64) * rows + e) * 64 + b % 64.
+__host__ __device__ inline size_t laneIndex(long rows, long e, long b) {
+    return ((size_t) (b >> 6) * rows + e) * 64 + (b & 63);
+}
+
+// Many instances of ONE problem in the block-interleaved layout: a wave whose lanes are the 64
+// instances of one block moves 64 consecutive words per access, a workgroup's state is one
+// contiguous slab per region, and the factor tables are read at wave-uniform addresses (scalar
+// loads shared by the wave).
+template <typename T>
+struct LaneInst {
+    int N, nl, nlf, S, E, nx, nu, yD, nD, ncost, lcost, ncon, lcon, con3, con5;
+    long oU, oX, oY, oT, oS, sPrim, oI, oII, oIII, oIV, oV, oVI, sDual, sIter;
+    T gamma, gammaRecip;
+    const int *anc, *chFrom, *numCh, *stageFrom, *stageTo;
+    DTab<T> BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, RtInv, NN, b, sQ, sR, q, r, trans, lsQ, lq, ltrans;
+    DTab<T> nG, nLB, nUB, lG, lLB, lUB, nGT, lGT;
+    long B;     // padded instance count (multiple of 64); a region of r rows holds r * B words
+    int count;  // instances in use
+    T *work[9];   // rotating roles z, cand, zPrev, res, resPrev, backup + cand2, dir, dirS
+    T *zio, *rio; // accepted iterate and its residual (warm-start input, solve output)
+    T *hS, *hY;   // Anderson rings, andBuff slots of sIter x B each
+    T *qd, *dd;   // dynamics sweep intermediates (nx*N and nu*nl rows)
+    const T *x0s; // scaled initial states (nx rows)
+    const T *tolFixed;  // per-instance given tolerance (> 0), or nullptr
+    T tolAbs, tolRel;
+    long maxIters;
+    int maxInner, andBuff, checkPeriod, method, warm;
+    double maxTicks;
 };
 
 // Phases timed inside the solve (wall_clock64 ticks, summed over the solve).

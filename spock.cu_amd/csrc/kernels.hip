@@ -1089,15 +1089,20 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
     }
     if (c.tid < 64) G[c.tid] = 0.;
     __syncthreads();
-    bool initFlag = true;
+    // a given tolerance (> 0) skips the reference's first-check calibration, so warm starts and
+    // polish solves are accepted against the threshold of the cold solve they continue
+    bool initFlag = !(P.tolFixed > T(0));
     int status = ST_NOTRUN;
     long iters = 0;
-    T tol = T(0), lastErr = T(NAN), wLast = T(NAN);
+    T tol = initFlag ? T(0) : P.tolFixed, lastErr = T(NAN), wLast = T(NAN);
     const T c0 = T(0.99), c1 = T(0.99), c2 = T(0.99), beta = T(0.5), sigma = T(0.1), lambda = T(1);
     const T inf = T(INFINITY);
     T zeta = inf, w = inf, wSafe = inf;
     long k0 = 0, k1 = 0, k2 = 0, k2bt = 0, k3 = 0;
-    const long hardCap = P.maxIters > 0 ? P.maxIters + 1 : 2000000;
+    // the iteration cap is only tested at error checks (src/cache.cuh:937-940): the last outer
+    // iteration is the first check index at or beyond maxIters
+    const long hardCap =
+        P.maxIters > 0 ? (P.maxIters + P.checkPeriod - 1) / P.checkPeriod * P.checkPeriod + 1 : 2000000;
     long it = 0;
     for (it = 0; it < hardCap; it++) {
         if (it != 0 && P.method == METHOD_SPOCK) {  // acceptCandidate: z <- cand

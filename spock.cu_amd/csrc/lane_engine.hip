@@ -1,0 +1,1196 @@
+// Lane engine: many instances of ONE problem solved together, lanes over instances.
+//
+// Layout (LaneInst in engine.hpp): element e of instance b of every state vector is at
+// base[e * B + b], so each memory instruction of a wave moves 64 consecutive instances, and the
+// factor tables are read at wave-uniform addresses (scalar loads, the operand of every FMA is a
+// scalar register).  A workgroup owns 64 instances; its waves split the node tasks of each tree
+// stage and the element ranges of the iterate-wide passes.  Per-instance norms are per-lane
+// sums, combined across waves through LDS in a fixed order, so results do not depend on timing.
+// SuperMann (src/cache.cuh:1008-1132) runs in lockstep outer iterations with per-lane decisions:
+// the buffer roles rotate uniformly where every lane takes the same step and per-lane masked
+// copies replace the rotations where lanes branch (K0 vs line search, K1/K2/K3).
+//
+// Operator semantics are those of the single-instance engine and of the reference
+// (src/operator.cuh:41-82, src/dynamics.cuh:74-174, src/cache.cuh:650-801).  Deviations: the
+// dynamics sweep always applies the precomputed (L L')^-1 and the Anderson least squares is the
+// FP64 Gram/Cholesky solve (the same arithmetic as the single-instance engine's Gram option).
+#include "engine.hpp"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#ifndef SPK_LANE_MAX_THREADS
+#define SPK_LANE_MAX_THREADS 1024
+#endif
+
+namespace spk {
+namespace lane {
+
+constexpr int MAXM = 4;  // Anderson memory supported by the lane engine
+
+struct Ctx {
+    int lane, wave, nw, tid;
+    int gl;  // this lane's instance index
+};
+
+__device__ __forceinline__ Ctx ctx() {
+    Ctx c;
+    c.tid = threadIdx.x;
+    c.lane = c.tid & 63;
+    c.wave = __builtin_amdgcn_readfirstlane(c.tid >> 6);
+    c.nw = blockDim.x >> 6;
+    c.gl = blockIdx.x * 64 + c.lane;
+    return c;
+}
+
+// A pointer every lane holds the same value of, moved to scalar registers: loads through it
+// become scalar loads (needed inside non-inlined functions, whose arguments arrive per lane).
+template <typename T>
+__device__ __forceinline__ T *uniformPtr(T *p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned) v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned) (v >> 32));
+    return reinterpret_cast<T *>(((unsigned long long) hi << 32) | lo);
+}
+
+// Constant address space: the descriptor, the tree arrays and the factor tables are never written
+// while a solve runs, and wave-uniform reads through such pointers become scalar loads.
+#define SPK_CONST __attribute__((address_space(4)))
+#define SPK_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+using CP = const SPK_CONST T *;
+template <typename T>
+using LP = const SPK_CONST LaneInst<T> &;
+
+template <typename T>
+__device__ __forceinline__ CP<T> cst(const T *p) {
+    return (CP<T>) p;
+}
+
+// Node n's entry of a per-node table, as a constant-space pointer.
+template <typename T>
+__device__ __forceinline__ CP<T> tab(const SPK_CONST DTab<T> &t, int n) {
+    return cst(t.data) + (size_t) cst(t.idx)[n] * t.rows * t.cols;
+}
+
+// Lane view of a region in the block-interleaved layout (each 64-instance block holds its rows
+// x 64 words contiguously, so a workgroup streams its own contiguous slab): v[e] is element e of
+// this lane's instance.  The row pointer p + 64 e is wave-uniform; the lane is the only per-lane
+// part of the address.
+template <typename T>
+struct LV {
+    T *p;     // this workgroup's block of the region
+    int off;  // lane
+    __device__ __forceinline__ SPK_GLOBAL T &operator[](long e) const { return ((SPK_GLOBAL T *) (p + e * 64))[off]; }
+    __device__ __forceinline__ LV shift(long e0) const { return LV{p + e0 * 64, off}; }
+};
+
+template <typename T>
+__device__ __forceinline__ LV<T> lview(T *region, long rows, int lane) {
+    return LV<T>{region + (long) blockIdx.x * rows * 64, lane};
+}
+
+template <typename T>
+struct RowChunk {
+    static constexpr int v = 16;  // result rows held in registers per chunk
+};
+
+template <typename T, int NY>
+__device__ __forceinline__ void zeroY(T (&y)[NY]) {
+#pragma unroll
+    for (int i = 0; i < NY; i++) y[i] = T(0);
+}
+
+// y[r] += sum_c M[r + c * ld] * x(c) for r < RR: M is wave-uniform, x(c) the lane's own value.
+// XB vector loads are issued before their FMAs; CB columns of the matrix at a time stay in
+// scalar registers (CB * RR <= 64).  Columns past the end are clamped to the last one and enter
+// with x = 0, so every load stays inside the table (which carries 64 elements of padding for the
+// RR rows that may run past a matrix's last column).
+template <int RR, typename T, int NY, typename XF>
+__device__ __forceinline__ void mvFixed(T (&y)[NY], CP<T> M, int ld, int cols, XF x) {
+    static_assert(RR <= NY, "row chunk");
+    constexpr int XB = 16;
+    constexpr int CB = RR >= 16 ? 4 : 8;
+    for (int c0 = 0; c0 < cols; c0 += XB) {
+        T xv[XB];
+#pragma unroll
+        for (int q = 0; q < XB; q++) {
+            const int cc = min(c0 + q, cols - 1);
+            const T v = x(cc);
+            xv[q] = (c0 + q < cols) ? v : T(0);
+        }
+#pragma unroll
+        for (int q0 = 0; q0 < XB; q0 += CB) {
+            if (c0 + q0 >= cols) break;
+#pragma unroll
+            for (int q = q0; q < q0 + CB; q++) {
+                CP<T> m = M + (size_t) min(c0 + q, cols - 1) * ld;
+#pragma unroll
+                for (int r = 0; r < RR; r++) y[r] += m[r] * xv[q];
+            }
+        }
+    }
+}
+
+// rows <= NY; the row count picks a fixed-size variant so every scalar load is unconditional.
+template <typename T, int NY, typename XF>
+__device__ __forceinline__ void mv(T (&y)[NY], CP<T> M, int rows, int ld, int cols, XF x) {
+    static_assert(NY == 16, "row chunk");
+    if (rows <= 0 || cols <= 0) return;
+    if (rows <= 8) mvFixed<8>(y, M, ld, cols, x);
+    else mvFixed<16>(y, M, ld, cols, x);
+}
+
+// Up to 32 result rows held at once (two 16-row blocks), for the in-place updates of d and of
+// the kernel projection, which read every input row before any output row is written.
+template <typename T, typename XF>
+__device__ __forceinline__ void mv32(T (&y0)[16], T (&y1)[16], CP<T> M, int rows, int ld, int cols, XF x) {
+    mv(y0, M, min(rows, 16), ld, cols, x);
+    if (rows > 16) mv(y1, M + 16, rows - 16, ld, cols, x);
+}
+
+template <typename T, int NY, typename F>
+__device__ __forceinline__ void each(const T (&y)[NY], int rows, F f) {
+#pragma unroll
+    for (int r = 0; r < NY; r++)
+        if (r < rows) f(r, y[r]);
+}
+
+template <typename T>
+__device__ __forceinline__ T projBox(T v, T lb, T ub) {  // src/definitions.cu:38-52
+    const int lower = (v <= lb), upper = (v >= ub);
+    const T lbi = isinf(lb) ? T(0) : T(lower) * lb;
+    const T ubi = isinf(ub) ? T(0) : T(upper) * ub;
+    return v * T(1 - lower) * T(1 - upper) + lbi + ubi;
+}
+
+// ------------------------------------------------------------------ L* per node
+// src/operator.cuh:63-82; src/costs.cuh:162-195 (quadratic), :418-458 (quadratic+linear);
+// src/constraints.cuh:171-191 (rectangle), :265-286 (polyhedron).
+// emit(kind, k, value): kind 0 = x (k < nx), 1 = u (k < nu), 2 = y (k < yD), 3 = s, 4 = t.
+template <typename T, typename E>
+__device__ __forceinline__ void adjNode(LP<T> P, const LV<T> &d, int n, E emit) {
+    constexpr int NY = RowChunk<T>::v;
+    const int nx = P.nx, nu = P.nu, nxu = nx + nu, d4 = nxu + 2;
+    T y[NY];
+    if (n < P.nl) {
+        const T II = d[P.oII + n];
+        emit(3, 0, II);
+        CP<T> bn = tab(P.b, n);
+        for (int k = 0; k < P.yD; k++) emit(2, k, d[P.oI + (long) n * P.yD + k] - bn[k] * II);
+        const long b3 = P.oIII + (long) n * P.con3;
+        const int c0 = cst(P.chFrom)[n], nc = cst(P.numCh)[n];
+        for (int part = 0; part < 2; part++) {  // x rows, then u rows
+            const int rows = part == 0 ? nx : nu;
+            for (int r0 = 0; r0 < rows; r0 += NY) {
+                const int rr = min(NY, rows - r0);
+                zeroY(y);
+                if (P.ncon == CON_RECT_DEV) {
+#pragma unroll
+                    for (int r = 0; r < NY; r++)
+                        if (r < rr) y[r] = d[b3 + part * nx + r0 + r];
+                } else if (P.ncon == CON_POLY_DEV) {
+                    mv(y, tab(P.nGT, n) + part * nx + r0, rr, nxu, P.con3, [&](int cc) { return d[b3 + cc]; });
+                }
+                for (int ci = 0; ci < nc; ci++) {  // children accumulate in order (src/costs.cuh:172-177)
+                    const int j = c0 + ci;
+                    const long iv = P.oIV + (long) j * d4;
+                    if (part == 0) mv(y, tab(P.sQ, j) + r0, rr, nx, nx, [&](int cc) { return d[iv + cc]; });
+                    else mv(y, tab(P.sR, j) + r0, rr, nu, nu, [&](int cc) { return d[iv + nx + cc]; });
+                    if (P.ncost == COST_QUADLIN_DEV) {
+                        const T a = d[iv + nxu], bb = d[iv + nxu + 1];
+                        CP<T> qj = (part == 0 ? tab(P.q, j) : tab(P.r, j)) + r0;
+#pragma unroll
+                        for (int r = 0; r < NY; r++)
+                            if (r < rr) y[r] = y[r] - T(0.5) * qj[r] * a - T(0.5) * qj[r] * bb;
+                    }
+                }
+                each(y, rr, [&](int r, T v) { emit(part, r0 + r, v); });
+            }
+        }
+    } else {
+        const int l = n - P.nl;
+        const long b5 = P.oV + (long) l * P.con5;
+        const long vi = P.oVI + (long) l * (nx + 2);
+        const T a = d[vi + nx], bb = d[vi + nx + 1];
+        for (int r0 = 0; r0 < nx; r0 += NY) {
+            const int rr = min(NY, nx - r0);
+            zeroY(y);
+            if (P.lcon == CON_RECT_DEV) {
+#pragma unroll
+                for (int r = 0; r < NY; r++)
+                    if (r < rr) y[r] = d[b5 + r0 + r];
+            } else if (P.lcon == CON_POLY_DEV) {
+                mv(y, tab(P.lGT, l) + r0, rr, nx, P.con5, [&](int cc) { return d[b5 + cc]; });
+            }
+            mv(y, tab(P.lsQ, l) + r0, rr, nx, nx, [&](int cc) { return d[vi + cc]; });
+            if (P.lcost == COST_QUADLIN_DEV) {
+                CP<T> lq = tab(P.lq, l) + r0;
+#pragma unroll
+                for (int r = 0; r < NY; r++)
+                    if (r < rr) y[r] = y[r] - T(0.5) * lq[r] * a - T(0.5) * lq[r] * bb;
+            }
+            each(y, rr, [&](int r, T v) { emit(0, r0 + r, v); });
+        }
+        emit(3, 0, (a + bb) * T(0.5));
+    }
+    if (n >= 1) {
+        const long iv = P.oIV + (long) n * d4;
+        emit(4, 0, (d[iv + nxu] + d[iv + nxu + 1]) * T(0.5));
+    } else {
+        emit(4, 0, T(0));
+    }
+}
+
+// ------------------------------------------------------------------ L per node
+// src/operator.cuh:41-60; src/costs.cuh:126-160, :379-416; src/constraints.cuh:158-169, :251-263.
+// Handler: elemI(n, k, v), elemII(n, v), elemBox(e, v, lb, ub) for III and V, and for the SOC
+// blocks IV and VI socBegin(), socPart(e, v, trans) for all but the last two entries, then
+// socEnd(base, dim, h, trans) for the two entries that both carry h.
+template <typename T, typename PV, typename H>
+__device__ __forceinline__ void opNode(LP<T> P, int n, PV pv, H &h) {
+    constexpr int NY = RowChunk<T>::v;
+    const int nx = P.nx, nu = P.nu, nxu = nx + nu;
+    T y[NY];
+    if (n < P.nl) {
+        CP<T> bn = tab(P.b, n);
+        T by = T(0);
+        for (int k = 0; k < P.yD; k++) {
+            const T yk = pv(P.oY + (long) n * P.yD + k);
+            by += bn[k] * yk;
+            h.elemI(n, k, yk);
+        }
+        h.elemII(n, pv(P.oS + n) - by);
+        const long base = P.oIII + (long) n * P.con3;
+        if (P.ncon == CON_RECT_DEV) {
+            CP<T> lb = tab(P.nLB, n), ub = tab(P.nUB, n);
+            for (int k = 0; k < nx; k++) h.elemBox(base + k, pv(P.oX + (long) n * nx + k), lb[k], ub[k]);
+            for (int k = 0; k < nu; k++) h.elemBox(base + nx + k, pv(P.oU + (long) n * nu + k), lb[nx + k], ub[nx + k]);
+        } else if (P.ncon == CON_POLY_DEV) {
+            CP<T> G = tab(P.nG, n), lb = tab(P.nLB, n), ub = tab(P.nUB, n);
+            for (int r0 = 0; r0 < P.con3; r0 += NY) {
+                const int rr = min(NY, P.con3 - r0);
+                zeroY(y);
+                mv(y, G + r0, rr, P.con3, nx, [&](int cc) { return pv(P.oX + (long) n * nx + cc); });
+                mv(y, G + (size_t) P.con3 * nx + r0, rr, P.con3, nu, [&](int cc) { return pv(P.oU + (long) n * nu + cc); });
+                each(y, rr, [&](int r, T v) { h.elemBox(base + r0 + r, v, lb[r0 + r], ub[r0 + r]); });
+            }
+        }
+    }
+    {  // IV
+        const long base = P.oIV + (long) n * (nxu + 2);
+        CP<T> tr = tab(P.trans, n);
+        h.socBegin();
+        if (n >= 1) {
+            const int a = cst(P.anc)[n];
+            for (int r0 = 0; r0 < nx; r0 += NY) {
+                const int rr = min(NY, nx - r0);
+                zeroY(y);
+                mv(y, tab(P.sQ, n) + r0, rr, nx, nx, [&](int cc) { return pv(P.oX + (long) a * nx + cc); });
+                each(y, rr, [&](int r, T v) { h.socPart(base + r0 + r, v, tr[r0 + r]); });
+            }
+            for (int r0 = 0; r0 < nu; r0 += NY) {
+                const int rr = min(NY, nu - r0);
+                zeroY(y);
+                mv(y, tab(P.sR, n) + r0, rr, nu, nu, [&](int cc) { return pv(P.oU + (long) a * nu + cc); });
+                each(y, rr, [&](int r, T v) { h.socPart(base + nx + r0 + r, v, tr[nx + r0 + r]); });
+            }
+            T hv = pv(P.oT + n) * T(0.5);
+            if (P.ncost == COST_QUADLIN_DEV) {
+                CP<T> qn = tab(P.q, n), rn = tab(P.r, n);
+                T pq = T(0), pr = T(0);
+                for (int k = 0; k < nx; k++) pq += qn[k] * pv(P.oX + (long) a * nx + k);
+                for (int k = 0; k < nu; k++) pr += rn[k] * pv(P.oU + (long) a * nu + k);
+                hv = hv + T(-0.5) * pq;
+                hv = hv + T(-0.5) * pr;
+            }
+            h.socEnd(base, nxu + 2, hv, tr);
+        } else {
+            for (int k = 0; k < nxu; k++) h.socPart(base + k, T(0), tr[k]);
+            h.socEnd(base, nxu + 2, T(0), tr);
+        }
+    }
+    if (n >= P.nl) {  // V and VI
+        const int l = n - P.nl;
+        const long b5 = P.oV + (long) l * P.con5;
+        if (P.lcon == CON_RECT_DEV) {
+            CP<T> lb = tab(P.lLB, l), ub = tab(P.lUB, l);
+            for (int k = 0; k < nx; k++) h.elemBox(b5 + k, pv(P.oX + (long) n * nx + k), lb[k], ub[k]);
+        } else if (P.lcon == CON_POLY_DEV) {
+            CP<T> lb = tab(P.lLB, l), ub = tab(P.lUB, l);
+            for (int r0 = 0; r0 < P.con5; r0 += NY) {
+                const int rr = min(NY, P.con5 - r0);
+                zeroY(y);
+                mv(y, tab(P.lG, l) + r0, rr, P.con5, nx, [&](int cc) { return pv(P.oX + (long) n * nx + cc); });
+                each(y, rr, [&](int r, T v) { h.elemBox(b5 + r0 + r, v, lb[r0 + r], ub[r0 + r]); });
+            }
+        }
+        const long base = P.oVI + (long) l * (nx + 2);
+        CP<T> tr = tab(P.ltrans, l);
+        h.socBegin();
+        for (int r0 = 0; r0 < nx; r0 += NY) {
+            const int rr = min(NY, nx - r0);
+            zeroY(y);
+            mv(y, tab(P.lsQ, l) + r0, rr, nx, nx, [&](int cc) { return pv(P.oX + (long) n * nx + cc); });
+            each(y, rr, [&](int r, T v) { h.socPart(base + r0 + r, v, tr[r0 + r]); });
+        }
+        T hs = pv(P.oS + n) * T(0.5);
+        if (P.lcost == COST_QUADLIN_DEV) {
+            CP<T> lq = tab(P.lq, l);
+            T pq = T(0);
+            for (int k = 0; k < nx; k++) pq += lq[k] * pv(P.oX + (long) n * nx + k);
+            hs = hs + T(-0.5) * pq;
+        }
+        h.socEnd(base, nx + 2, hs, tr);
+    }
+}
+
+// Dual prox of the CP step (src/cache.cuh:770-789): w = (L gamma + z_d)/gamma + c and
+// cand_d = gamma (w - Pi(w)), Pi the masked NNOC (src/risks.cuh:99-106), NNOC (src/cones.cuh:134-137),
+// box (src/constraints.cuh:70-74) or SOC (src/projections.cuh:102-128).  SOC blocks stage w in
+// cand_d, accumulate |w|^2 and rescale in socEnd.
+template <typename T>
+struct Prox {
+    LP<T> P;
+    LV<T> zd, cd;
+    T ss;
+    __device__ __forceinline__ T wOf(T Lv, long e, T tr) const { return (Lv * P.gamma + zd[e]) * P.gammaRecip + tr; }
+    __device__ __forceinline__ void elemI(int n, int k, T v) {
+        const long e = P.oI + (long) n * P.yD + k;
+        const T w = wOf(v, e, T(0));
+        const int idx = (k < 2 * cst(P.numCh)[n]) && (w < T(0));
+        cd[e] = (w - T(1 - idx) * w) * P.gamma;
+    }
+    __device__ __forceinline__ void elemII(int n, T v) {
+        const long e = P.oII + n;
+        const T w = wOf(v, e, T(0));
+        cd[e] = (w - fmax(T(0), w)) * P.gamma;
+    }
+    __device__ __forceinline__ void elemBox(long e, T v, T lb, T ub) {
+        const T w = wOf(v, e, T(0));
+        cd[e] = (w - projBox(w, lb, ub)) * P.gamma;
+    }
+    __device__ __forceinline__ void socBegin() { ss = T(0); }
+    __device__ __forceinline__ void socPart(long e, T v, T tr) {
+        const T w = wOf(v, e, tr);
+        cd[e] = w;
+        ss += w * w;
+    }
+    __device__ __forceinline__ void socEnd(long base, int dim, T hv, CP<T> tr) {
+        const long k1 = base + dim - 2, k2 = base + dim - 1;
+        const T w1 = wOf(hv, k1, tr[dim - 2]);
+        const T w2 = wOf(hv, k2, tr[dim - 1]);
+        const T sq = ss + w1 * w1;
+        const T nrm = sqrt(sq), tl = w2;
+        const int i1 = nrm <= tl;
+        const int i2 = (nrm <= -tl) && !i1;
+        const int i3 = !i1 && !i2;
+        const T sc = (nrm + tl) / (T(2) * nrm + T(1 - i3));
+        const T mult = T(1 - i2) * (T(i3) * sc + T(1 - i3));
+        for (int k = 0; k < dim - 2; k++) {
+            const T w = cd[base + k];
+            cd[base + k] = (w - w * mult) * P.gamma;
+        }
+        const T p1 = w1 * mult;
+        const int i1b = (1 - i2) * (1 - i3);
+        const T p2 = T(i1b) * (w2 * T(1 - i2)) + T((1 - i1b) * (1 - i2)) * (T(i3) * (sc * nrm - T(1)) + T(1));
+        cd[k1] = (w1 - p1) * P.gamma;
+        cd[k2] = (w2 - p2) * P.gamma;
+    }
+};
+
+// ell(y) = [L* y_d; L y_p] reductions: sum x.(y - gamma ell) and max |y/gamma - ell|
+// (src/cache.cuh:722-734, 917-931).
+template <typename T>
+struct Red {
+    LP<T> P;
+    LV<T> x, y;
+    T dot, mx;
+    __device__ __forceinline__ void acc(long e, T ell) {
+        const T yk = y[e];
+        dot += x[e] * (ell * (-P.gamma) + yk);
+        mx = fmax(mx, fabs(yk * P.gammaRecip - ell));
+    }
+    __device__ __forceinline__ void elemI(int n, int k, T v) { acc(P.sPrim + P.oI + (long) n * P.yD + k, v); }
+    __device__ __forceinline__ void elemII(int n, T v) { acc(P.sPrim + P.oII + n, v); }
+    __device__ __forceinline__ void elemBox(long e, T v, T, T) { acc(P.sPrim + e, v); }
+    __device__ __forceinline__ void socBegin() {}
+    __device__ __forceinline__ void socPart(long e, T v, T) { acc(P.sPrim + e, v); }
+    __device__ __forceinline__ void socEnd(long base, int dim, T hv, CP<T>) {
+        acc(P.sPrim + base + dim - 2, hv);
+        acc(P.sPrim + base + dim - 1, hv);
+    }
+};
+
+template <typename T>
+struct Store {
+    LP<T> P;
+    LV<T> out;  // dual part
+    __device__ __forceinline__ void elemI(int n, int k, T v) { out[P.oI + (long) n * P.yD + k] = v; }
+    __device__ __forceinline__ void elemII(int n, T v) { out[P.oII + n] = v; }
+    __device__ __forceinline__ void elemBox(long e, T v, T, T) { out[e] = v; }
+    __device__ __forceinline__ void socBegin() {}
+    __device__ __forceinline__ void socPart(long e, T v, T) { out[e] = v; }
+    __device__ __forceinline__ void socEnd(long base, int dim, T hv, CP<T>) {
+        out[base + dim - 2] = hv;
+        out[base + dim - 1] = hv;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ long primIndex(LP<T> P, int n, int kind, int k) {
+    if (kind == 0) return P.oX + (long) n * P.nx + k;
+    if (kind == 1) return P.oU + (long) n * P.nu + k;
+    if (kind == 2) return P.oY + (long) n * P.yD + k;
+    if (kind == 3) return P.oS + n;
+    return P.oT + n;
+}
+
+// ------------------------------------------------------------------ T phases
+// modifyPrimal + proxRootS (src/cache.cuh:748-753, 639-642); leaves seed q = -xbar (src/dynamics.cuh:79-81).
+template <typename T>
+__device__ __forceinline__ void phaseAdj(LP<T> P, const Ctx &c, const LV<T> &z, const LV<T> &cand,
+                                         const LV<T> &qd) {
+    const T g = -P.gamma;
+    const LV<T> zd = z.shift(P.sPrim);
+    for (int n = c.wave; n < P.N; n += c.nw) {
+        adjNode(P, zd, n, [&](int kind, int k, T v) {
+            const long e = primIndex(P, n, kind, k);
+            T val = v * g + z[e];
+            if (kind == 3 && n == 0) val -= P.gamma;
+            cand[e] = val;
+            if (kind == 0 && n >= P.nl) qd[(long) n * P.nx + k] = -val;
+        });
+    }
+}
+
+// Backward sweep step for node i (src/dynamics.cuh:85-139) with R~^-1 applied as one matvec.
+template <typename T>
+__device__ __forceinline__ void dynBackward(LP<T> P, const LV<T> &cand, const LV<T> &qd,
+                                            const LV<T> &dd, int i, int stage) {
+    constexpr int NY = RowChunk<T>::v;
+    const int nx = P.nx, nu = P.nu;  // nu <= 32 (checked on the host)
+    const int c0 = cst(P.chFrom)[i], nc = cst(P.numCh)[i];
+    T y0[NY], y1[NY];
+    // e = ubar - sum_j B_j' q_j, stored in dd, then d = R~^-1 e in place
+    zeroY(y0);
+    zeroY(y1);
+    for (int ci = 0; ci < nc; ci++) {
+        const int j = c0 + ci;
+        mv32(y0, y1, tab(P.BTr, j), nu, nu, nx, [&](int cc) { return qd[(long) j * nx + cc]; });
+    }
+    auto storeE = [&](int r, T v) { dd[(long) i * nu + r] = -v + cand[P.oU + (long) i * nu + r]; };
+    each(y0, min(nu, 16), storeE);
+    each(y1, nu - 16, [&](int r, T v) { storeE(16 + r, v); });
+    zeroY(y0);
+    zeroY(y1);
+    mv32(y0, y1, tab(P.RtInv, i), nu, nu, nu, [&](int cc) { return dd[(long) i * nu + cc]; });
+    each(y0, min(nu, 16), [&](int r, T v) { dd[(long) i * nu + r] = v; });
+    each(y1, nu - 16, [&](int r, T v) { dd[(long) i * nu + 16 + r] = v; });
+    if (stage == 0) return;
+    // q_i = sum_j (APB_j d_i + ABKTr_j q_j) + K_i'(d_i - u_i) - x_i
+    T y[NY];
+    for (int r0 = 0; r0 < nx; r0 += NY) {
+        const int rr = min(NY, nx - r0);
+        zeroY(y);
+        for (int ci = 0; ci < nc; ci++) {
+            const int j = c0 + ci;
+            mv(y, tab(P.APB, j) + r0, rr, nx, nu, [&](int cc) { return dd[(long) i * nu + cc]; });
+            mv(y, tab(P.ABKTr, j) + r0, rr, nx, nx, [&](int cc) { return qd[(long) j * nx + cc]; });
+        }
+        mv(y, tab(P.KTr, i) + r0, rr, nx, nu,
+           [&](int cc) { return dd[(long) i * nu + cc] - cand[P.oU + (long) i * nu + cc]; });
+        each(y, rr, [&](int r, T v) { qd[(long) i * nx + r0 + r] = v - cand[P.oX + (long) i * nx + r0 + r]; });
+    }
+}
+
+// Kernel projection for nonleaf node i (src/cache.cuh:650-676); padded child slots enter as 0.
+template <typename T>
+__device__ __forceinline__ void kernelProj(LP<T> P, const LV<T> &cand, int i) {
+    constexpr int NY = RowChunk<T>::v;
+    const int yD = P.yD, E = P.E, nc = cst(P.numCh)[i], c0 = cst(P.chFrom)[i];
+    auto vin = [&](int k) -> T {
+        if (k < yD) return cand[P.oY + (long) i * yD + k];
+        if (k < yD + E) return (k - yD < nc) ? cand[P.oT + c0 + (k - yD)] : T(0);
+        return (k - yD - E < nc) ? cand[P.oS + c0 + (k - yD - E)] : T(0);
+    };
+    T y0[NY], y1[NY];
+    zeroY(y0);
+    zeroY(y1);
+    mv32(y0, y1, tab(P.NN, i), P.nD, P.nD, P.nD, vin);  // nD = 4E+1 <= 32 (checked on the host)
+    auto put = [&](int k, T v) {
+        if (k < yD) cand[P.oY + (long) i * yD + k] = v;
+        else if (k < yD + E) {
+            if (k - yD < nc) cand[P.oT + c0 + (k - yD)] = v;
+        } else if (k - yD - E < nc) {
+            cand[P.oS + c0 + (k - yD - E)] = v;
+        }
+    };
+    each(y0, min(P.nD, 16), put);
+    each(y1, P.nD - 16, [&](int k, T v) { put(16 + k, v); });
+}
+
+// Forward rollout for node i (src/dynamics.cuh:144-173).
+template <typename T>
+__device__ __forceinline__ void dynForward(LP<T> P, const LV<T> &cand, const LV<T> &x0s,
+                                           const LV<T> &dd, int i) {
+    constexpr int NY = RowChunk<T>::v;
+    const int nx = P.nx, nu = P.nu;
+    if (i == 0)
+        for (int k = 0; k < nx; k++) cand[P.oX + k] = x0s[k];
+    T y[NY];
+    for (int r0 = 0; r0 < nu; r0 += NY) {
+        const int rr = min(NY, nu - r0);
+        zeroY(y);
+        mv(y, tab(P.K, i) + r0, rr, nu, nx, [&](int cc) { return cand[P.oX + (long) i * nx + cc]; });
+        each(y, rr, [&](int r, T v) { cand[P.oU + (long) i * nu + r0 + r] = v + dd[(long) i * nu + r0 + r]; });
+    }
+    const int c0 = cst(P.chFrom)[i], nc = cst(P.numCh)[i];
+    for (int ci = 0; ci < nc; ci++) {
+        const int j = c0 + ci;
+        CP<T> AB = tab(P.AB, j);
+        for (int r0 = 0; r0 < nx; r0 += NY) {
+            const int rr = min(NY, nx - r0);
+            zeroY(y);
+            mv(y, AB + r0, rr, nx, nx, [&](int cc) { return cand[P.oX + (long) i * nx + cc]; });
+            mv(y, AB + (size_t) nx * nx + r0, rr, nx, nu, [&](int cc) { return cand[P.oU + (long) i * nu + cc]; });
+            each(y, rr, [&](int r, T v) { cand[P.oX + (long) j * nx + r0 + r] = v; });
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void phaseProj(LP<T> P, const Ctx &c, const LV<T> &cand, const LV<T> &qd,
+                                          const LV<T> &dd, const LV<T> &x0s, bool doDyn, bool doKernel) {
+    if (doDyn) {
+        for (int stage = P.S - 2; stage >= 0; stage--) {
+            const int f = cst(P.stageFrom)[stage], nStage = cst(P.stageTo)[stage] - f + 1;
+            const int extra = (doKernel && stage == P.S - 2) ? P.nl : 0;
+            for (int task = c.wave; task < nStage + extra; task += c.nw) {
+                if (task < nStage) dynBackward(P, cand, qd, dd, f + task, stage);
+                else kernelProj(P, cand, task - nStage);
+            }
+            __syncthreads();
+        }
+        for (int stage = 0; stage <= P.S - 2; stage++) {
+            const int f = cst(P.stageFrom)[stage], nStage = cst(P.stageTo)[stage] - f + 1;
+            for (int task = c.wave; task < nStage; task += c.nw) dynForward(P, cand, x0s, dd, f + task);
+            __syncthreads();
+        }
+    } else if (doKernel) {
+        for (int task = c.wave; task < P.nl; task += c.nw) kernelProj(P, cand, task);
+        __syncthreads();
+    }
+}
+
+// modifyDual + proximalDual (src/cache.cuh:770-789) on 2 xbar - z_p.
+template <typename T>
+__device__ __forceinline__ void phaseDual(LP<T> P, const Ctx &c, const LV<T> &z, const LV<T> &cand) {
+    Prox<T> h{P, z.shift(P.sPrim), cand.shift(P.sPrim), T(0)};
+    auto pv = [&](long e) { return cand[e] * T(2) - z[e]; };
+    for (int n = c.wave; n < P.N; n += c.nw) opNode(P, n, pv, h);
+}
+
+// Device wall-clock per phase, accumulated by thread 0 into LDS (uniform control flow, so
+// thread 0's view is the workgroup's).
+struct LaneClock {
+    long last;
+    long *acc;
+    __device__ __forceinline__ void mark(int ph) {
+        const long t = wall_clock64();
+        if (threadIdx.x == 0) acc[ph] += t - last;
+        last = t;
+    }
+    __device__ __forceinline__ void skip() { last = wall_clock64(); }
+};
+
+// The CP operator T (src/cache.cuh:796-801): cand = T(z).
+template <typename T>
+__device__ __noinline__ void applyT(const LaneInst<T> *Pp, T *z, T *cand, long *phaseAcc) {
+    LaneClock clk{wall_clock64(), phaseAcc};
+    LP<T> P = *cst(uniformPtr(Pp));
+    z = uniformPtr(z);
+    cand = uniformPtr(cand);
+    const Ctx c = ctx();
+    const LV<T> zv = lview(z, P.sIter, c.lane), cv = lview(cand, P.sIter, c.lane);
+    const LV<T> qd = lview(P.qd, (long) P.nx * P.N, c.lane), dd = lview(P.dd, (long) P.nu * P.nl, c.lane);
+    const LV<T> x0s = lview(const_cast<T *>(P.x0s), P.nx, c.lane);
+    phaseAdj(P, c, zv, cv, qd);
+    __syncthreads();
+    clk.mark(PH_ADJ);
+    phaseProj(P, c, cv, qd, dd, x0s, true, true);
+    clk.mark(PH_PROJ);
+    phaseDual(P, c, zv, cv);
+    __syncthreads();
+    clk.mark(PH_DUAL);
+}
+
+// Per-lane combination of per-wave partials through LDS, in wave order.
+__device__ __forceinline__ double laneSum(double v, double *lds, const Ctx &c) {
+    __syncthreads();
+    lds[c.wave * 64 + c.lane] = v;
+    __syncthreads();
+    double s = 0.;
+    for (int w = 0; w < c.nw; w++) s += lds[w * 64 + c.lane];
+    return s;
+}
+
+__device__ __forceinline__ double laneMax(double v, double *lds, const Ctx &c) {
+    __syncthreads();
+    lds[c.wave * 64 + c.lane] = v;
+    __syncthreads();
+    double s = 0.;
+    for (int w = 0; w < c.nw; w++) s = fmax(s, lds[w * 64 + c.lane]);
+    return s;
+}
+
+template <typename T>
+struct DotMax {
+    T dot, mx;
+};
+
+// sum x.(y - gamma ell(y)) and max |y/gamma - ell(y)| per lane.
+template <typename T>
+__device__ __noinline__ DotMax<T> reduceEll(const LaneInst<T> *Pp, T *x, T *y, double *lds) {
+    LP<T> P = *cst(uniformPtr(Pp));
+    x = uniformPtr(x);
+    y = uniformPtr(y);
+    const Ctx c = ctx();
+    const LV<T> xv = lview(x, P.sIter, c.lane), yv = lview(y, P.sIter, c.lane);
+    const LV<T> yd = yv.shift(P.sPrim);
+    Red<T> h{P, xv, yv, T(0), T(0)};
+    T dot = T(0), mx = T(0);
+    auto pv = [&](long e) { return yv[e]; };
+    for (int n = c.wave; n < P.N; n += c.nw) {
+        adjNode(P, yd, n, [&](int kind, int k, T v) {
+            const long e = primIndex(P, n, kind, k);
+            const T yk = yv[e];
+            dot += xv[e] * (v * (-P.gamma) + yk);
+            mx = fmax(mx, fabs(yk * P.gammaRecip - v));
+        });
+        opNode(P, n, pv, h);
+    }
+    DotMax<T> r;
+    r.dot = (T) laneSum((double) (dot + h.dot), lds, c);
+    r.mx = (T) laneMax((double) fmax(mx, h.mx), lds, c);
+    return r;
+}
+
+// Iterate-wide pass: wave w takes a contiguous range of elements, U loads in flight per lane.
+template <int U, typename Ld, typename St>
+__device__ __forceinline__ void pass(long n, const Ctx &c, Ld ld, St st) {
+    using Vals = decltype(ld(0L));
+    const long per = (n + c.nw - 1) / c.nw;
+    const long e0 = (long) c.wave * per, e1 = min(n, e0 + per);
+    long e = e0;
+    for (; e + U <= e1; e += U) {
+        Vals v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = ld(e + u);
+#pragma unroll
+        for (int u = 0; u < U; u++) st(e + u, v[u]);
+    }
+    for (; e < e1; e++) st(e, ld(e));
+}
+
+template <typename T>
+struct V2 {
+    T a, b;
+};
+template <typename T>
+struct V4 {
+    T a, b, c, d;
+};
+template <typename T, int M>
+struct VH {  // two vector values and M history values of one element
+    T a, b, h[M];
+};
+template <typename T, int M>
+struct V4H {
+    T a, b, c, d, h[M];
+};
+
+__device__ __forceinline__ bool anyLane(bool v) { return __syncthreads_or(v ? 1 : 0) != 0; }
+
+// Least squares argmin ||Y g - r|| from the Gram matrix: Cholesky of G + 1e-12 max(diag) I in FP64.
+__device__ __forceinline__ bool gramSolve(const double (&G)[MAXM][MAXM], const double (&gb)[MAXM], int m,
+                                          double (&g)[MAXM]) {
+    double Lm[MAXM][MAXM];
+    double dmax = 0.;
+#pragma unroll
+    for (int i = 0; i < MAXM; i++)
+        if (i < m) dmax = fmax(dmax, G[i][i]);
+    if (!(dmax > 0.)) return false;
+    const double ridge = 1e-12 * dmax;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < MAXM; i++) {
+#pragma unroll
+        for (int j = 0; j < MAXM; j++) {
+            Lm[i][j] = 0.;
+            if (i < m && j <= i) {
+                double v = G[i][j] + (i == j ? ridge : 0.);
+#pragma unroll
+                for (int k = 0; k < MAXM; k++)
+                    if (k < j) v -= Lm[i][k] * Lm[j][k];
+                if (i == j) {
+                    ok = ok && (v > 0.);
+                    Lm[i][i] = sqrt(fmax(v, 0.));
+                } else {
+                    Lm[i][j] = v / Lm[j][j];
+                }
+            }
+        }
+    }
+    if (!ok) return false;
+    double y[MAXM];
+#pragma unroll
+    for (int i = 0; i < MAXM; i++) {
+        y[i] = 0.;
+        if (i < m) {
+            double v = gb[i];
+#pragma unroll
+            for (int k = 0; k < MAXM; k++)
+                if (k < i) v -= Lm[i][k] * y[k];
+            y[i] = v / Lm[i][i];
+        }
+    }
+#pragma unroll
+    for (int i = MAXM - 1; i >= 0; i--) {
+        g[i] = 0.;
+        if (i < m) {
+            double v = y[i];
+#pragma unroll
+            for (int k = 0; k < MAXM; k++)
+                if (k > i && k < m) v -= Lm[k][i] * g[k];
+            g[i] = v / Lm[i][i];
+        }
+    }
+    return true;
+}
+
+}  // namespace lane
+
+// ---------------------------------------------------------------- the lockstep solve
+template <typename T>
+__device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *phaseAcc) {
+    using namespace lane;
+    LP<T> P = *cst(Pp);
+    if (threadIdx.x < NUM_PHASES) phaseAcc[threadIdx.x] = 0;
+    __syncthreads();
+    LaneClock clk{wall_clock64(), phaseAcc};
+    const Ctx c = ctx();
+    const long n = P.sIter, B = P.B;
+    const int gl = c.gl;
+    const bool real = gl < P.count;
+    const int m = P.andBuff;
+    auto V = [&](T *p) { return lview(p, n, c.lane); };
+    T *Z = P.work[0], *C = P.work[1], *ZP = P.work[2], *RR = P.work[3], *RP = P.work[4], *BK = P.work[5];
+    T *C2 = P.work[6], *D = P.work[7], *DS = P.work[8];
+    const long t0 = wall_clock64();
+    // initialiseIter / initialisePrev (src/cache.cuh:611-632)
+    {
+        const LV<T> zio = V(P.zio), rio = V(P.rio), z = V(Z), zp = V(ZP), rp = V(RP);
+        const bool spock = P.method == METHOD_SPOCK, warm = P.warm != 0;
+        pass<16>(
+            n, c, [&](long e) { return V2<T>{zio[e], rio[e]}; },
+            [&](long e, const V2<T> &v) {
+                z[e] = v.a;
+                if (spock) {
+                    zp[e] = warm ? v.a : T(1);
+                    rp[e] = warm ? v.b : T(1);
+                }
+            });
+    }
+    __syncthreads();
+    double G[MAXM][MAXM], gb[MAXM];
+#pragma unroll
+    for (int i = 0; i < MAXM; i++) {
+        gb[i] = 0.;
+#pragma unroll
+        for (int j = 0; j < MAXM; j++) G[i][j] = 0.;
+    }
+    int head = m - 1;
+    bool active = real;
+    const T given = (real && P.tolFixed) ? P.tolFixed[gl] : T(0);
+    bool initFlag = !(given > T(0));
+    int status = ST_NOTRUN;
+    long iters = 0;
+    T tol = initFlag ? T(0) : given, lastErr = T(NAN), wLast = T(NAN);
+    const T c0 = T(0.99), c1 = T(0.99), c2 = T(0.99), beta = T(0.5), sigma = T(0.1), lambda = T(1);
+    const T inf = T(INFINITY);
+    T zeta = inf, w = inf, wSafe = inf;
+    long k0 = 0, k1 = 0, k2 = 0, k2bt = 0, k3 = 0, nT = 0, nEll = 0;
+    // the iteration cap is only tested at error checks (src/cache.cuh:937-940): the last outer
+    // iteration is the first check index at or beyond maxIters
+    const long hardCap =
+        P.maxIters > 0 ? (P.maxIters + P.checkPeriod - 1) / P.checkPeriod * P.checkPeriod + 1 : 2000000;
+    long it;
+    for (it = 0; it < hardCap; it++) {
+        if (!anyLane(active)) break;
+        if (it != 0 && P.method == METHOD_SPOCK) {  // acceptCandidate: z <- cand
+            T *t = Z;
+            Z = C;
+            C = t;
+        }
+        clk.mark(PH_OTHER);
+        applyT(Pp, Z, C, phaseAcc);
+        clk.skip();
+        if (active) nT++;
+        {  // r = z - T(z)
+            const LV<T> rr = V(RR), z = V(Z), cc = V(C);
+            pass<16>(n, c, [&](long e) { return V2<T>{z[e], cc[e]}; }, [&](long e, const V2<T> &v) { rr[e] = v.a - v.b; });
+        }
+        __syncthreads();
+        clk.mark(PH_RES);
+        const DotMax<T> dm = reduceEll(Pp, RR, RR, lds);
+        const T dotRR = dm.dot, mx = dm.mx;
+        if (active) nEll++;
+        clk.mark(PH_ELL);
+        bool done = false;
+        if (active && it % P.checkPeriod == 0) {  // computeError (src/cache.cuh:910-961)
+            if (initFlag) {
+                tol = fmax(P.tolAbs, P.tolRel * mx);
+                initFlag = false;
+            } else {
+                lastErr = mx;
+                int code = ST_NOTRUN;
+                if (mx <= tol) code = ST_CONVERGED;
+                if (P.maxIters && it >= P.maxIters) code = ST_OUTOFITER;
+                if (P.maxTicks > 0 && it % 1000 == 0 && (double) (wall_clock64() - t0) >= P.maxTicks)
+                    code = ST_OUTOFTIME;
+                if (code != ST_NOTRUN) {
+                    status = code;
+                    iters = it;
+                    done = true;
+                }
+            }
+        }
+        if (anyLane(done)) {  // accepted candidate and its residual are the solve's output
+            const LV<T> zio = V(P.zio), rio = V(P.rio), cc = V(C), rr = V(RR);
+            pass<16>(
+                n, c, [&](long e) { return V2<T>{cc[e], rr[e]}; },
+                [&](long e, const V2<T> &v) {
+                    if (done) {
+                        zio[e] = v.a;
+                        rio[e] = v.b;
+                    }
+                });
+            __syncthreads();
+        }
+        if (done) active = false;
+        if (P.method == METHOD_CP) {
+            T *t = Z;
+            Z = C;
+            C = t;
+            continue;
+        }
+        {  // backup (src/cache.cuh:804-806)
+            T *t = BK;
+            BK = C;
+            C = t;
+        }
+        w = sqrt(dotRR);
+        if (active) wLast = w;
+        head = (head + 1) % m;
+        {  // Anderson column (src/cache.cuh:861-896) with the Gram entries of the new column
+            double acc[2 * MAXM];
+#pragma unroll
+            for (int q = 0; q < 2 * MAXM; q++) acc[q] = 0.;
+            const LV<T> Yh = V(P.hY + (long) head * n * B), Sh = V(P.hS + (long) head * n * B);
+            const LV<T> rr = V(RR), rp = V(RP), z = V(Z), zp = V(ZP);
+            LV<T> Ys[MAXM];
+#pragma unroll
+            for (int s2 = 0; s2 < MAXM; s2++) Ys[s2] = V(P.hY + (long) (s2 < m ? s2 : 0) * n * B);
+            pass<4>(
+                n, c,
+                [&](long e) {
+                    V4H<T, MAXM> v;
+                    v.a = rr[e];
+                    v.b = rp[e];
+                    v.c = z[e];
+                    v.d = zp[e];
+#pragma unroll
+                    for (int s2 = 0; s2 < MAXM; s2++) v.h[s2] = (s2 < m && s2 != head) ? Ys[s2][e] : T(0);
+                    return v;
+                },
+                [&](long e, const V4H<T, MAXM> &v) {
+                    const T dr = v.a - v.b;
+                    Yh[e] = dr;
+                    Sh[e] = (v.c - v.d) - dr;
+                    const double drd = (double) dr, rd = (double) v.a;
+#pragma unroll
+                    for (int s2 = 0; s2 < MAXM; s2++) {
+                        if (s2 < m) {
+                            const double ys = (s2 == head) ? drd : (double) v.h[s2];
+                            acc[s2] += drd * ys;
+                            acc[MAXM + s2] += ys * rd;
+                        }
+                    }
+                });
+#pragma unroll
+            for (int q = 0; q < 2 * MAXM; q++)
+                if ((q & (MAXM - 1)) < m) acc[q] = laneSum(acc[q], lds, c);
+#pragma unroll
+            for (int s2 = 0; s2 < MAXM; s2++) {
+                if (s2 < m) {
+#pragma unroll
+                    for (int hh = 0; hh < MAXM; hh++) {
+                        if (hh == head) {
+                            G[hh][s2] = acc[s2];
+                            G[s2][hh] = acc[s2];
+                        }
+                    }
+                    gb[s2] = acc[MAXM + s2];
+                }
+            }
+        }
+        clk.mark(PH_ACOL);
+        {  // saveToPrev as role moves
+            T *t = ZP;
+            ZP = Z;
+            Z = t;
+            t = RP;
+            RP = RR;
+            RR = t;
+        }
+        double gs[MAXM];
+#pragma unroll
+        for (int q = 0; q < MAXM; q++) gs[q] = 0.;
+        const bool useLS = (it >= m) && gramSolve(G, gb, m, gs);
+        clk.mark(PH_ALS);
+        const bool blind = active && (w <= c0 * zeta);
+        {  // d = -S g - r; K0 lanes write cand = z_prev + d (src/cache.cuh:897-898, 1055-1057)
+            T gg[MAXM];
+            LV<T> Ss[MAXM];
+#pragma unroll
+            for (int s2 = 0; s2 < MAXM; s2++) {
+                gg[s2] = (useLS && s2 < m) ? (T) gs[s2] : T(0);
+                Ss[s2] = V(P.hS + (long) (s2 < m ? s2 : 0) * n * B);
+            }
+            const LV<T> rp = V(RP), zp = V(ZP), dd = V(D), cc = V(C);
+            pass<4>(
+                n, c,
+                [&](long e) {
+                    VH<T, MAXM> v;
+                    v.a = rp[e];
+                    v.b = zp[e];
+#pragma unroll
+                    for (int s2 = 0; s2 < MAXM; s2++) v.h[s2] = (s2 < m) ? Ss[s2][e] : T(0);
+                    return v;
+                },
+                [&](long e, const VH<T, MAXM> &v) {
+                    T sg = T(0);
+#pragma unroll
+                    for (int s2 = 0; s2 < MAXM; s2++)
+                        if (s2 < m) sg += v.h[s2] * gg[s2];
+                    const T d = -sg - v.a;
+                    if (blind) cc[e] = v.b + d;
+                    else dd[e] = d;
+                });
+        }
+        __syncthreads();
+        clk.mark(PH_AAPPLY);
+        if (blind) {
+            zeta = w;
+            k0++;
+        }
+        bool ls = active && !blind;
+        T tau = T(1);
+        for (int in = 0; in < P.maxInner; in++) {  // line search (src/cache.cuh:1063-1099)
+            if (!anyLane(ls)) break;
+            {
+                const LV<T> d = V(D), zp = V(ZP), ds = V(DS), z = V(Z);
+                pass<16>(
+                    n, c, [&](long e) { return V2<T>{d[e], zp[e]}; },
+                    [&](long e, const V2<T> &v) {
+                        if (ls) {
+                            const T s = v.a * tau;
+                            ds[e] = s;
+                            z[e] = v.b + s;
+                        }
+                    });
+            }
+            __syncthreads();
+            clk.mark(PH_LINE);
+            applyT(Pp, Z, C2, phaseAcc);
+            clk.skip();
+            if (ls) nT++;
+            {
+                const LV<T> rr = V(RR), z = V(Z), cc = V(C2);
+                pass<16>(n, c, [&](long e) { return V2<T>{z[e], cc[e]}; },
+                        [&](long e, const V2<T> &v) { rr[e] = v.a - v.b; });
+            }
+            __syncthreads();
+            clk.mark(PH_RES);
+            const T wTilde = sqrt(reduceEll(Pp, RR, RR, lds).dot);
+            if (ls) nEll++;
+            clk.mark(PH_ELL);
+            const bool takeK1 = ls && (w <= wSafe) && (wTilde <= c1 * w);
+            if (anyLane(takeK1)) {  // K1: cand <- z
+                const LV<T> cc = V(C), z = V(Z);
+                pass<16>(n, c, [&](long e) { return z[e]; }, [&](long e, T v) {
+                    if (takeK1) cc[e] = v;
+                });
+                __syncthreads();
+            }
+            if (takeK1) {
+                wSafe = wTilde + pow(c2, T(it));
+                k1++;
+            }
+            const bool still = ls && !takeK1;
+            T dotRD = T(0);
+            if (anyLane(still)) {
+                clk.mark(PH_LINE);
+                dotRD = reduceEll(Pp, RR, DS, lds).dot;
+                if (still) nEll++;
+                clk.mark(PH_ELL);
+            }
+            const T rho = wTilde * wTilde - dotRD;
+            const bool takeK2 = still && (rho >= sigma * wTilde * w);
+            if (anyLane(takeK2)) {  // K2: cand <- z_prev - f r
+                const T f = lambda * rho / (wTilde * wTilde);
+                const LV<T> cc = V(C), rr = V(RR), zp = V(ZP);
+                pass<16>(
+                    n, c, [&](long e) { return V2<T>{rr[e], zp[e]}; },
+                    [&](long e, const V2<T> &v) {
+                        if (takeK2) cc[e] = v.b - v.a * f;
+                    });
+                __syncthreads();
+            }
+            if (takeK2) k2++;
+            const bool back = still && !takeK2;
+            if (back) {
+                tau *= beta;
+                k2bt++;
+            }
+            const bool takeK3 = back && (in >= P.maxInner - 1);
+            if (anyLane(takeK3)) {  // K3: restore the plain CP step
+                const LV<T> cc = V(C), bk = V(BK);
+                pass<16>(n, c, [&](long e) { return bk[e]; }, [&](long e, T v) {
+                    if (takeK3) cc[e] = v;
+                });
+                __syncthreads();
+            }
+            if (takeK3) k3++;
+            ls = back;
+            clk.mark(PH_LINE);
+        }
+    }
+    const bool capped = active;
+    if (anyLane(capped)) {  // iteration cap of the launch: the current iterate is the output
+        const LV<T> zio = V(P.zio), rio = V(P.rio), z = V(Z), rr = V(RR);
+        pass<16>(
+            n, c, [&](long e) { return V2<T>{z[e], rr[e]}; },
+            [&](long e, const V2<T> &v) {
+                if (capped) {
+                    zio[e] = v.a;
+                    rio[e] = v.b;
+                }
+            });
+    }
+    if (capped) {
+        status = ST_OUTOFITER;
+        iters = it;
+    }
+    __syncthreads();
+    if (c.wave == 0 && real) {
+        Out &o = outs[gl];
+        o.status = status;
+        o.iters = iters;
+        o.err = (double) lastErr;
+        o.tol = (double) tol;
+        o.w = (double) wLast;
+        o.k0 = k0;
+        o.k1 = k1;
+        o.k2 = k2;
+        o.k2bt = k2bt;
+        o.k3 = k3;
+        o.nT = nT;
+        o.nEll = nEll;
+        for (int i = 0; i < NUM_PHASES; i++) o.ticks[i] = phaseAcc[i];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) k_lane_solve(const LaneInst<T> *__restrict__ P, Out *outs) {
+    __shared__ double lds[16 * 64];
+    __shared__ long phaseAcc[NUM_PHASES];
+    laneSolve(P, outs, lds, phaseAcc);
+}
+
+// Single operators on every instance (parity tests): z = zio, cand = work[1].
+template <typename T>
+__global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) k_lane_debug(const LaneInst<T> *__restrict__ Pp, Out *outs,
+                                                                     int op) {
+    using namespace lane;
+    __shared__ double lds[16 * 64];
+    LP<T> P = *cst(Pp);
+    const Ctx c = ctx();
+    T *z = P.zio, *cand = P.work[1];
+    const LV<T> zv = lview(z, P.sIter, c.lane), cv = lview(cand, P.sIter, c.lane);
+    if (op == 0) {  // L
+        Store<T> h{P, cv.shift(P.sPrim)};
+        auto pv = [&](long e) { return zv[e]; };
+        for (int n = c.wave; n < P.N; n += c.nw) opNode(P, n, pv, h);
+    } else if (op == 1) {  // L*
+        const LV<T> zd = zv.shift(P.sPrim);
+        for (int n = c.wave; n < P.N; n += c.nw)
+            adjNode(P, zd, n, [&](int kind, int k, T v) { cv[primIndex(P, n, kind, k)] = v; });
+    } else if (op == 2 || op == 3) {  // dynamics / kernel projection of the primal part
+        pass<16>(P.sPrim, c, [&](long e) { return zv[e]; }, [&](long e, T v) { cv[e] = v; });
+        __syncthreads();
+        const LV<T> qd = lview(P.qd, (long) P.nx * P.N, c.lane), dd = lview(P.dd, (long) P.nu * P.nl, c.lane);
+        const LV<T> x0s = lview(const_cast<T *>(P.x0s), P.nx, c.lane);
+        if (op == 2)
+            for (int n = c.wave; n < P.N; n += c.nw)
+                if (n >= P.nl)
+                    for (int r = 0; r < P.nx; r++) qd[(long) n * P.nx + r] = -cv[P.oX + (long) n * P.nx + r];
+        __syncthreads();
+        phaseProj(P, c, cv, qd, dd, x0s, op == 2, op == 3);
+    } else if (op == 4) {  // T
+        __shared__ long phaseAcc[NUM_PHASES];
+        applyT(Pp, z, cand, phaseAcc);
+    } else if (op == 5) {  // ell reductions
+        const DotMax<T> r = reduceEll(Pp, z, cand, lds);
+        if (c.wave == 0 && c.gl < P.count) {
+            outs[c.gl].w = (double) r.dot;
+            outs[c.gl].err = (double) r.mx;
+        }
+    }
+}
+
+// [instance][element] rows (leading dimension ld) to the block-interleaved layout of a region of
+// n rows and back, with dtype conversion.
+template <typename S, typename D>
+__global__ void k_interleave(const S *__restrict__ in, long ld, D *__restrict__ out, long n, int count, long) {
+    const long total = n * (long) count;
+    for (long t = blockIdx.x * (long) blockDim.x + threadIdx.x; t < total; t += (long) gridDim.x * blockDim.x) {
+        const long b = t % count, e = t / count;
+        out[laneIndex(n, e, b)] = (D) in[b * ld + e];
+    }
+}
+
+template <typename S, typename D>
+__global__ void k_deinterleave(const S *__restrict__ in, long, D *__restrict__ out, long ld, long n, int count) {
+    const long total = n * (long) count;
+    for (long t = blockIdx.x * (long) blockDim.x + threadIdx.x; t < total; t += (long) gridDim.x * blockDim.x) {
+        const long e = t % n, b = t / n;
+        out[b * ld + e] = (D) in[laneIndex(n, e, b)];
+    }
+}
+
+template __global__ void k_lane_solve<float>(const LaneInst<float> *, Out *);
+template __global__ void k_lane_solve<double>(const LaneInst<double> *, Out *);
+template __global__ void k_lane_debug<float>(const LaneInst<float> *, Out *, int);
+template __global__ void k_lane_debug<double>(const LaneInst<double> *, Out *, int);
+template __global__ void k_interleave<float, float>(const float *, long, float *, long, int, long);
+template __global__ void k_interleave<float, double>(const float *, long, double *, long, int, long);
+template __global__ void k_interleave<double, float>(const double *, long, float *, long, int, long);
+template __global__ void k_interleave<double, double>(const double *, long, double *, long, int, long);
+template __global__ void k_deinterleave<float, float>(const float *, long, float *, long, long, int);
+template __global__ void k_deinterleave<float, double>(const float *, long, double *, long, long, int);
+template __global__ void k_deinterleave<double, float>(const double *, long, float *, long, long, int);
+template __global__ void k_deinterleave<double, double>(const double *, long, double *, long, long, int);
+
+}  // namespace spk

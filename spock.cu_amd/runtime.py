@@ -20,8 +20,8 @@ STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
 EXPORTS = [
     "spk_default_options", "spk_ctx_create", "spk_ctx_destroy", "spk_last_error", "spk_ctx_stream",
     "spk_ctx_synchronize", "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
-    "spk_problem_step_size", "spk_batch_create", "spk_batch_destroy", "spk_batch_count", "spk_batch_reset",
-    "spk_batch_set_x0", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
+    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_reset",
+    "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
     "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve",
     "spk_batch_results", "spk_batch_debug_op",
 ]
@@ -81,10 +81,13 @@ def load_library(path=LIB_PATH):
     lib.spk_problem_step_size.argtypes = [vp]
     lib.spk_problem_step_size.restype = C.c_double
     lib.spk_batch_create.argtypes = [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    lib.spk_batch_create_shared.argtypes = [vp, vp, C.c_int, C.c_int, C.POINTER(vp)]
+    lib.spk_batch_is_shared.argtypes = [vp]
     lib.spk_batch_destroy.argtypes = [vp]
     lib.spk_batch_count.argtypes = [vp]
     lib.spk_batch_reset.argtypes = [vp]
     lib.spk_batch_set_x0.argtypes = [vp, _dp]
+    lib.spk_batch_set_tolerances.argtypes = [vp, _dp]
     lib.spk_batch_set_iterate.argtypes = [vp, C.c_int, _dp]
     lib.spk_batch_get_iterate.argtypes = [vp, C.c_int, _dp]
     lib.spk_batch_set_iterates.argtypes = [vp, _dp]
@@ -164,15 +167,26 @@ class Problem:
 
 
 class Batch:
-    """A set of solve instances (``spk_batch``); one instance = one Cache of the reference."""
+    """A set of solve instances (``spk_batch``); one instance = one Cache of the reference.
 
-    def __init__(self, ctx, problems, dtype=F64):
+    ``shared=True`` (all instances of one problem) selects the instance-interleaved layout
+    solved by the lane engine (``spk_batch_create_shared``)."""
+
+    def __init__(self, ctx, problems, dtype=F64, shared=False):
         self.ctx = ctx
         self.problems = list(problems)
         self.dtype = dtype
-        arr = (C.c_void_p * len(self.problems))(*[p.h for p in self.problems])
+        self.shared = bool(shared)
         h = C.c_void_p()
-        ctx.check(ctx.lib.spk_batch_create(ctx.h, len(self.problems), arr, dtype, C.byref(h)), "batch create")
+        if self.shared:
+            p0 = self.problems[0]
+            if any(p is not p0 for p in self.problems):
+                raise SpockError("a shared batch holds instances of one problem")
+            ctx.check(ctx.lib.spk_batch_create_shared(ctx.h, p0.h, len(self.problems), dtype, C.byref(h)),
+                      "shared batch create")
+        else:
+            arr = (C.c_void_p * len(self.problems))(*[p.h for p in self.problems])
+            ctx.check(ctx.lib.spk_batch_create(ctx.h, len(self.problems), arr, dtype, C.byref(h)), "batch create")
         self.h = h
         self.sizes = [p.n_iter for p in self.problems]
         self.total = int(sum(self.sizes))
@@ -192,6 +206,15 @@ class Batch:
         x0 = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in x0])
                                   if isinstance(x0, (list, tuple)) else np.asarray(x0, dtype=np.float64).ravel())
         self.ctx.check(self.ctx.lib.spk_batch_set_x0(self.h, _d(x0)), "set_x0")
+
+    def set_tolerances(self, tol):
+        """Per-instance acceptance tolerances (scaled units, as ``results()[k]['tol']``); None clears."""
+        if tol is None:
+            self.ctx.check(self.ctx.lib.spk_batch_set_tolerances(self.h, None), "set_tolerances")
+            return
+        t = np.ascontiguousarray(np.asarray(tol, dtype=np.float64).ravel())
+        assert t.size == len(self.problems)
+        self.ctx.check(self.ctx.lib.spk_batch_set_tolerances(self.h, _d(t)), "set_tolerances")
 
     def set_iterates(self, z):
         z = np.ascontiguousarray(np.asarray(z, dtype=np.float64).ravel())
