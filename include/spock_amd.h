@@ -37,6 +37,7 @@ extern "C" {
 typedef struct spk_ctx spk_ctx;
 typedef struct spk_problem spk_problem;
 typedef struct spk_batch spk_batch;
+typedef struct spk_mlp spk_mlp;
 
 enum spk_dtype { SPK_F64 = 0, SPK_F32 = 1 };
 enum spk_method { SPK_SPOCK = 0, SPK_CP = 1 };
@@ -129,6 +130,10 @@ int spk_batch_get_iterate(spk_batch *b, int k, double *z);
 /* all iterates at once: host arrays concatenated over instances */
 int spk_batch_set_iterates(spk_batch *b, const double *z);
 int spk_batch_get_iterates(spk_batch *b, double *z);
+/* the accepted iterates as device rows: FP32, instance-major, leading dimension ld >= size_iterate
+ * (all instances of one problem size).  set also zeroes the residual history, as a warm start. */
+int spk_batch_set_iterates_device(spk_batch *b, const float *src, long ld);
+int spk_batch_get_iterates_device(spk_batch *b, float *dst, long ld);
 /* device pointer to instance k's accepted iterate (dtype of the batch); contiguous for
  * per-instance batches, stride 64 elements for shared batches */
 void *spk_batch_iterate_ptr(spk_batch *b, int k);
@@ -144,6 +149,36 @@ int spk_batch_results(spk_batch *b, spk_result *out, double *kernel_seconds);
  * scalars[2k] = <in, M out_in> and scalars[2k+1] = max|out_in/gamma - ell(out_in)| where out_in
  * is taken from `out` on entry. */
 int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, double *scalars);
+
+/* ------------------------------------------------------------------ learned warm start
+ * An MLP x0 -> primal-dual iterate (no counterpart in the reference, whose solves start cold or
+ * from a caller-given previous solution, src/cache.cuh:611-632).  in -> `layers` hidden layers of
+ * `hidden` units with GELU -> out; bf16 weights feed the matrix cores with FP32 accumulation and FP32
+ * master weights take the Adam steps.  Outputs are standardized per component (mean and standard
+ * deviation of the training targets); spk_mlp_predict writes mean + std * output into a batch as its
+ * warm-start iterates.  Parameters are concatenated per layer: W (d_in x d_out, row-major), then b. */
+int spk_mlp_create(spk_ctx *ctx, int in, int hidden, int layers, int out, unsigned long long seed, spk_mlp **out_mlp);
+int spk_mlp_destroy(spk_mlp *m);
+long spk_mlp_num_params(const spk_mlp *m);
+int spk_mlp_get_params(spk_mlp *m, float *params);
+int spk_mlp_set_params(spk_mlp *m, const float *params);
+int spk_mlp_get_grads(spk_mlp *m, float *grads);
+int spk_mlp_get_normalization(spk_mlp *m, float *in_mean, float *in_std, float *out_mean, float *out_std);
+int spk_mlp_set_normalization(spk_mlp *m, const float *in_mean, const float *in_std, const float *out_mean,
+                              const float *out_std);
+/* forward of host rows (raw inputs, normalized inside); y (nullable) gets the standardized outputs */
+int spk_mlp_forward(spk_mlp *m, const float *x, int rows, float *y);
+/* after spk_mlp_forward of the same rows: mean squared error against standardized targets (host),
+ * gradients of every parameter into the gradient buffer */
+int spk_mlp_loss_backward(spk_mlp *m, const float *target, int rows, double *loss);
+int spk_mlp_adam(spk_mlp *m, float lr, float beta1, float beta2, float eps);
+/* device-resident training set from a solved batch (targets = its accepted iterates, standardized)
+ * and the x0 of its instances (host, count x in); sets the normalization */
+int spk_mlp_dataset(spk_mlp *m, spk_batch *solved, const double *x0);
+/* epochs of shuffled minibatch Adam on the training set; epoch_loss (nullable) gets the mean loss */
+int spk_mlp_train(spk_mlp *m, int epochs, int minibatch, float lr, unsigned long long seed, double *epoch_loss);
+/* warm-start iterates of `rows` initial states (host, rows x in) written into dst (rows = its count) */
+int spk_mlp_predict(spk_mlp *m, const double *x0, int rows, spk_batch *dst);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@
 #include "../../include/spock_amd.h"
 #include "engine.hpp"
 #include "loader.hpp"
+#include "common.hpp"
 
 namespace spk {
 template <typename T, int R, int V>
@@ -37,42 +38,11 @@ __global__ void k_deinterleave(const S *, long, D *, long, long, int);
 using spk::HostProblem;
 using spk::Table;
 
-struct spk_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    double tickRateHz = 1e8;
-};
-
 namespace {
 
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void check(hipError_t e, const char *what) {
-    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t b) {
-        release();
-        if (b == 0) b = 16;
-        check(hipMalloc(&p, b), "hipMalloc");
-        bytes = b;
-    }
-};
+using spk::check;
+using spk::DevBuf;
+using spk::fail;
 
 template <typename T>
 struct DevTableSet {  // device copies of one table: data as T, index as int
@@ -293,11 +263,6 @@ void toHost(spk_batch *b, double *dst, const T *src, size_t n) {
 template <typename T>
 T *corePtr(spk_batch *b, int k) {
     return static_cast<T *>(b->core.p) + b->coreOff[k];
-}
-
-int fail(spk_ctx *ctx, const std::exception &e) {
-    if (ctx) ctx->err = e.what();
-    return -1;
 }
 
 void validateProblem(const HostProblem &h) {
@@ -837,6 +802,80 @@ int spk_batch_get_iterates(spk_batch *b, double *z) {
         pos += b->probs[k]->hp.sIter;
     }
     return 0;
+}
+
+// Device rows (instance-major, FP32, leading dimension ld) <-> the batch's accepted iterates.
+int spk_batch_set_iterates_device(spk_batch *b, const float *src, long ld) {
+    try {
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        const size_t n = b->probs[0]->hp.sIter;
+        for (int k = 0; k < b->count; k++)
+            if ((size_t) b->probs[k]->hp.sIter != n) throw std::runtime_error("device rows need one problem size");
+        if (ld < (long) n) throw std::runtime_error("leading dimension smaller than the iterate");
+        hipStream_t st = b->ctx->stream;
+        if (b->lanes) {
+            if (b->dtype == SPK_F64) {
+                hipLaunchKernelGGL((spk::k_interleave<float, double>), dim3(1024), dim3(256), 0, st, src, ld,
+                                   lanePtr<double>(b, LR_ZIO), (long) n, b->count, b->Bpad);
+                fillDevice<double>(b, lanePtr<double>(b, LR_RIO), n * b->Bpad);
+            } else {
+                hipLaunchKernelGGL((spk::k_interleave<float, float>), dim3(1024), dim3(256), 0, st, src, ld,
+                                   lanePtr<float>(b, LR_ZIO), (long) n, b->count, b->Bpad);
+                fillDevice<float>(b, lanePtr<float>(b, LR_RIO), n * b->Bpad);
+            }
+        } else {
+            for (int k = 0; k < b->count; k++) {
+                if (b->dtype == SPK_F64) {
+                    hipLaunchKernelGGL(spk::k_f32_to_f64, dim3(64), dim3(256), 0, st, src + (size_t) k * ld,
+                                       corePtr<double>(b, k), (long) n);
+                    fillDevice<double>(b, corePtr<double>(b, k) + 4 * n, n);
+                } else {
+                    check(hipMemcpyAsync(corePtr<float>(b, k), src + (size_t) k * ld, n * sizeof(float),
+                                         hipMemcpyDeviceToDevice, st), "copy");
+                    fillDevice<float>(b, corePtr<float>(b, k) + 4 * n, n);
+                }
+            }
+        }
+        check(hipGetLastError(), "set iterates (device)");
+        check(hipStreamSynchronize(st), "sync");
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_get_iterates_device(spk_batch *b, float *dst, long ld) {
+    try {
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        const size_t n = b->probs[0]->hp.sIter;
+        for (int k = 0; k < b->count; k++)
+            if ((size_t) b->probs[k]->hp.sIter != n) throw std::runtime_error("device rows need one problem size");
+        if (ld < (long) n) throw std::runtime_error("leading dimension smaller than the iterate");
+        hipStream_t st = b->ctx->stream;
+        check(hipStreamSynchronize(st), "sync");
+        if (b->lanes) {
+            if (b->dtype == SPK_F64)
+                hipLaunchKernelGGL((spk::k_deinterleave<double, float>), dim3(1024), dim3(256), 0, st,
+                                   lanePtr<double>(b, LR_ZIO), b->Bpad, dst, ld, (long) n, b->count);
+            else
+                hipLaunchKernelGGL((spk::k_deinterleave<float, float>), dim3(1024), dim3(256), 0, st,
+                                   lanePtr<float>(b, LR_ZIO), b->Bpad, dst, ld, (long) n, b->count);
+        } else {
+            for (int k = 0; k < b->count; k++) {
+                if (b->dtype == SPK_F64)
+                    hipLaunchKernelGGL(spk::k_f64_to_f32, dim3(64), dim3(256), 0, st, corePtr<double>(b, k),
+                                       dst + (size_t) k * ld, (long) n);
+                else
+                    check(hipMemcpyAsync(dst + (size_t) k * ld, corePtr<float>(b, k), n * sizeof(float),
+                                         hipMemcpyDeviceToDevice, st), "copy");
+            }
+        }
+        check(hipGetLastError(), "get iterates (device)");
+        check(hipStreamSynchronize(st), "sync");
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
 }
 
 void *spk_batch_iterate_ptr(spk_batch *b, int k) {

@@ -23,7 +23,10 @@ EXPORTS = [
     "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_reset",
     "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
     "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve",
-    "spk_batch_results", "spk_batch_debug_op",
+    "spk_batch_results", "spk_batch_debug_op", "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
+    "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
+    "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
+    "spk_mlp_loss_backward", "spk_mlp_adam", "spk_mlp_dataset", "spk_mlp_train", "spk_mlp_predict",
 ]
 
 
@@ -98,6 +101,23 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_solve.argtypes = [vp, C.POINTER(Options), C.c_int]
     lib.spk_batch_results.argtypes = [vp, C.POINTER(Result), _dp]
     lib.spk_batch_debug_op.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    lib.spk_batch_set_iterates_device.argtypes = [vp, vp, C.c_long]
+    lib.spk_batch_get_iterates_device.argtypes = [vp, vp, C.c_long]
+    _fp = C.POINTER(C.c_float)
+    lib.spk_mlp_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)]
+    lib.spk_mlp_destroy.argtypes = [vp]
+    lib.spk_mlp_num_params.argtypes = [vp]
+    lib.spk_mlp_num_params.restype = C.c_long
+    for name in ("spk_mlp_get_params", "spk_mlp_set_params", "spk_mlp_get_grads"):
+        getattr(lib, name).argtypes = [vp, _fp]
+    lib.spk_mlp_get_normalization.argtypes = [vp, _fp, _fp, _fp, _fp]
+    lib.spk_mlp_set_normalization.argtypes = [vp, _fp, _fp, _fp, _fp]
+    lib.spk_mlp_forward.argtypes = [vp, _fp, C.c_int, _fp]
+    lib.spk_mlp_loss_backward.argtypes = [vp, _fp, C.c_int, _dp]
+    lib.spk_mlp_adam.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    lib.spk_mlp_dataset.argtypes = [vp, vp, _dp]
+    lib.spk_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _dp]
+    lib.spk_mlp_predict.argtypes = [vp, _dp, C.c_int, vp]
     _LIB = lib
     return lib
 
@@ -231,6 +251,13 @@ class Batch:
         self.ctx.check(self.ctx.lib.spk_batch_get_iterate(self.h, k, _d(z)), "get_iterate")
         return z
 
+    def set_iterates_device(self, ptr, ld):
+        """Iterates from device rows (FP32, instance-major, leading dimension ld)."""
+        self.ctx.check(self.ctx.lib.spk_batch_set_iterates_device(self.h, C.c_void_p(ptr), ld), "set_iterates_device")
+
+    def get_iterates_device(self, ptr, ld):
+        self.ctx.check(self.ctx.lib.spk_batch_get_iterates_device(self.h, C.c_void_p(ptr), ld), "get_iterates_device")
+
     def copy_iterates_from(self, other):
         self.ctx.check(self.ctx.lib.spk_batch_copy_iterates(self.h, other.h), "copy_iterates")
 
@@ -250,3 +277,96 @@ class Batch:
         sc = np.zeros(2 * len(self.problems))
         self.ctx.check(self.ctx.lib.spk_batch_debug_op(self.h, op, _d(inp), _d(o), _d(sc)), "debug_op")
         return o, sc
+
+
+_fp = C.POINTER(C.c_float)
+
+
+def _f(a):
+    return a.ctypes.data_as(_fp)
+
+
+class WarmStartMLP:
+    """The learned warm start (``spk_mlp``): x0 -> primal-dual iterate, trained on solved iterates."""
+
+    def __init__(self, ctx, n_in, n_out, hidden=512, layers=3, seed=0):
+        self.ctx = ctx
+        self.n_in, self.n_out, self.hidden, self.layers = n_in, n_out, hidden, layers
+        h = C.c_void_p()
+        ctx.check(ctx.lib.spk_mlp_create(ctx.h, n_in, hidden, layers, n_out, seed, C.byref(h)), "mlp create")
+        self.h = h
+        self.n_params = int(ctx.lib.spk_mlp_num_params(h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.spk_mlp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.spk_mlp_get_params(self.h, _f(p)), "mlp params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self.ctx.check(self.ctx.lib.spk_mlp_set_params(self.h, _f(p)), "mlp set params")
+
+    def grads(self):
+        g = np.zeros(self.n_params, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.spk_mlp_get_grads(self.h, _f(g)), "mlp grads")
+        return g
+
+    def normalization(self):
+        a, b = np.zeros(self.n_in, np.float32), np.zeros(self.n_in, np.float32)
+        c, d = np.zeros(self.n_out, np.float32), np.zeros(self.n_out, np.float32)
+        self.ctx.check(self.ctx.lib.spk_mlp_get_normalization(self.h, _f(a), _f(b), _f(c), _f(d)), "mlp norm")
+        return a, b, c, d
+
+    def set_normalization(self, in_mean, in_std, out_mean, out_std):
+        arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in (in_mean, in_std, out_mean, out_std)]
+        self.ctx.check(self.ctx.lib.spk_mlp_set_normalization(self.h, *[_f(v) for v in arrs]), "mlp set norm")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
+        self.ctx.check(self.ctx.lib.spk_mlp_forward(self.h, _f(x), x.shape[0], _f(y)), "mlp forward")
+        return y
+
+    def loss_backward(self, target):
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = C.c_double()
+        self.ctx.check(self.ctx.lib.spk_mlp_loss_backward(self.h, _f(t), t.shape[0], C.byref(loss)), "mlp backward")
+        return loss.value
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.ctx.check(self.ctx.lib.spk_mlp_adam(self.h, lr, beta1, beta2, eps), "mlp adam")
+
+    def dataset(self, solved_batch, x0):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.spk_mlp_dataset(self.h, solved_batch.h, _d(x0)), "mlp dataset")
+
+    def train(self, epochs, minibatch=256, lr=1e-3, seed=0):
+        losses = np.zeros(epochs)
+        self.ctx.check(self.ctx.lib.spk_mlp_train(self.h, epochs, minibatch, lr, seed, _d(losses)), "mlp train")
+        return losses
+
+    def predict(self, x0, batch):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.spk_mlp_predict(self.h, _d(x0), x0.shape[0], batch.h), "mlp predict")
+
+    def save(self, path):
+        a, b, c, d = self.normalization()
+        np.savez(path, params=self.params(), in_mean=a, in_std=b, out_mean=c, out_std=d,
+                 dims=np.array([self.n_in, self.hidden, self.layers, self.n_out]))
+
+    def load(self, path):
+        z = np.load(path, allow_pickle=False)
+        dims = [int(v) for v in z["dims"]]
+        if dims != [self.n_in, self.hidden, self.layers, self.n_out]:
+            raise SpockError(f"{path}: network dims {dims} do not match this model")
+        self.set_params(z["params"])
+        self.set_normalization(z["in_mean"], z["in_std"], z["out_mean"], z["out_std"])

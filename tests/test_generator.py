@@ -1,0 +1,175 @@
+"""The problem generator (spock.cu_amd/api) against the known answers of the reference's own tests:
+tests/python/testTreeFactory.py (Markov chain and i.i.d. trees), tests/cuda/testTree.cu (the tree of
+tests/cuda/test.py), and the AV@R data against the textbook definition of the risk.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.optimize import linprog
+
+from spock_amd.api import btio, build, configs
+from spock_amd.api.tree import IidProcess, MarkovChain
+
+
+@pytest.fixture(scope="module")
+def markov():
+    p = np.array([[0.1, 0.8, 0.1], [0.4, 0.6, 0], [0, 0.3, 0.7]])
+    v = np.array([0.5, 0.5, 0.0])
+    return MarkovChain(p, v, 4, 3).build()
+
+
+@pytest.fixture(scope="module")
+def iid():
+    return IidProcess(np.array([0.1, 0.2, 0.7]), 4, 2).build()
+
+
+# tests/python/testTreeFactory.py:79-246
+def test_markov_structure(markov):
+    t = markov
+    assert (t.num_nodes, t.num_nonleaf_nodes, t.num_events, t.num_stages) == (32, 20, 3, 5)
+    for node, anc in [(1, 0), (2, 0), (3, 1), (4, 1), (5, 1), (6, 2), (7, 2), (8, 3), (9, 3), (10, 3), (13, 5)]:
+        assert t.ancestor_of_node(node) == anc
+    for i in range(12):
+        assert t.ancestor_of_node(20 + i) == 8 + i
+    for node, n in [(0, 2), (1, 3), (2, 2), (5, 2), (6, 3)]:
+        assert len(t.children_of_node(node)) == n
+    for idx in range(8, 20):
+        assert len(t.children_of_node(idx)) == 1
+    with pytest.raises(IndexError):
+        t.children_of_node(20)
+    assert t.stage_of_node(0) == 0
+    assert [t.stage_of_node(i) for i in (1, 2)] == [1, 1]
+    assert all(t.stage_of_node(i) == 2 for i in range(3, 8))
+    assert all(t.stage_of_node(i) == 3 for i in range(8, 20))
+    assert all(t.stage_of_node(i) == 4 for i in range(20, 32))
+    with pytest.raises(ValueError):
+        t.stage_of_node(-1)
+    with pytest.raises(IndexError):
+        t.stage_of_node(32)
+    assert [len(t.nodes_of_stage(s)) for s in range(5)] == [1, 2, 5, 12, 12]
+    assert list(t.nodes_of_stage(2)) == list(range(3, 8))
+    assert list(t.nodes_of_stage(4)) == list(range(20, 32))
+
+
+def test_markov_probabilities_and_events(markov):
+    t = markov
+    expect = {0: 1.0, 1: 0.5, 2: 0.5, 3: 0.05, 4: 0.4, 5: 0.05, 6: 0.2, 7: 0.3, 8: 0.005, 20: 0.005,
+              29: 0.5 * 0.4 * 0.1}
+    for node, pr in expect.items():
+        assert t.probability_of_node(node) == pytest.approx(pr, abs=1e-10)
+    assert [len(t.siblings_of_node(i)) for i in (0, 1, 2, 3, 7, 11)] == [1, 2, 2, 3, 2, 2]
+    assert all(len(t.siblings_of_node(i)) == 1 for i in range(20, 32))
+    assert list(t.event_of_node(range(1, 3))) == [0, 1]
+    assert list(t.event_of_node(range(3, 8))) == [0, 1, 2, 0, 1]
+    assert list(t.event_of_node(range(8, 20))) == [0, 1, 2, 0, 1, 1, 2, 0, 1, 2, 0, 1]
+    assert list(t.event_of_node(range(8, 20))) == list(t.event_of_node(range(20, 32)))
+    for stage in range(t.num_stages - 1):
+        for node in t.nodes_of_stage(stage):
+            assert sum(t.cond_prob_of_children_of_node(node)) == pytest.approx(1.0, abs=1e-10)
+
+
+def test_markov_large_tree_and_stopping_stage():
+    rng = np.random.default_rng(0)
+    p = rng.random((4, 4))
+    p /= p.sum(1, keepdims=True)
+    v = rng.random(4)
+    v /= v.sum()
+    t = MarkovChain(p, v, 20, 5).build()
+    for stage in range(t.num_stages - 1):
+        for node in t.nodes_of_stage(stage):
+            assert sum(t.cond_prob_of_children_of_node(node)) == pytest.approx(1.0, abs=1e-10)
+    with pytest.raises(ValueError):
+        MarkovChain(p[:3, :3] / p[:3, :3].sum(1, keepdims=True), v[:3] / v[:3].sum(), 4, 5).build()
+    q = np.array([[0.1, 0.8, 0.1], [0.4, 0.6, 0], [0, 0.3, 0.7]])
+    MarkovChain(q, np.array([0.5, 0.4, 0.1]), 1, 1).build()
+    MarkovChain(q, np.array([0.5, 0.4, 0.1]), 3, 1).build()
+
+
+# tests/python/testTreeFactory.py:248-400
+def test_iid_tree(iid):
+    t = iid
+    assert (t.num_nodes, t.num_nonleaf_nodes, t.num_events, t.num_stages) == (31, 22, 3, 5)
+    for node, anc in [(1, 0), (2, 0), (3, 0), (4, 1), (5, 1), (6, 1), (7, 2), (8, 2), (9, 2), (10, 3), (11, 3), (12, 3)]:
+        assert t.ancestor_of_node(node) == anc
+    for i in range(13, t.num_nodes):
+        assert t.ancestor_of_node(i) == i - 9
+    assert [len(t.children_of_node(i)) for i in range(4)] == [3, 3, 3, 3]
+    assert all(len(t.children_of_node(i)) == 1 for i in range(4, t.num_nonleaf_nodes))
+    with pytest.raises(IndexError):
+        t.children_of_node(22)
+    assert [len(t.nodes_of_stage(s)) for s in range(5)] == [1, 3, 9, 9, 9]
+    probs = [1.0, .1, .2, .7, .01, .02, .07, .02, .04, .14, .07, .14, .49]
+    for node, pr in enumerate(probs):
+        assert t.probability_of_node(node) == pytest.approx(pr, abs=1e-10)
+    for idx in range(13, t.num_nodes):
+        assert t.probability_of_node(idx) == pytest.approx(t.probability_of_node(t.ancestor_of_node(idx)), abs=1e-10)
+    assert all(len(t.siblings_of_node(i)) == 3 for i in range(1, 13))
+    assert list(t.event_of_node(range(4, 13))) == [0, 1, 2] * 3
+    assert list(t.event_of_node(range(4, 13))) == list(t.event_of_node(range(22, 31)))
+    with pytest.raises(ValueError):
+        IidProcess(np.array([0.2, 0.3, 0.5]), 4, 5).build()
+    IidProcess(np.array([0.5, 0.4, 0.1]), 1, 1).build()
+
+
+def test_fixture_tree_files(tmp_path):
+    """tests/cuda/testTree.cu:42-176 on the files written for the tree of tests/cuda/test.py."""
+    d = str(tmp_path / "fixture")
+    configs.generate("fixture", d)
+    rd = lambda name: btio.read_flat(os.path.join(d, name))
+    assert len(rd("stages_u.bt")) == 11
+    assert int(rd("numChildren_u.bt").size) == 7
+    assert int(rd("stages_u.bt").max()) + 1 == 4
+    assert int(rd("stages_u.bt")[5]) == 2
+    assert int(rd("ancestors_u.bt")[5]) == 2
+    assert rd("probabilities_d.bt")[4] == pytest.approx(0.21)
+    assert rd("conditionalProbabilities_d.bt")[5] == pytest.approx(0.3)
+    assert int(rd("events_u.bt")[4]) == 1
+    assert int(rd("childrenFrom_u.bt")[2]) == 5
+    assert int(rd("childrenTo_u.bt")[2]) == 6
+    assert int(rd("numChildren_u.bt")[2]) == 2
+    assert int(rd("stageFrom_u.bt")[2]) == 3
+    assert int(rd("stageTo_u.bt")[2]) == 6
+
+
+def test_bt_round_trip(tmp_path):
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((3, 4, 5))
+    btio.write_bt(str(tmp_path / "a_d.bt"), a, "d")
+    assert np.array_equal(btio.read_bt(str(tmp_path / "a_d.bt")), a)
+    u = rng.integers(0, 1000, 17).astype(np.int64)
+    btio.write_bt(str(tmp_path / "u_u.bt"), u, "u")
+    assert np.array_equal(btio.read_flat(str(tmp_path / "u_u.bt")).astype(np.int64), u)
+
+
+@pytest.mark.parametrize("alpha", [0.9, 0.95, 0.5])
+def test_avar_data_gives_the_textbook_risk(alpha):
+    """min b'y s.t. E'y = Z, y_1..2n >= 0, y_2n+1 free (the masked cone of the dual prox) equals
+    AV@R_alpha(Z) = max{sum mu Z : 0 <= mu <= pi/alpha, sum mu = 1}."""
+    rng = np.random.default_rng(int(alpha * 100))
+    for n in (2, 3, 5):
+        pi = rng.dirichlet(np.ones(n))
+        Z = rng.standard_normal(n)
+        E, b = build.AVaR(alpha).make(pi)
+        bounds = [(0, None)] * (2 * n) + [(None, None)]
+        lp = linprog(b.ravel(), A_eq=E.T, b_eq=Z, bounds=bounds, method="highs")
+        assert lp.status == 0
+        order = np.argsort(-Z)
+        mass, val = 1.0, 0.0
+        for i in order:
+            w = min(pi[i] / alpha, mass)
+            val += w * Z[i]
+            mass -= w
+        assert lp.fun == pytest.approx(val, abs=1e-9)
+
+
+def test_workload_sizes(tmp_path):
+    """The 229-node cfg2 tree (3 events, horizon 10, branching stops at stage 3) and the 31-node cfg1."""
+    d = str(tmp_path / "mid")
+    configs.generate("mid", d)
+    st = btio.read_flat(os.path.join(d, "stages_u.bt"))
+    assert len(st) == 229 and int((st < st.max()).sum()) == 202
+    d = str(tmp_path / "small")
+    configs.generate("small", d)
+    st = btio.read_flat(os.path.join(d, "stages_u.bt"))
+    assert len(st) == 31 and int((st < st.max()).sum()) == 15

@@ -1,0 +1,204 @@
+"""Pins of the FP64 CPU oracle (oracle/spock_oracle.cpp) by what the mathematics fixes and by the
+known answers of the reference's tests, so that the GPU parity tests compare against a checked
+checker:
+
+* L* is the adjoint of L, and gamma ||L|| = 0.99 (the step size of api/python/spock/problem.py:402-417);
+* the dynamics projection equals the KKT solution of min ||x - xbar||^2 + ||u - ubar||^2 subject to
+  the dynamics (the QP of api/python/spock/problem.py:233-261, solved here with numpy);
+* the kernel projection is the orthogonal projector onto ker [E', -I, -I] of every nonleaf node;
+* SOC projection known answers (tests/cuda/testProjections.cu:150-170) and the textbook projection;
+* the box projection (tests/cuda/testKernels.cu:14-36) and the AV@R mask (tests/cuda/testRisks.cu:46-49);
+* least squares against numpy; the M inner product against its definition;
+* T maps into the dynamics-feasible set and a converged solve is a fixed point of T;
+* committed golden vectors (tests/golden) reproduce.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from oracle import numpy_ref as nr
+from oracle.oracle import Oracle, box, lstsq, lstsq_gram, soc
+from spock_amd.api import btio, build, configs
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def inert_zero(o, z):
+    z = z.copy()
+    z[o.sU + o.sX + o.sY] = 0.0
+    iv0 = o.n_prim + o.sI + o.sII + o.sIII
+    z[iv0:iv0 + o.nx + o.nu + 2] = 0.0
+    return z
+
+
+@pytest.mark.parametrize("name", ["fixture", "small", "mid"])
+def test_adjoint(problem_dirs, name):
+    o = Oracle(problem_dirs[name])
+    rng = np.random.default_rng(0)
+    p = rng.standard_normal(o.n_prim)
+    p[o.sU + o.sX + o.sY] = 0.0  # t[0] is an inert slot (no dual block refers to it)
+    d = rng.standard_normal(o.n_dual)
+    lhs, rhs = p @ o.Ladj(d), o.L(p) @ d
+    assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))
+
+
+@pytest.mark.parametrize("name", ["fixture", "small"])
+def test_step_size_is_099_over_norm(problem_dirs, name):
+    o = Oracle(problem_dirs[name])
+    Ld = nr.dense_operator(o.L, o.n_prim)
+    assert o.gamma * np.linalg.norm(Ld, 2) == pytest.approx(0.99, abs=2e-3)
+
+
+@pytest.mark.parametrize("name", ["fixture", "small"])
+def test_dynamics_projection_is_the_kkt_solution(problem_dirs, name):
+    d = problem_dirs[name]
+    o = Oracle(d)
+    N, nl, nx, nu = o.info["numNodes"], o.info["numNonleafNodes"], o.nx, o.nu
+    AB = btio.read_bt(os.path.join(d, "dynamics_AB_d.bt")).transpose(2, 0, 1)
+    anc = btio.read_flat(os.path.join(d, "ancestors_u.bt")).astype(np.int64)
+    rng = np.random.default_rng(1)
+    xb, ub, x0 = 10 * rng.standard_normal((N, nx)), 10 * rng.standard_normal((nl, nu)), rng.standard_normal(nx)
+    xk, uk = nr.kkt_dynamics_projection(AB, anc, nl, nx, nu, x0, xb, ub)
+    xo, uo = o.dyn_project(x0, xb.reshape(-1), ub.reshape(-1))
+    assert rel_err(xo, xk.reshape(-1)) < 1e-9
+    assert rel_err(uo, uk.reshape(-1)) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["fixture", "small"])
+def test_kernel_projection_is_orthogonal_onto_the_risk_kernel(problem_dirs, name):
+    d = problem_dirs[name]
+    o = Oracle(d)
+    info = o.info
+    nl, E = info["numNonleafNodes"], info["numEvents"]
+    yD = 2 * E + 1
+    ch_from = btio.read_flat(os.path.join(d, "childrenFrom_u.bt")).astype(int)
+    num_ch = btio.read_flat(os.path.join(d, "numChildren_u.bt")).astype(int)
+    cond = btio.read_flat(os.path.join(d, "conditionalProbabilities_d.bt"))
+    alpha = info["risk"]["alpha"]
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(o.n_prim)
+    kv = o.kernel_project(v)
+    assert rel_err(o.kernel_project(kv), kv) < 1e-12
+    w = rng.standard_normal(o.n_prim)
+    assert abs(kv @ w - v @ o.kernel_project(w)) <= 1e-10 * np.linalg.norm(v) * np.linalg.norm(w)
+    oY, oT, oS = o.sU + o.sX, o.sU + o.sX + o.sY, o.sU + o.sX + o.sY + o.sT
+    for i in range(nl):
+        n, c0 = num_ch[i], ch_from[i]
+        Emat, _ = build.AVaR(alpha).make(cond[c0:c0 + n])
+        y = kv[oY + i * yD:oY + i * yD + 2 * n + 1]
+        t = kv[oT + c0:oT + c0 + n]
+        s = kv[oS + c0:oS + c0 + n]
+        assert np.max(np.abs(Emat.T @ y - t - s)) < 1e-10 * max(1.0, np.max(np.abs(v)))
+
+
+def test_soc_known_answers():
+    """tests/cuda/testProjections.cu:150-170."""
+    assert soc([1, 2, 3, 4, 0.5]) == pytest.approx(
+        [0.5456435464587639, 1.0912870929175278, 1.6369306393762917, 2.1825741858350556, 2.988612787525831],
+        abs=1e-12)
+    assert soc([5, 6, 7, 8, -200]) == pytest.approx([0, 0, 0, 0, 0], abs=1e-12)
+    assert soc([9, -10, 11, -12, 100]) == pytest.approx([9, -10, 11, -12, 100], abs=1e-12)
+
+
+def test_soc_matches_textbook_projection():
+    rng = np.random.default_rng(3)
+    for dim in (2, 3, 8, 72, 1034):
+        for scale in (0.1, 1.0, 10.0):
+            v = rng.standard_normal(dim)
+            v[-1] *= scale * np.sqrt(dim)
+            p = soc(v)
+            assert rel_err(p, nr.soc_project(v)) < 1e-12
+            assert np.linalg.norm(p[:-1]) <= p[-1] * (1 + 1e-12) + 1e-12  # inside the cone
+            assert abs((v - p) @ p) <= 1e-9 * max(1.0, np.linalg.norm(v) ** 2)  # orthogonality
+
+
+def test_box_known_answer():
+    """tests/cuda/testKernels.cu:14-36 (with an infinite first bound)."""
+    n, ub, oob = 1025, 4.0, 0.1
+    v = np.full(n, ub + oob)
+    v[::2] *= -1
+    v[::3] *= 0.5
+    lbs, ubs = np.full(n, -ub), np.full(n, ub)
+    lbs[0], ubs[0] = -np.inf, np.inf
+    out = np.array([box(v[i], lbs[i], ubs[i]) for i in range(n)])
+    assert out[0] == pytest.approx(-(ub + oob) * 0.5, abs=1e-15)
+    assert np.all(out >= -ub) and np.all(out <= ub)
+    assert np.array_equal(out[1:], np.clip(v[1:], -ub, ub))
+
+
+def test_avar_mask_known_answer(problem_dirs):
+    """tests/cuda/testRisks.cu:46-49: projecting -10 leaves exactly these entries of the I block."""
+    o = Oracle(problem_dirs["fixture"])
+    w = np.zeros(o.n_dual)
+    w[:o.sI] = -10.0
+    p = o.project_dual(w)[:o.sI]
+    expected = np.zeros(o.sI)
+    expected[[4, 9, 14, 17, 18, 19, 22, 23, 24, 27, 28, 29, 32, 33, 34]] = -10.0
+    assert np.array_equal(p, expected)
+
+
+def test_least_squares_against_numpy():
+    rng = np.random.default_rng(4)
+    for n, m in ((10, 1), (500, 3), (5000, 5)):
+        Y = rng.standard_normal((n, m))
+        r = rng.standard_normal(n)
+        ref = np.linalg.lstsq(Y, r, rcond=None)[0]
+        assert rel_err(lstsq(Y, r), ref) < 1e-12
+        assert rel_err(lstsq_gram(Y, r), ref) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["fixture", "small"])
+def test_dotM_definition(problem_dirs, name):
+    o = Oracle(problem_dirs[name])
+    rng = np.random.default_rng(5)
+    x = inert_zero(o, rng.standard_normal(o.n_iter))
+    y = inert_zero(o, rng.standard_normal(o.n_iter))
+    p = o.n_prim
+    ref = x @ y - o.gamma * (x[:p] @ o.Ladj(y[p:]) + x[p:] @ o.L(y[:p]))
+    assert o.dotM(x, y) == pytest.approx(ref, rel=1e-12, abs=1e-12)
+
+
+@pytest.mark.parametrize("name", ["fixture", "small"])
+def test_T_lands_on_the_dynamics(problem_dirs, name):
+    """The primal part of T(z) is a projection onto the dynamics with x_0 = x0."""
+    d = problem_dirs[name]
+    o = Oracle(d)
+    N, nl, nx, nu = o.info["numNodes"], o.info["numNonleafNodes"], o.nx, o.nu
+    AB = btio.read_bt(os.path.join(d, "dynamics_AB_d.bt")).transpose(2, 0, 1)
+    anc = btio.read_flat(os.path.join(d, "ancestors_u.bt")).astype(np.int64)
+    rng = np.random.default_rng(6)
+    x0 = rng.uniform(-1, 1, nx)
+    z = inert_zero(o, rng.standard_normal(o.n_iter))
+    tz = o.T(o.scaled_x0(x0), z)
+    x = tz[o.sU:o.sU + o.sX].reshape(N, nx)
+    u = tz[:o.sU].reshape(nl, nu)
+    assert rel_err(x[0], o.scaled_x0(x0)) < 1e-14
+    for j in range(1, N):
+        a = anc[j]
+        assert np.max(np.abs(x[j] - AB[j][:, :nx] @ x[a] - AB[j][:, nx:] @ u[a])) < 1e-10 * max(1.0, np.abs(x).max())
+
+
+def test_converged_solve_is_a_fixed_point(problem_dirs):
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    x0 = configs.initial_states(1, o.nx, 1, "uniform")[0]
+    r = o.solve(x0, tol_abs=1e-4, tol_rel=1e-4, max_iters=20000)
+    assert r["status"] == 0
+    z = r["z"]
+    assert np.max(np.abs(z - o.T(o.scaled_x0(x0), z))) <= 5e-4 * np.max(np.abs(z))
+
+
+def test_golden_vectors_reproduce(tmp_path):
+    """tests/golden/fixture_golden.npz, written by tests/golden/make_golden.py from the oracle."""
+    g = np.load(os.path.join(GOLDEN, "fixture_golden.npz"), allow_pickle=False)
+    d = str(tmp_path / "fixture")
+    configs.generate("fixture", d)
+    o = Oracle(d)
+    assert rel_err(o.L(g["prim"]), g["L_prim"]) < 1e-14
+    assert rel_err(o.Ladj(g["dual"]), g["Ladj_dual"]) < 1e-14
+    assert rel_err(o.T(o.scaled_x0(g["x0"]), g["z"]), g["T_z"]) < 1e-14
+    r = o.solve(g["x0"], tol_abs=1e-30, max_iters=75)
+    assert rel_err(r["z"], g["spock75"]) < 1e-12
+    assert [r[k] for k in ("k0", "k1", "k2", "k2bt", "k3")] == list(g["spock75_counts"])
