@@ -64,6 +64,8 @@ typedef struct spk_options {
     int block_size;
     int ls_method;   /* enum spk_ls_method */
     int dyn_inverse; /* 1: apply (L L')^-1 as one matvec; 0: two triangular solves as the reference */
+    long chunk_iters; /* shared batches: > 0 ends a launch after this many outer iterations with the
+                         whole solver state kept on the device; spk_batch_resume continues it exactly */
 } spk_options;
 
 typedef struct spk_result {
@@ -141,6 +143,10 @@ void *spk_batch_iterate_ptr(spk_batch *b, int k);
 int spk_batch_copy_iterates(spk_batch *dst, const spk_batch *src);
 /* enqueue the solve of every instance; warm != 0 keeps the current iterates as previous solution */
 int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm);
+/* continue a chunked solve of a shared batch for another chunk_iters outer iterations; the
+ * continued iterates, branch decisions and counters are those of one uninterrupted solve.  Results
+ * after a chunk report SPK_NOT_RUN (and the iterations so far) for instances still running. */
+int spk_batch_resume(spk_batch *b);
 /* wait for the last solve; fills count results and the kernel time in seconds (nullable) */
 int spk_batch_results(spk_batch *b, spk_result *out, double *kernel_seconds);
 /* single operator on every instance: in/out are host arrays concatenated over instances

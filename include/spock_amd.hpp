@@ -1,0 +1,260 @@
+// C++ drop-in for the reference's solver classes, over the C ABI of libspock_amd.so.
+//
+// The class and method names, argument meanings and error behaviour follow the reference so
+// that its example drivers compile against this header with the include swapped:
+//   ScenarioTree<T>(path)                       src/tree.cuh:102-179
+//   ProblemData<T>(tree)                        src/problem.cuh:126-153
+//   CacheBuilder<T>(tree, data).tol(...)...     src/cache.cuh:1154-1244 (defaults: tolAbs 1e-3,
+//                                               tolRel 0, maxTime 0, maxIters 0, maxInner 8, m 3)
+//   Cache<T>::runSpock / runCp (x0, previous)   src/cache.cuh:353-355, 967-1132
+//   status / solveIter / solveTime / states / inputs / solution / reset / sizePrimal / sizeDual
+//                                               src/cache.cuh:357-412, 436-479
+//   enum CacheStatus and toString               src/cache.cuh:68-88
+// Errors are std::invalid_argument with the library's message, as the reference throws.
+// T = double solves in FP64, T = float in FP32 (the reference instantiates both,
+// src/definitions.cu:24-443).  solution() returns the iterate as a host vector (the reference
+// returns its device tensor).
+#pragma once
+
+#include <chrono>
+#include <cstdint>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "spock_amd.h"
+
+namespace spock_amd {
+
+enum CacheStatus { notRun = -1, converged = 0, outOfIter = 1, outOfTime = 2 };
+
+inline std::string toString(int status) {
+    switch (status) {
+        case notRun: return "notRun";
+        case converged: return "converged";
+        case outOfIter: return "outOfIter";
+        case outOfTime: return "outOfTime";
+        default: return "unknown";
+    }
+}
+
+namespace detail {
+inline spk_ctx *context() {  // one context (and HIP stream) per host thread
+    thread_local struct Holder {
+        spk_ctx *c = nullptr;
+        Holder() {
+            if (spk_ctx_create(0, &c) != 0) throw std::invalid_argument("[spock_amd] no HIP device");
+        }
+        ~Holder() { spk_ctx_destroy(c); }
+    } h;
+    return h.c;
+}
+inline void check(int rc, const char *what) {
+    if (rc != 0) throw std::invalid_argument(std::string("[") + what + "] " + spk_last_error(context()));
+}
+}  // namespace detail
+
+// Reads a .bt tensor file (three uint64 dimensions, then the data in column-major order; the
+// element type is fixed by the file suffix: _d double, _f float, _u uint64) into a flat vector,
+// as DTensor<T>::parseFromFile does for the reference's drivers (examples/random/main.cu:48-50).
+template <typename T>
+std::vector<T> parseFromFile(const std::string &file) {
+    std::ifstream in(file, std::ios::binary);
+    if (!in) throw std::invalid_argument("[parseFromFile] cannot open " + file);
+    uint64_t dims[3];
+    in.read(reinterpret_cast<char *>(dims), sizeof(dims));
+    const size_t n = (size_t) (dims[0] * dims[1] * dims[2]);
+    std::vector<T> out(n);
+    const std::string sfx = file.size() >= 5 ? file.substr(file.size() - 5) : "";
+    if (sfx == "_d.bt") {
+        std::vector<double> v(n);
+        in.read(reinterpret_cast<char *>(v.data()), n * sizeof(double));
+        for (size_t k = 0; k < n; k++) out[k] = (T) v[k];
+    } else if (sfx == "_f.bt") {
+        std::vector<float> v(n);
+        in.read(reinterpret_cast<char *>(v.data()), n * sizeof(float));
+        for (size_t k = 0; k < n; k++) out[k] = (T) v[k];
+    } else {
+        std::vector<uint64_t> v(n);
+        in.read(reinterpret_cast<char *>(v.data()), n * sizeof(uint64_t));
+        for (size_t k = 0; k < n; k++) out[k] = (T) v[k];
+    }
+    if (!in) throw std::invalid_argument("[parseFromFile] short file " + file);
+    return out;
+}
+
+template <typename T>
+class ScenarioTree {
+public:
+    explicit ScenarioTree(std::string path = "./data/") : m_path(std::move(path)) {
+        if (!m_path.empty() && m_path.back() != '/') m_path += '/';
+        std::ifstream in(m_path + "data.json");
+        if (!in) throw std::invalid_argument("[ScenarioTree] no data.json in " + m_path);
+        std::stringstream ss;
+        ss << in.rdbuf();
+        m_json = ss.str();
+    }
+    const std::string &path() const { return m_path; }
+    std::string fpFileExt() const { return sizeof(T) == 8 ? "_d.bt" : "_f.bt"; }
+    size_t numStates() const { return field("numStates"); }
+    size_t numInputs() const { return field("numInputs"); }
+    size_t numNodes() const { return field("numNodes"); }
+    size_t numNonleafNodes() const { return field("numNonleafNodes"); }
+    size_t numStages() const { return field("numStages"); }
+    size_t numEvents() const { return field("numEvents"); }
+
+private:
+    size_t field(const char *key) const {
+        const size_t k = m_json.find(std::string("\"") + key + "\"");
+        if (k == std::string::npos) throw std::invalid_argument(std::string("[ScenarioTree] missing ") + key);
+        return (size_t) std::stoul(m_json.substr(m_json.find(':', k) + 1));
+    }
+    std::string m_path, m_json;
+};
+
+template <typename T>
+class ProblemData {
+public:
+    explicit ProblemData(ScenarioTree<T> &tree) {
+        detail::check(spk_problem_load(detail::context(), tree.path().c_str(), &m_p), "ProblemData");
+        long s[16];
+        spk_problem_sizes(m_p, s);
+        m_numNodes = s[0];
+        m_numNonleaf = s[1];
+        m_numStates = s[4];
+        m_numInputs = s[5];
+        m_sizePrim = s[6];
+        m_sizeDual = s[7];
+        m_scaling.resize(m_numStates + m_numInputs);
+        spk_problem_scaling(m_p, m_scaling.data());
+    }
+    ~ProblemData() { spk_problem_destroy(m_p); }
+    ProblemData(const ProblemData &) = delete;
+    ProblemData &operator=(const ProblemData &) = delete;
+    spk_problem *handle() const { return m_p; }
+    size_t numStates() const { return m_numStates; }
+    size_t numInputs() const { return m_numInputs; }
+    size_t numNodes() const { return m_numNodes; }
+    size_t numNonleafNodes() const { return m_numNonleaf; }
+    size_t sizePrimal() const { return m_sizePrim; }
+    size_t sizeDual() const { return m_sizeDual; }
+    const std::vector<double> &scaling() const { return m_scaling; }
+    T stepSize() const { return (T) spk_problem_step_size(m_p); }
+
+private:
+    spk_problem *m_p = nullptr;
+    size_t m_numNodes = 0, m_numNonleaf = 0, m_numStates = 0, m_numInputs = 0, m_sizePrim = 0, m_sizeDual = 0;
+    std::vector<double> m_scaling;
+};
+
+template <typename T>
+class Cache {
+public:
+    Cache(ScenarioTree<T> &, ProblemData<T> &data, const spk_options &opts) : m_data(&data), m_opts(opts) {
+        spk_problem *p = data.handle();
+        detail::check(spk_batch_create(detail::context(), 1, &p, sizeof(T) == 8 ? SPK_F64 : SPK_F32, &m_b),
+                      "Cache");
+    }
+    ~Cache() { spk_batch_destroy(m_b); }
+    Cache(Cache &&o) noexcept : m_data(o.m_data), m_opts(o.m_opts), m_b(o.m_b), m_exit(o.m_exit), m_iters(o.m_iters),
+                                m_time(o.m_time) {
+        o.m_b = nullptr;
+    }
+    Cache(const Cache &) = delete;
+
+    int runSpock(std::vector<T> &x0, std::vector<T> *previousSolution = nullptr) {
+        return run(SPK_SPOCK, x0, previousSolution);
+    }
+    int runCp(std::vector<T> &x0, std::vector<T> *previousSolution = nullptr) { return run(SPK_CP, x0, previousSolution); }
+
+    int status() const { return m_exit; }
+    size_t solveIter() const { return m_iters; }
+    T solveTime() const { return m_time; }
+    size_t sizePrimal() const { return m_data->sizePrimal(); }
+    size_t sizeDual() const { return m_data->sizeDual(); }
+
+    std::vector<T> solution() {
+        std::vector<double> z(sizePrimal() + sizeDual());
+        detail::check(spk_batch_get_iterate(m_b, 0, z.data()), "solution");
+        return std::vector<T>(z.begin(), z.end());
+    }
+
+    // states x (numNodes x numStates) and inputs u (numNonleafNodes x numInputs) of the solution,
+    // undoing the preconditioning (src/cache.cuh:386-412)
+    std::vector<T> states() {
+        const std::vector<T> z = solution();
+        const size_t nx = m_data->numStates(), nu = m_data->numInputs(), oX = m_data->numNonleafNodes() * nu;
+        std::vector<T> x(z.begin() + oX, z.begin() + oX + m_data->numNodes() * nx);
+        for (size_t k = 0; k < x.size(); k++) x[k] /= (T) m_data->scaling()[k % nx];
+        return x;
+    }
+    std::vector<T> inputs() {
+        const std::vector<T> z = solution();
+        const size_t nx = m_data->numStates(), nu = m_data->numInputs();
+        std::vector<T> u(z.begin(), z.begin() + m_data->numNonleafNodes() * nu);
+        for (size_t k = 0; k < u.size(); k++) u[k] /= (T) m_data->scaling()[nx + k % nu];
+        return u;
+    }
+
+    void reset() {
+        detail::check(spk_batch_reset(m_b), "reset");
+        m_exit = notRun;
+    }
+
+private:
+    int run(int method, std::vector<T> &x0, std::vector<T> *prev) {
+        if (x0.size() != m_data->numStates())
+            throw std::invalid_argument("[initialiseState] Error initialising state: problem setup for " +
+                                        std::to_string(m_data->numStates()) + " but given " +
+                                        std::to_string(x0.size()) + " states");
+        const auto t0 = std::chrono::high_resolution_clock::now();
+        std::vector<double> xd(x0.begin(), x0.end());
+        detail::check(spk_batch_set_x0(m_b, xd.data()), "initialiseState");
+        if (prev) {
+            std::vector<double> zd(prev->begin(), prev->end());
+            detail::check(spk_batch_set_iterate(m_b, 0, zd.data()), "initialisePrev");
+        }
+        spk_options o = m_opts;
+        o.method = method;
+        detail::check(spk_batch_solve(m_b, &o, prev ? 1 : 0), method == SPK_SPOCK ? "runSpock" : "runCp");
+        spk_result r;
+        detail::check(spk_batch_results(m_b, &r, nullptr), "results");
+        m_exit = r.status;
+        m_iters = (size_t) r.iters;
+        m_time = (T) std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+        return m_exit;
+    }
+
+    ProblemData<T> *m_data;
+    spk_options m_opts;
+    spk_batch *m_b = nullptr;
+    int m_exit = notRun;
+    size_t m_iters = 0;
+    T m_time = 0;
+};
+
+template <typename T>
+class CacheBuilder {
+public:
+    CacheBuilder(ScenarioTree<T> &tree, ProblemData<T> &data) : m_tree(tree), m_data(data) {
+        spk_default_options(&m_opts);  // tolAbs 1e-3, tolRel 0, maxInner 8, Anderson 3 (src/cache.cuh:1154-1163)
+    }
+    CacheBuilder &toleranceAbsolute(T tol) { m_opts.tol_abs = tol; return *this; }
+    CacheBuilder &toleranceRelative(T tol) { m_opts.tol_rel = tol; return *this; }
+    CacheBuilder &tol(T tol) { m_opts.tol_abs = tol; m_opts.tol_rel = tol; return *this; }
+    CacheBuilder &maxTimeSecs(T time) { m_opts.max_time_secs = time; return *this; }
+    CacheBuilder &maxIters(size_t iters) { m_opts.max_iters = (long) iters; return *this; }
+    CacheBuilder &maxItersInner(size_t iters) { m_opts.max_inner = (int) iters; return *this; }
+    CacheBuilder &andersonBuffer(size_t buffer) { m_opts.anderson_buffer = (int) buffer; return *this; }
+    CacheBuilder &enableDebug(bool = true) { return *this; }  // per-iteration traces: not provided
+    Cache<T> build() { return Cache<T>(m_tree, m_data, m_opts); }
+
+private:
+    ScenarioTree<T> &m_tree;
+    ProblemData<T> &m_data;
+    spk_options m_opts;
+};
+
+}  // namespace spock_amd

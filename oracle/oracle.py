@@ -149,13 +149,15 @@ class Oracle:
         return lib().orc_dotM(self.h, _d(x), _d(y))
 
     def solve(self, x0, method="spock", warm=None, tol_abs=1e-3, tol_rel=0.0, max_time=0.0, max_iters=0,
-              max_inner=8, anderson=3, check_period=25, trace_len=0, ls="householder", tol_fixed=0.0):
+              max_inner=8, anderson=3, check_period=25, trace_len=0, ls="householder", tol_fixed=0.0,
+              adjoint_dots=False):
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         z = np.zeros(self.n_iter)
         out = np.zeros(7, dtype=np.int64)
         outd = np.zeros(2)
         dopts = np.array([tol_abs, tol_rel, max_time, tol_fixed])
-        iopts = np.array([max_iters, max_inner, anderson, check_period, 1 if ls == "gram" else 0], dtype=np.int64)
+        iopts = np.array([max_iters, max_inner, anderson, check_period, 1 if ls == "gram" else 0,
+                          1 if adjoint_dots else 0], dtype=np.int64)
         wp = _d(np.ascontiguousarray(warm, dtype=np.float64)) if warm is not None else None
         tb = np.zeros(trace_len, dtype=np.int32) if trace_len else None
         tw = np.zeros(trace_len) if trace_len else None

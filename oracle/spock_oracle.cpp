@@ -504,6 +504,18 @@ double dotM(const Problem &p, const double *x, const double *y) {
     return acc;
 }
 
+// <x, M y> through adjoints only: x'y - gamma (x_p' L* y_d + (L* x_d)' y_p), equal to dotM since
+// x_d' L y_p = (L* x_d)' y_p.  The shared-batch GPU engine computes SuperMann's norms this way.
+double dotMAdj(const Problem &p, const double *x, const double *y) {
+    Vec ax(p.sPrim), ay(p.sPrim);
+    opLadj(p, y + p.sPrim, ay.data());
+    opLadj(p, x + p.sPrim, ax.data());
+    double a = 0., b = 0.;
+    for (size_t k = 0; k < p.sIter; k++) a += x[k] * y[k];
+    for (size_t k = 0; k < p.sPrim; k++) b += x[k] * ay[k] + ax[k] * y[k];
+    return a - p.gamma * b;
+}
+
 // -------------------------------------------------------- Householder least squares
 // argmin ||Y g - r|| for Y n x m (column major), via Householder QR (geqrf + ormqr + trsm).
 bool lstsqHouseholder(const double *Yin, size_t n, int m, const double *rin, double *g) {
@@ -586,6 +598,7 @@ struct Opts {
     long maxIters, maxInner, andBuff, checkPeriod;
     long lsGram = 0;        // 1: Gram/Cholesky least squares instead of Householder
     double tolFixed = 0.;   // > 0: accept against this tolerance from the first check on
+    long adjDot = 0;        // 1: SuperMann's norms and inner products through dotMAdj
 };
 
 struct Trace {  // per outer iteration: branch taken and the residual norm
@@ -702,10 +715,14 @@ struct Solver {
             if (computeError(it, initFlag, t0, ellR)) { z = cand; break; }
             backup = cand;
             // normM(r): w = sqrt(r' (r - gamma ell(r)))  (src/cache.cuh:740-742)
-            ell(p, r.data(), ellR.data());
-            double acc = 0.;
-            for (size_t k = 0; k < n; k++) acc += r[k] * (ellR[k] * (-p.gamma) + r[k]);
-            w = std::sqrt(acc);
+            if (o.adjDot) {
+                w = std::sqrt(dotMAdj(p, r.data(), r.data()));
+            } else {
+                ell(p, r.data(), ellR.data());
+                double acc = 0.;
+                for (size_t k = 0; k < n; k++) acc += r[k] * (ellR[k] * (-p.gamma) + r[k]);
+                w = std::sqrt(acc);
+            }
             updateDirection(it);
             zPrev = z;
             rPrev = r;
@@ -723,7 +740,7 @@ struct Solver {
                 for (size_t k = 0; k < n; k++) { dirS[k] = dir[k] * tau; z[k] = zPrev[k] + dirS[k]; }
                 opT(p, x0s.data(), z.data(), cand.data());
                 for (size_t k = 0; k < n; k++) r[k] = z[k] - cand[k];
-                wTilde = std::sqrt(dotM(p, r.data(), r.data()));
+                wTilde = std::sqrt(o.adjDot ? dotMAdj(p, r.data(), r.data()) : dotM(p, r.data(), r.data()));
                 if (w <= wSafe && wTilde <= c1 * w) {  // K1
                     cand = z;
                     wSafe = wTilde + std::pow(c2, (double) it);
@@ -731,7 +748,8 @@ struct Solver {
                     br = 1;
                     break;
                 }
-                double rho = wTilde * wTilde - dotM(p, r.data(), dirS.data());
+                double rho = wTilde * wTilde - (o.adjDot ? dotMAdj(p, r.data(), dirS.data())
+                                                         : dotM(p, r.data(), dirS.data()));
                 if (rho >= sigma * wTilde * w) {  // K2
                     double f = lambda * rho / (wTilde * wTilde);
                     for (size_t k = 0; k < n; k++) { r[k] *= f; cand[k] = zPrev[k] - r[k]; }
@@ -824,14 +842,14 @@ int orc_lstsq_gram(const double *Y, long n, int m, const double *r, double *g) {
 }
 
 // method 0 = SPOCK (runSpock), 1 = CP (runCp).  dopts = {tolAbs, tolRel, maxTime, tolFixed};
-// iopts = {maxIters, maxInner, andBuff, checkPeriod, lsGram}.  out = {iters, status, k0, k1, k2, k2bt, k3}
+// iopts = {maxIters, maxInner, andBuff, checkPeriod, lsGram, adjDot}.  out = {iters, status, k0, k1, k2, k2bt, k3}
 // outD = {lastErr, tol}.  z_out receives the final iterate; trace arrays (nullable) get per-iteration
 // branch codes and residual norms up to traceLen entries.
 int orc_solve(void *h, int method, const double *x0, const double *warm, const double *dopts, const long *iopts,
               double *zOut, long *out, double *outD, int *traceBranch, double *traceW, long traceLen) {
     auto &p = *static_cast<Problem *>(h);
     Opts o{dopts[0], dopts[1], dopts[2], iopts[0], iopts[1], iopts[2], iopts[3] > 0 ? iopts[3] : 25, iopts[4],
-           dopts[3]};
+           dopts[3], iopts[5]};
     Solver s(p, o);
     bool tr = traceBranch || traceW;
     int st = method == 0 ? s.runSpock(x0, warm, tr) : s.runCp(x0, warm);

@@ -76,6 +76,10 @@ struct spk_batch {
     bool lanes = false;
     long Bpad = 0;     // count rounded up to 64
     DevBuf lbuf, ltol; // interleaved state; per-instance given tolerances
+    DevBuf lsave, lgsave;   // solver state kept between the launches of a chunked solve
+    spk_options lastOpts{};
+    int lastWarm = 0;
+    bool resumable = false;
     size_t lRegion[16] = {};  // element offsets: work[0..8], zio, rio, hS, hY, qd, dd, x0
     static constexpr int LANE_MAXM = 4;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -366,7 +370,7 @@ void validateLanes(const HostProblem &h, int) {
 }
 
 template <typename T>
-void buildLaneInst(spk_batch *b, const spk_options &o, int warm) {
+void buildLaneInst(spk_batch *b, const spk_options &o, int warm, int resume = 0) {
     spk::LaneInst<T> L;
     std::memset(static_cast<void *>(&L), 0, sizeof(L));
     spk_problem *p = b->probs[0];
@@ -393,6 +397,10 @@ void buildLaneInst(spk_batch *b, const spk_options &o, int warm) {
     L.method = o.method;
     L.warm = warm;
     L.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
+    L.chunkIters = o.chunk_iters > 0 ? o.chunk_iters : 0;
+    L.resume = resume;
+    L.save = static_cast<spk::LaneSave *>(b->lsave.p);
+    L.gsave = static_cast<spk::LaneGroupSave *>(b->lgsave.p);
     b->hInst.assign(sizeof(L), 0);
     std::memcpy(b->hInst.data(), &L, sizeof(L));
     b->dInst.alloc(sizeof(L));
@@ -406,8 +414,11 @@ int laneBlock(const spk_options &o) {
 }
 
 template <typename T>
-void launchLaneSolve(spk_batch *b, const spk_options &o, int warm) {
-    buildLaneInst<T>(b, o, warm);
+void launchLaneSolve(spk_batch *b, const spk_options &o, int warm, int resume = 0) {
+    buildLaneInst<T>(b, o, warm, resume);
+    b->lastOpts = o;
+    b->lastWarm = warm;
+    b->resumable = o.chunk_iters > 0;
     b->dOut.alloc(sizeof(spk::Out) * b->count);
     check(hipEventRecord(b->ev0, b->ctx->stream), "event");
     hipLaunchKernelGGL(spk::k_lane_solve<T>, dim3((unsigned) (b->Bpad / 64)), dim3(laneBlock(o)), 0, b->ctx->stream,
@@ -494,6 +505,7 @@ void spk_default_options(spk_options *o) {
     o->block_size = 256;
     o->ls_method = SPK_LS_HOUSEHOLDER;
     o->dyn_inverse = 1;
+    o->chunk_iters = 0;
 }
 
 int spk_ctx_create(int device, spk_ctx **out) {
@@ -640,6 +652,8 @@ int spk_batch_create_shared(spk_ctx *ctx, spk_problem *problem, int count, int d
         else ensureUploaded<float>(problem);
         const size_t total = laneLayout(b.get());
         b->lbuf.alloc(total * b->elSize());
+        b->lsave.alloc(sizeof(spk::LaneSave) * b->Bpad);
+        b->lgsave.alloc(sizeof(spk::LaneGroupSave) * (b->Bpad / 64));
         check(hipMemset(b->lbuf.p, 0, total * b->elSize()), "zero state");
         check(hipEventCreate(&b->ev0), "event");
         check(hipEventCreate(&b->ev1), "event");
@@ -975,6 +989,7 @@ int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm) {
         if (o.ls_method != SPK_LS_HOUSEHOLDER && o.ls_method != SPK_LS_GRAM)
             throw std::runtime_error("ls_method must be SPK_LS_HOUSEHOLDER or SPK_LS_GRAM");
         check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        if (!b->lanes && o.chunk_iters > 0) throw std::runtime_error("chunked solves need a shared batch");
         if (b->lanes) {
             if (o.anderson_buffer > spk_batch::LANE_MAXM)
                 throw std::runtime_error("shared batches support anderson_buffer <= 4");
@@ -985,6 +1000,19 @@ int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm) {
         ensureAnderson(b, o.anderson_buffer);
         if (b->dtype == SPK_F64) launchSolve<double>(b, o, warm);
         else launchSolve<float>(b, o, warm);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_resume(spk_batch *b) {
+    try {
+        if (!b->lanes || !b->resumable) throw std::runtime_error("no chunked solve of a shared batch to resume");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        if (b->dtype == SPK_F64) launchLaneSolve<double>(b, b->lastOpts, b->lastWarm, 1);
+        else launchLaneSolve<float>(b, b->lastOpts, b->lastWarm, 1);
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);

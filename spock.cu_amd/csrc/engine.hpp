@@ -43,6 +43,22 @@ __host__ __device__ inline size_t laneIndex(long rows, long e, long b) {
     return ((size_t) (b >> 6) * rows + e) * 64 + (b & 63);
 }
 
+// SuperMann state of one lane kept between the launches of a chunked solve.
+struct LaneSave {
+    double G[4][4], gb[4];
+    double tol, lastErr, wLast, zeta, w, wSafe;
+    long iters, k0, k1, k2, k2bt, k3, nT, nEll;
+    int status, active, initFlag, pad;
+};
+
+// Per-workgroup state of a chunked solve: buffer roles (indices of z, cand, zPrev, res, resPrev,
+// backup into work[0..5]), Anderson ring head, next outer iteration and device ticks spent.
+struct LaneGroupSave {
+    int roles[6];
+    int head, pad;
+    long it, ticks;
+};
+
 // Many instances of ONE problem in the block-interleaved layout: a wave whose lanes are the 64
 // instances of one block moves 64 consecutive words per access, a workgroup's state is one
 // contiguous slab per region, and the factor tables are read at wave-uniform addresses (scalar
@@ -67,6 +83,10 @@ struct LaneInst {
     long maxIters;
     int maxInner, andBuff, checkPeriod, method, warm;
     double maxTicks;
+    long chunkIters;          // > 0: stop after this many outer iterations, state saved for a resume
+    int resume;               // 1: continue the saved solve
+    LaneSave *save;           // B entries
+    LaneGroupSave *gsave;     // B / 64 entries
 };
 
 // Phases timed inside the solve (wall_clock64 ticks, summed over the solve).

@@ -646,7 +646,7 @@ __device__ __forceinline__ double laneMax(double v, double *lds, const Ctx &c) {
 
 template <typename T>
 struct DotMax {
-    T dot, mx;
+    T dot, mx, cross;  // cross = sum over primal elements of x_p . (L* y_d)
 };
 
 // sum x.(y - gamma ell(y)) and max |y/gamma - ell(y)| per lane.
@@ -660,12 +660,14 @@ __device__ __noinline__ DotMax<T> reduceEll(const LaneInst<T> *Pp, T *x, T *y, d
     const LV<T> yd = yv.shift(P.sPrim);
     Red<T> h{P, xv, yv, T(0), T(0)};
     T dot = T(0), mx = T(0);
+    T cross = T(0);
     auto pv = [&](long e) { return yv[e]; };
     for (int n = c.wave; n < P.N; n += c.nw) {
         adjNode(P, yd, n, [&](int kind, int k, T v) {
             const long e = primIndex(P, n, kind, k);
-            const T yk = yv[e];
-            dot += xv[e] * (v * (-P.gamma) + yk);
+            const T yk = yv[e], xk = xv[e];
+            dot += xk * (v * (-P.gamma) + yk);
+            cross += xk * v;
             mx = fmax(mx, fabs(yk * P.gammaRecip - v));
         });
         opNode(P, n, pv, h);
@@ -673,6 +675,38 @@ __device__ __noinline__ DotMax<T> reduceEll(const LaneInst<T> *Pp, T *x, T *y, d
     DotMax<T> r;
     r.dot = (T) laneSum((double) (dot + h.dot), lds, c);
     r.mx = (T) laneMax((double) fmax(mx, h.mx), lds, c);
+    r.cross = (T) laneSum((double) cross, lds, c);
+    return r;
+}
+
+template <typename T>
+struct Cross2 {
+    T c1, c2;
+};
+
+// The adjoint-only terms of <x, M x> and <x, M y> (M = [[I, -gamma L*], [-gamma L, I]], and
+// x_d' L y_p = (L* x_d)' y_p):  c1 = sum_p x_p . (L* x_d),  c2 = sum_p x_p . (L* y_d) + (L* x_d) . y_p.
+template <typename T>
+__device__ __noinline__ Cross2<T> adjDots(const LaneInst<T> *Pp, T *x, T *y, double *lds, bool withY) {
+    LP<T> P = *cst(uniformPtr(Pp));
+    x = uniformPtr(x);
+    y = uniformPtr(y);
+    const Ctx c = ctx();
+    const LV<T> xv = lview(x, P.sIter, c.lane), yv = lview(y, P.sIter, c.lane);
+    const LV<T> xd = xv.shift(P.sPrim), yd = yv.shift(P.sPrim);
+    T c1 = T(0), c2 = T(0);
+    for (int n = c.wave; n < P.N; n += c.nw) {
+        adjNode(P, xd, n, [&](int kind, int k, T v) {
+            const long e = primIndex(P, n, kind, k);
+            c1 += xv[e] * v;
+            if (withY) c2 += v * yv[e];
+        });
+        if (withY)
+            adjNode(P, yd, n, [&](int kind, int k, T v) { c2 += xv[primIndex(P, n, kind, k)] * v; });
+    }
+    Cross2<T> r;
+    r.c1 = (T) laneSum((double) c1, lds, c);
+    r.c2 = withY ? (T) laneSum((double) c2, lds, c) : T(0);
     return r;
 }
 
@@ -788,8 +822,22 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
     T *Z = P.work[0], *C = P.work[1], *ZP = P.work[2], *RR = P.work[3], *RP = P.work[4], *BK = P.work[5];
     T *C2 = P.work[6], *D = P.work[7], *DS = P.work[8];
     const long t0 = wall_clock64();
+    const bool resume = P.resume != 0;
+    LaneGroupSave *gs = P.gsave + blockIdx.x;
+    long itStart = 0, ticksBefore = 0;
+    int headSaved = m - 1;
+    if (resume) {  // roles of the six rotating buffers, ring head, next iteration (uniform)
+        T **roles[6] = {&Z, &C, &ZP, &RR, &RP, &BK};
+#pragma unroll
+        for (int i = 0; i < 6; i++) *roles[i] = P.work[__builtin_amdgcn_readfirstlane(gs->roles[i])];
+        headSaved = __builtin_amdgcn_readfirstlane(gs->head);
+        const long itv = gs->it;
+        itStart = ((long) __builtin_amdgcn_readfirstlane((int) (itv >> 32)) << 32) |
+                  (unsigned) __builtin_amdgcn_readfirstlane((int) itv);
+        ticksBefore = gs->ticks;
+    }
     // initialiseIter / initialisePrev (src/cache.cuh:611-632)
-    {
+    if (!resume) {
         const LV<T> zio = V(P.zio), rio = V(P.rio), z = V(Z), zp = V(ZP), rp = V(RP);
         const bool spock = P.method == METHOD_SPOCK, warm = P.warm != 0;
         pass<16>(
@@ -810,7 +858,7 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
 #pragma unroll
         for (int j = 0; j < MAXM; j++) G[i][j] = 0.;
     }
-    int head = m - 1;
+    int head = headSaved;
     bool active = real;
     const T given = (real && P.tolFixed) ? P.tolFixed[gl] : T(0);
     bool initFlag = !(given > T(0));
@@ -821,13 +869,38 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
     const T inf = T(INFINITY);
     T zeta = inf, w = inf, wSafe = inf;
     long k0 = 0, k1 = 0, k2 = 0, k2bt = 0, k3 = 0, nT = 0, nEll = 0;
+    if (resume) {  // this lane's SuperMann state
+        const LaneSave &sv = P.save[gl];
+#pragma unroll
+        for (int i = 0; i < MAXM; i++) {
+            gb[i] = sv.gb[i];
+#pragma unroll
+            for (int j = 0; j < MAXM; j++) G[i][j] = sv.G[i][j];
+        }
+        tol = (T) sv.tol;
+        lastErr = (T) sv.lastErr;
+        wLast = (T) sv.wLast;
+        zeta = (T) sv.zeta;
+        w = (T) sv.w;
+        wSafe = (T) sv.wSafe;
+        iters = sv.iters;
+        k0 = sv.k0; k1 = sv.k1; k2 = sv.k2; k2bt = sv.k2bt; k3 = sv.k3; nT = sv.nT; nEll = sv.nEll;
+        status = sv.status;
+        active = sv.active != 0;
+        initFlag = sv.initFlag != 0;
+    }
     // the iteration cap is only tested at error checks (src/cache.cuh:937-940): the last outer
     // iteration is the first check index at or beyond maxIters
     const long hardCap =
         P.maxIters > 0 ? (P.maxIters + P.checkPeriod - 1) / P.checkPeriod * P.checkPeriod + 1 : 2000000;
     long it;
-    for (it = 0; it < hardCap; it++) {
+    bool chunkExit = false;
+    for (it = itStart; it < hardCap; it++) {
         if (!anyLane(active)) break;
+        if (P.chunkIters > 0 && it >= itStart + P.chunkIters) {  // uniform: save and leave
+            chunkExit = true;
+            break;
+        }
         if (it != 0 && P.method == METHOD_SPOCK) {  // acceptCandidate: z <- cand
             T *t = Z;
             Z = C;
@@ -837,18 +910,33 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
         applyT(Pp, Z, C, phaseAcc);
         clk.skip();
         if (active) nT++;
-        {  // r = z - T(z)
+        T sRR;
+        {  // r = z - T(z) and ||r||^2
             const LV<T> rr = V(RR), z = V(Z), cc = V(C);
-            pass<16>(n, c, [&](long e) { return V2<T>{z[e], cc[e]}; }, [&](long e, const V2<T> &v) { rr[e] = v.a - v.b; });
+            T acc = T(0);
+            pass<16>(n, c, [&](long e) { return V2<T>{z[e], cc[e]}; }, [&](long e, const V2<T> &v) {
+                const T d = v.a - v.b;
+                rr[e] = d;
+                acc += d * d;
+            });
+            sRR = (T) laneSum((double) acc, lds, c);
         }
-        __syncthreads();
         clk.mark(PH_RES);
-        const DotMax<T> dm = reduceEll(Pp, RR, RR, lds);
-        const T dotRR = dm.dot, mx = dm.mx;
+        // <r, M r> = ||r||^2 - 2 gamma r_p . L* r_d; the full [L*; L] only at the error checks
+        const bool checkIt = it % P.checkPeriod == 0;
+        T mx = T(0), crossRR;
+        if (checkIt) {
+            const DotMax<T> dm = reduceEll(Pp, RR, RR, lds);
+            mx = dm.mx;
+            crossRR = dm.cross;
+        } else {
+            crossRR = adjDots(Pp, RR, RR, lds, false).c1;
+        }
+        const T dotRR = sRR - T(2) * P.gamma * crossRR;
         if (active) nEll++;
         clk.mark(PH_ELL);
         bool done = false;
-        if (active && it % P.checkPeriod == 0) {  // computeError (src/cache.cuh:910-961)
+        if (active && checkIt) {  // computeError (src/cache.cuh:910-961)
             if (initFlag) {
                 tol = fmax(P.tolAbs, P.tolRel * mx);
                 initFlag = false;
@@ -857,7 +945,8 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
                 int code = ST_NOTRUN;
                 if (mx <= tol) code = ST_CONVERGED;
                 if (P.maxIters && it >= P.maxIters) code = ST_OUTOFITER;
-                if (P.maxTicks > 0 && it % 1000 == 0 && (double) (wall_clock64() - t0) >= P.maxTicks)
+                if (P.maxTicks > 0 && it % 1000 == 0 &&
+                    (double) (wall_clock64() - t0 + ticksBefore) >= P.maxTicks)
                     code = ST_OUTOFTIME;
                 if (code != ST_NOTRUN) {
                     status = code;
@@ -1016,14 +1105,25 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
             applyT(Pp, Z, C2, phaseAcc);
             clk.skip();
             if (ls) nT++;
-            {
-                const LV<T> rr = V(RR), z = V(Z), cc = V(C2);
-                pass<16>(n, c, [&](long e) { return V2<T>{z[e], cc[e]}; },
-                        [&](long e, const V2<T> &v) { rr[e] = v.a - v.b; });
+            T sRRt, sRD;
+            {  // r~ = z - T(z), ||r~||^2 and r~ . ds
+                const LV<T> rr = V(RR), z = V(Z), cc = V(C2), ds = V(DS);
+                T a1 = T(0), a2 = T(0);
+                pass<16>(n, c, [&](long e) { return V4<T>{z[e], cc[e], ds[e], T(0)}; },
+                        [&](long e, const V4<T> &v) {
+                            const T d = v.a - v.b;
+                            rr[e] = d;
+                            a1 += d * d;
+                            a2 += d * v.c;
+                        });
+                sRRt = (T) laneSum((double) a1, lds, c);
+                sRD = (T) laneSum((double) a2, lds, c);
             }
-            __syncthreads();
             clk.mark(PH_RES);
-            const T wTilde = sqrt(reduceEll(Pp, RR, RR, lds).dot);
+            // <r~, M r~> and <r~, M ds> from two adjoints (src/cache.cuh:1066,1080)
+            const Cross2<T> cr = adjDots(Pp, RR, DS, lds, true);
+            const T wTilde = sqrt(sRRt - T(2) * P.gamma * cr.c1);
+            const T dotRD = sRD - P.gamma * cr.c2;
             if (ls) nEll++;
             clk.mark(PH_ELL);
             const bool takeK1 = ls && (w <= wSafe) && (wTilde <= c1 * w);
@@ -1039,13 +1139,6 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
                 k1++;
             }
             const bool still = ls && !takeK1;
-            T dotRD = T(0);
-            if (anyLane(still)) {
-                clk.mark(PH_LINE);
-                dotRD = reduceEll(Pp, RR, DS, lds).dot;
-                if (still) nEll++;
-                clk.mark(PH_ELL);
-            }
             const T rho = wTilde * wTilde - dotRD;
             const bool takeK2 = still && (rho >= sigma * wTilde * w);
             if (anyLane(takeK2)) {  // K2: cand <- z_prev - f r
@@ -1077,7 +1170,37 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
             clk.mark(PH_LINE);
         }
     }
-    const bool capped = active;
+    if (chunkExit) {  // keep everything a resume needs; results report the progress so far
+        if (threadIdx.x == 0) {
+            T *cur[6] = {Z, C, ZP, RR, RP, BK};
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++)
+                    if (cur[i] == P.work[j]) gs->roles[i] = j;
+            gs->head = head;
+            gs->it = it;
+            gs->ticks = ticksBefore + (long) (wall_clock64() - t0);
+        }
+        if (c.wave == 0) {
+            LaneSave &sv = P.save[gl];
+            for (int i = 0; i < MAXM; i++) {
+                sv.gb[i] = gb[i];
+                for (int j = 0; j < MAXM; j++) sv.G[i][j] = G[i][j];
+            }
+            sv.tol = (double) tol;
+            sv.lastErr = (double) lastErr;
+            sv.wLast = (double) wLast;
+            sv.zeta = (double) zeta;
+            sv.w = (double) w;
+            sv.wSafe = (double) wSafe;
+            sv.iters = active ? it : iters;
+            sv.k0 = k0; sv.k1 = k1; sv.k2 = k2; sv.k2bt = k2bt; sv.k3 = k3; sv.nT = nT; sv.nEll = nEll;
+            sv.status = status;
+            sv.active = active ? 1 : 0;
+            sv.initFlag = initFlag ? 1 : 0;
+        }
+        if (active) iters = it;
+    }
+    const bool capped = active && !chunkExit;
     if (anyLane(capped)) {  // iteration cap of the launch: the current iterate is the output
         const LV<T> zio = V(P.zio), rio = V(P.rio), z = V(Z), rr = V(RR);
         pass<16>(

@@ -22,7 +22,7 @@ EXPORTS = [
     "spk_ctx_synchronize", "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
     "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_reset",
     "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
-    "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve",
+    "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve", "spk_batch_resume",
     "spk_batch_results", "spk_batch_debug_op", "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
     "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
     "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
@@ -34,7 +34,7 @@ class Options(C.Structure):
     _fields_ = [("tol_abs", C.c_double), ("tol_rel", C.c_double), ("max_time_secs", C.c_double),
                 ("max_iters", C.c_long), ("max_inner", C.c_int), ("anderson_buffer", C.c_int),
                 ("check_period", C.c_int), ("method", C.c_int), ("block_size", C.c_int),
-                ("ls_method", C.c_int), ("dyn_inverse", C.c_int)]
+                ("ls_method", C.c_int), ("dyn_inverse", C.c_int), ("chunk_iters", C.c_long)]
 
 
 class Result(C.Structure):
@@ -99,6 +99,7 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_iterate_ptr.restype = vp
     lib.spk_batch_copy_iterates.argtypes = [vp, vp]
     lib.spk_batch_solve.argtypes = [vp, C.POINTER(Options), C.c_int]
+    lib.spk_batch_resume.argtypes = [vp]
     lib.spk_batch_results.argtypes = [vp, C.POINTER(Result), _dp]
     lib.spk_batch_debug_op.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     lib.spk_batch_set_iterates_device.argtypes = [vp, vp, C.c_long]
@@ -264,6 +265,24 @@ class Batch:
     def solve(self, opts=None, warm=False):
         o = opts if opts is not None else options()
         self.ctx.check(self.ctx.lib.spk_batch_solve(self.h, C.byref(o), 1 if warm else 0), "solve")
+
+    def resume(self):
+        """Continue a chunked solve (shared batches, options.chunk_iters > 0)."""
+        self.ctx.check(self.ctx.lib.spk_batch_resume(self.h), "resume")
+
+    def solve_chunked(self, opts, warm=False, on_chunk=None):
+        """Run a chunked solve to its end; on_chunk(results, kernel_seconds) after every chunk.
+        Returns the final results and the summed kernel seconds."""
+        self.solve(opts, warm)
+        total = 0.0
+        while True:
+            res, sec = self.results()
+            total += sec
+            if on_chunk is not None:
+                on_chunk(res, sec)
+            if all(r["status"] != -1 for r in res):
+                return res, total
+            self.resume()
 
     def results(self):
         res = (Result * len(self.problems))()

@@ -1,8 +1,10 @@
 """Shared-problem batches (the lane engine: 64 instances per wave, interleaved layout) vs the
 FP64 CPU oracle, through the C ABI.
 
-The lane engine solves its Anderson least squares from the Gram matrix, so whole-iteration parity
-is checked against the oracle run with the same Gram solve (``ls="gram"``).  The Gram solve is
+The lane engine solves its Anderson least squares from the Gram matrix and forms SuperMann's
+M-inner products from adjoints only (<x, M y> = x'y - gamma (x_p' L* y_d + (L* x_d)' y_p)), so
+whole-iteration parity is checked against the oracle run the same way (``ls="gram"``,
+``adjoint_dots=True``).  The Gram solve is
 less well conditioned than Householder, so rounding differences in the accumulated Gram entries
 grow faster along SuperMann; iterate parity is checked over the first 50 SPOCK iterations and 250
 CP iterations.  Operators agree to FP64 rounding.  Batches of 70 instances exercise two
@@ -101,7 +103,7 @@ def test_shared_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, me
     b.solve(rt.options(tol_abs=1e-30, max_iters=iters, method=rt.SPOCK if method == "spock" else rt.CP))
     res, _ = b.results()
     for k in range(count):
-        ref = o.solve(x0s[k], method=method, tol_abs=1e-30, max_iters=iters, ls="gram")
+        ref = o.solve(x0s[k], method=method, tol_abs=1e-30, max_iters=iters, ls="gram", adjoint_dots=True)
         r = res[k]
         assert r["status"] == ref["status"] == 1
         assert r["iters"] == ref["iters"]
@@ -203,3 +205,27 @@ def test_shared_layout_round_trips(ctx, problem_dirs):
     assert rel_err(b2.iterates(), Z) < 1e-7
     b.reset()
     assert not np.any(b.iterates())
+
+
+@pytest.mark.parametrize("dtype", [rt.F64, rt.F32])
+def test_chunked_solve_equals_uninterrupted(ctx, problem_dirs, dtype):
+    """A solve cut into launches of 37 outer iterations (resumed from the saved device state) gives
+    bit-identical iterates, statuses and branch counts."""
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    count = 70
+    x0s = x0s_for("small", o.nx, count)
+    p, a = shared(ctx, d, count, dtype)
+    a.set_x0(x0s)
+    opts = dict(tol_abs=1e-3, tol_rel=1e-3, max_iters=500)
+    a.solve(rt.options(**opts))
+    ra, _ = a.results()
+    b = rt.Batch(ctx, [p] * count, dtype, shared=True)
+    b.set_x0(x0s)
+    chunks = []
+    rb, _ = b.solve_chunked(rt.options(chunk_iters=37, **opts), on_chunk=lambda r, s: chunks.append(r))
+    assert len(chunks) > 3
+    for k in range(count):
+        for f in ("status", "iters", "k0", "k1", "k2", "k2bt", "k3", "num_T"):
+            assert ra[k][f] == rb[k][f], (k, f)
+    assert np.array_equal(a.iterates(), b.iterates())
