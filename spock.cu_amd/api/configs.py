@@ -11,6 +11,12 @@ Choices the family descriptions leave open, fixed here:
 * The terminal weight of the 50/20 family is the discrete Riccati solution of
   event 0's (A, B) with Q = I, R = 0.1 I.
 * Leaf boxes repeat the nonleaf state bounds.
+* The polyhedral family (4-event i.i.d. tree, nx=100, nu=40): the i.i.d. distribution is a
+  Dirichlet(1) draw, Q = I, R = 0.1 I, leaf P = I, the linear weights q, r have standard
+  deviation 0.1, and the 2 nx constraint rows (over [x; u] at nonleaf nodes, over x at leaves)
+  have N(0, 1) entries with bounds -5 <= row . v <= 5.  Initial states for it are drawn with
+  standard deviation 0.05 (``initial_states(..., kind="small")``): at 0.5 the rows of the
+  polyhedron at the root would be violated with no input able to fix them.
 """
 import os
 import numpy as np
@@ -98,14 +104,44 @@ def uneven_instance(folder, k, dt="d"):
     return _box_family(folder, rng, tr, nx, nu, 0.02, 0.1, 10.0, 1.0, alpha, True, dt)
 
 
+def polyhedral_iid(folder, seed=3, dt="d", events=4, horizon=6, nx=100, nu=40):
+    """i.i.d. tree with branching at every stage; quadratic-plus-linear costs; polyhedral
+    constraints of 2 nx rows; AV@R 0.95.  The defaults are the 5461-node nx=100/nu=40 tree."""
+    rng = np.random.default_rng(seed)
+    tr = tree_mod.IidProcess(rng.dirichlet(np.ones(events)), horizon, horizon, dt=dt).build()
+    dyn = []
+    for _ in range(events):
+        A = np.eye(nx) + 0.02 * rng.standard_normal((nx, nx))
+        B = 0.1 * rng.standard_normal((nx, nu))
+        dyn.append(build.Dynamics(A, B))
+    Q, R = np.eye(nx), 0.1 * np.eye(nu)
+    costs = [build.CostQuadraticPlusLinear(Q, 0.1 * rng.standard_normal(nx), R, 0.1 * rng.standard_normal(nu))
+             for _ in range(events)]
+    leaf = build.CostQuadraticPlusLinear(np.eye(nx), 0.1 * rng.standard_normal(nx), leaf=True)
+    rows = 2 * nx
+    ncon = build.Polyhedron(rng.standard_normal((rows, nx + nu)), -5.0 * np.ones(rows), 5.0 * np.ones(rows))
+    lcon = build.Polyhedron(rng.standard_normal((rows, nx)), -5.0 * np.ones(rows), 5.0 * np.ones(rows))
+    return (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(ncon).with_constraint_leaf(lcon)
+            .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
+
+
+def polyhedral_small(folder, seed=33, dt="d"):
+    """The polyhedral family at test size: 3 events, horizon 3 (40 nodes), nx=4, nu=2."""
+    return polyhedral_iid(folder, seed=seed, dt=dt, events=3, horizon=3, nx=4, nu=2)
+
+
 def initial_states(seed, nx, count, kind="normal"):
-    """Initial states: ``normal`` ~ N(0, 0.5^2 I), ``uniform`` ~ U[-1, 1]."""
+    """Initial states: ``normal`` ~ N(0, 0.5^2 I), ``small`` ~ N(0, 0.05^2 I), ``uniform`` ~ U[-1, 1]."""
     rng = np.random.default_rng([seed, 1])
     if kind == "uniform":
         return rng.uniform(-1.0, 1.0, size=(count, nx))
+    if kind == "small":
+        return 0.05 * rng.standard_normal((count, nx))
     return 0.5 * rng.standard_normal((count, nx))
 
 
 def generate(name, folder, **kw):
     os.makedirs(folder, exist_ok=True)
-    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov}[name](folder, **kw)
+    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov, "poly": polyhedral_small,
+            "large": polyhedral_iid}[name](folder, **kw)

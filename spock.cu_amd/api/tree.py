@@ -118,6 +118,12 @@ class Tree:
         for name, v in {"probabilities": self.prob, "conditionalProbabilities": self.cond_prob}.items():
             write_bt(os.path.join(folder, f"{name}_{self.dt}.bt"), v, self.dt)
 
+    def write_to_file_fp(self, folder, name, tensor):
+        """A floating-point tensor in the tree's precision, e.g. the drivers' ``initialState``
+        (``api/python/spock/tree.py:237-238``, ``examples/random/main.py:128``)."""
+        os.makedirs(folder, exist_ok=True)
+        write_bt(os.path.join(folder, f"{name}_{self.dt}.bt"), np.asarray(tensor, dtype=np.float64), self.dt)
+
     def __repr__(self):
         return (f"Scenario tree with {self.num_nodes} nodes, {self.num_stages} stages "
                 f"and {len(self.nodes_of_stage(self.num_stages - 1))} scenarios")

@@ -18,9 +18,9 @@
 
 namespace spk {
 template <typename T, int R, int V>
-__global__ void k_solve(const Inst<T> *, Out *, int, double, int);
+__global__ void k_solve(const Inst<T> *, Out *, int, double, int, int);
 template <typename T, int R, int V>
-__global__ void k_debug(const Inst<T> *, Out *, int, int, int);
+__global__ void k_debug(const Inst<T> *, Out *, int, int, int, int);
 __global__ void k_f32_to_f64(const float *, double *, long);
 __global__ void k_f64_to_f32(const double *, float *, long);
 template <typename T>
@@ -70,6 +70,8 @@ struct spk_batch {
     size_t coreElems = 0, andElems = 0, x0Elems = 0;
     int mAlloc = 0;
     DevBuf core, anders, x0buf, dInst, dOut;  // x0buf: scaled initial states (inputs, kept by reset)
+    int wgs = 0;    // workgroups per instance (0: automatic, see gridWorkgroups)
+    DevBuf gpart;   // grid-mode reduction partials (single-instance batches)
     std::vector<char> hInst;  // array of Inst<T>
     std::vector<double> tolFixed;  // per-instance given tolerance (0 = calibrate at the first check)
     // shared-problem batch in the instance-interleaved layout (lane engine)
@@ -183,6 +185,7 @@ double tolScaleOf(const HostProblem &h) {
 
 template <typename T>
 void buildInsts(spk_batch *b, const spk_options &o, int warm) {
+    if (b->count == 1 && !b->gpart.p) b->gpart.alloc((size_t) 2 * spk::GRID_MAXK * spk::GRID_MAXWG * sizeof(T));
     b->hInst.assign(sizeof(spk::Inst<T>) * b->count, 0);
     auto *arr = reinterpret_cast<spk::Inst<T> *>(b->hInst.data());
     T *core = static_cast<T *>(b->core.p);
@@ -231,6 +234,7 @@ void buildInsts(spk_batch *b, const spk_options &o, int warm) {
         I.lsMethod = o.ls_method;
         I.dynInverse = o.dyn_inverse;
         I.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
+        I.gpart = static_cast<T *>(b->gpart.p);
     }
     b->dInst.alloc(b->hInst.size());
     check(hipMemcpyAsync(b->dInst.p, b->hInst.data(), b->hInst.size(), hipMemcpyHostToDevice, b->ctx->stream),
@@ -301,6 +305,47 @@ int ldsPoolInts(const spk_batch *b) {
     return mx <= budget ? mx : 0;
 }
 
+// Iterate size from which a single instance is solved by all compute units together by default.
+constexpr size_t GRID_AUTO_ELEMS = 200000;
+
+// Workgroups that cooperate on one instance: 1 for a batch of several instances; for a single
+// instance the count set by spk_batch_set_workgroups, or by default one per compute unit when the
+// iterate has at least GRID_AUTO_ELEMS elements.  Bounded by what can be co-resident, which a
+// cooperative launch requires.
+int gridWorkgroups(spk_batch *b, const void *kernel, int bs, size_t shm) {
+    if (b->count != 1 || b->wgs == 1) return 1;
+    if (b->wgs == 0 && b->probs[0]->hp.sIter < GRID_AUTO_ELEMS) return 1;
+    const int dev = b->ctx->device;
+    int coop = 0, cus = 0, perCU = 0;
+    check(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev), "device attribute");
+    if (!coop) {
+        if (b->wgs > 1) throw std::runtime_error("the device does not support cooperative launches");
+        return 1;
+    }
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, bs, shm), "occupancy");
+    const int most = std::min(spk::GRID_MAXWG, cus * perCU);
+    if (most < 1) throw std::runtime_error("the solve kernel does not fit on a compute unit");
+    if (b->wgs > most)
+        throw std::runtime_error("at most " + std::to_string(most) + " workgroups of " + std::to_string(bs) +
+                                 " threads can cooperate on one instance on this device");
+    return b->wgs > 1 ? b->wgs : std::min(cus, most);
+}
+
+template <typename K, typename... A>
+void launchMaybeGrid(spk_batch *b, K kernel, int bs, size_t shm, A... args) {
+    const void *fn = reinterpret_cast<const void *>(kernel);
+    const int g = gridWorkgroups(b, fn, bs, shm);
+    int grid = g > 1 ? 1 : 0;
+    if (!grid) {
+        hipLaunchKernelGGL(kernel, dim3(b->count), dim3(bs), shm, b->ctx->stream, args..., grid);
+        return;
+    }
+    void *argv[] = {static_cast<void *>(&args)..., static_cast<void *>(&grid)};
+    check(hipLaunchCooperativeKernel(fn, dim3(g), dim3(bs), argv, (unsigned) shm, b->ctx->stream),
+          "cooperative launch");
+}
+
 template <typename T>
 void launchKernel(spk_batch *b, int bs, bool debug, int op) {
     const int cls = chunkClass(b);
@@ -308,13 +353,14 @@ void launchKernel(spk_batch *b, int bs, bool debug, int op) {
     const size_t shm = (size_t) pool * sizeof(int);
     auto *ins = static_cast<const spk::Inst<T> *>(b->dInst.p);
     auto *outs = static_cast<spk::Out *>(b->dOut.p);
-    hipStream_t st = b->ctx->stream;
+    const int count = b->count;
+    const double tick = b->ctx->tickRateHz;
     if (!debug) {
-        if (cls == 1) hipLaunchKernelGGL((spk::k_solve<T, 1, 1>), dim3(b->count), dim3(bs), shm, st, ins, outs, b->count, b->ctx->tickRateHz, pool);
-        else hipLaunchKernelGGL((spk::k_solve<T, 4, 4>), dim3(b->count), dim3(bs), shm, st, ins, outs, b->count, b->ctx->tickRateHz, pool);
+        if (cls == 1) launchMaybeGrid(b, spk::k_solve<T, 1, 1>, bs, shm, ins, outs, count, tick, pool);
+        else launchMaybeGrid(b, spk::k_solve<T, 4, 4>, bs, shm, ins, outs, count, tick, pool);
     } else {
-        if (cls == 1) hipLaunchKernelGGL((spk::k_debug<T, 1, 1>), dim3(b->count), dim3(bs), shm, st, ins, outs, b->count, op, pool);
-        else hipLaunchKernelGGL((spk::k_debug<T, 4, 4>), dim3(b->count), dim3(bs), shm, st, ins, outs, b->count, op, pool);
+        if (cls == 1) launchMaybeGrid(b, spk::k_debug<T, 1, 1>, bs, shm, ins, outs, count, op, pool);
+        else launchMaybeGrid(b, spk::k_debug<T, 4, 4>, bs, shm, ins, outs, count, op, pool);
     }
 }
 
@@ -676,6 +722,20 @@ int spk_batch_destroy(spk_batch *b) {
 }
 
 int spk_batch_count(const spk_batch *b) { return b->count; }
+
+int spk_batch_set_workgroups(spk_batch *b, int workgroups) {
+    try {
+        if (workgroups < 0 || workgroups > spk::GRID_MAXWG)
+            throw std::runtime_error("workgroups must be in 0.." + std::to_string(spk::GRID_MAXWG));
+        if (b->lanes) throw std::runtime_error("shared batches run one workgroup per 64 instances");
+        if (workgroups > 1 && b->count != 1)
+            throw std::runtime_error("several workgroups per instance need a batch of one instance");
+        b->wgs = workgroups;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
 
 int spk_batch_reset(spk_batch *b) {
     try {

@@ -34,7 +34,12 @@ struct Inst {
     long maxIters;
     int maxInner, andBuff, checkPeriod, method, warm, lsMethod, dynInverse;
     double maxTicks;  // wall-clock cap in device ticks (0 = none)
+    T *gpart;         // grid mode: 2 x GRID_MAXK x GRID_MAXWG reduction partials
 };
+
+// Grid mode limits: values per grid reduction and workgroups per cooperative launch.
+constexpr int GRID_MAXK = 16;
+constexpr int GRID_MAXWG = 1024;
 
 // Block-interleaved layout of a region of `rows` rows: instances are grouped in blocks of 64 and
 // each block stores rows x 64 words contiguously, element e of instance b at

@@ -1,5 +1,9 @@
 // Fused SPOCK engine for gfx950: one workgroup owns one problem instance and runs the
 // whole SuperMann/Anderson solve (src/cache.cuh:1008-1132) inside a single launch.
+// Grid mode: for one large tree, every workgroup of a cooperative launch works on the same
+// instance; the work distribution below then spans the grid, barriers are grid barriers and
+// reductions combine per-workgroup partials (in a fixed order, so every workgroup takes the
+// same branch decisions).
 //
 // Work decomposition inside the workgroup: a "node task" is handled by one 64-lane
 // wavefront with lanes over the rows of the node's blocks (node data are column major,
@@ -11,6 +15,7 @@
 // reference leaves unwritten (t[0] of L*, the IV block of node 0 of L) are zero here.
 #include "engine.hpp"
 
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -46,18 +51,62 @@ __device__ __forceinline__ T waveMax(T v) {
     return v;
 }
 
+// Work distribution of a thread: tid/nt and wave/nw index the threads and waves that share one
+// instance (the workgroup, or the whole grid in grid mode); ltid/lw/lnw are workgroup-local.
 struct Tc {
     int lane, wave, nw, tid, nt;
+    int ltid, lw, lnw;
+    int grid;
+    void *gpart;      // grid mode: two buffers of GRID_MAXK x GRID_MAXWG reduction partials
+    mutable int seq;  // reductions done so far; selects the partials buffer
 };
 
-__device__ __forceinline__ Tc threadCtx() {
+__device__ __forceinline__ Tc threadCtx(int grid = 0, void *gpart = nullptr) {
     Tc c;
-    c.tid = threadIdx.x;
-    c.nt = blockDim.x;
+    c.ltid = threadIdx.x;
+    c.lw = c.ltid >> 6;
+    c.lnw = blockDim.x >> 6;
+    c.grid = grid;
+    c.gpart = gpart;
+    c.seq = 0;
+    c.tid = grid ? (int) (blockIdx.x * blockDim.x + threadIdx.x) : c.ltid;
+    c.nt = grid ? (int) (gridDim.x * blockDim.x) : (int) blockDim.x;
     c.lane = c.tid & 63;
     c.wave = c.tid >> 6;
     c.nw = c.nt >> 6;
     return c;
+}
+
+// Barrier over the threads that share the instance.
+__device__ __forceinline__ void sync(const Tc &c) {
+    if (c.grid) cooperative_groups::this_grid().sync();
+    else __syncthreads();
+}
+
+// Grid mode: combine the workgroup values v[0..K) of every workgroup.  Each workgroup publishes its
+// values, then every wave reads all partials lane-strided and reduces them with the same shuffle
+// tree, so all threads of the grid hold bitwise identical results.  Two partial buffers alternate:
+// a workgroup can only start overwriting one after every workgroup passed the barrier of the
+// reduction in between, which is after its reads of the same buffer.
+template <typename T, int K, bool MAX>
+__device__ __forceinline__ void gridCombine(T (&v)[K], const Tc &c) {
+    static_assert(K <= GRID_MAXK, "too many values per grid reduction");
+    T *part = static_cast<T *>(c.gpart) + (size_t) (c.seq & 1) * GRID_MAXK * GRID_MAXWG;
+    c.seq++;
+    if (c.ltid == 0)
+#pragma unroll
+        for (int i = 0; i < K; i++) part[i * GRID_MAXWG + blockIdx.x] = v[i];
+    cooperative_groups::this_grid().sync();
+    const int nb = gridDim.x;
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        T a = T(0);
+        for (int b = c.lane; b < nb; b += 64) {
+            const T p = part[i * GRID_MAXWG + b];
+            a = MAX ? fmax(a, p) : a + p;
+        }
+        v[i] = MAX ? waveMax(a) : waveSum(a);
+    }
 }
 
 // Per-phase wall-clock accounting; mark() is called right after a workgroup barrier so the
@@ -136,25 +185,31 @@ __device__ __forceinline__ void blockSum(T (&v)[K], T *lds, const Tc &c) {
     for (int i = 0; i < K; i++) v[i] = waveSum(v[i]);
     if (c.lane == 0)
 #pragma unroll
-        for (int i = 0; i < K; i++) lds[i * 32 + c.wave] = v[i];
+        for (int i = 0; i < K; i++) lds[i * 32 + c.lw] = v[i];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < K; i++) {
         T s = T(0);
-        for (int w = 0; w < c.nw; w++) s += lds[i * 32 + w];
+        for (int w = 0; w < c.lnw; w++) s += lds[i * 32 + w];
         v[i] = s;
     }
     __syncthreads();
+    if (c.grid) gridCombine<T, K, false>(v, c);
 }
 
 template <typename T>
 __device__ __forceinline__ T blockMax(T v, T *lds, const Tc &c) {
     v = waveMax(v);
-    if (c.lane == 0) lds[c.wave] = v;
+    if (c.lane == 0) lds[c.lw] = v;
     __syncthreads();
     T s = T(0);
-    for (int w = 0; w < c.nw; w++) s = fmax(s, lds[w]);
+    for (int w = 0; w < c.lnw; w++) s = fmax(s, lds[w]);
     __syncthreads();
+    if (c.grid) {
+        T a[1] = {s};
+        gridCombine<T, 1, true>(a, c);
+        s = a[0];
+    }
     return s;
 }
 
@@ -706,16 +761,16 @@ __device__ __noinline__ void phaseProjections(const Inst<T> &P, const Tc &c, T *
                 if (task < nStage) dynBackwardNode<T, R, V>(P, cand, f + task, stage, c.lane);
                 else kernelNode<T, R, V>(P, cand, task - nStage, c.lane);
             }
-            __syncthreads();
+            sync(c);
         }
         for (int stage = 0; stage <= P.S - 2; stage++) {
             const int f = P.stageFrom[stage], nStage = P.stageTo[stage] - f + 1;
             for (int task = c.wave; task < nStage; task += c.nw) dynForwardNode<T, R, V>(P, cand, f + task, c.lane);
-            __syncthreads();
+            sync(c);
         }
     } else if (doKernel) {
         for (int task = c.wave; task < P.nl; task += c.nw) kernelNode<T, R, V>(P, cand, task, c.lane);
-        __syncthreads();
+        sync(c);
     }
 }
 
@@ -725,14 +780,14 @@ __device__ __noinline__ void phaseDualProx(const Inst<T> &P, const Tc &c, const 
     ProxL<T, R> h{P, z + P.sPrim, cand + P.sPrim, c.lane};
     auto pv = [&](long k) { return cand[k] * T(2) - z[k]; };
     for (int n = c.wave; n < P.N; n += c.nw) opNode<T, R, V>(P, n, c.lane, pv, h);
-    __syncthreads();
+    sync(c);
 }
 
 // The CP operator T (src/cache.cuh:796-801): cand = T(z).
 template <typename T, int R, int V>
 __device__ __forceinline__ void applyT(const Inst<T> &P, const Tc &c, const T *z, T *cand, PhaseClock *clk = nullptr) {
     phaseAdjAxpy<T, R, V>(P, c, z, cand);
-    __syncthreads();
+    sync(c);
     if (clk) clk->mark(PH_ADJ);
     phaseProjections<T, R, V>(P, c, cand, true, true);
     if (clk) clk->mark(PH_PROJ);
@@ -907,7 +962,7 @@ __device__ __noinline__ void andersonHouseholder(const Inst<T> &P, const Tc &c, 
 #pragma unroll
     for (int lc = 0; lc < 8; lc++)
         if (lc < m) gs[slot(lc)] = (double) g[lc];
-    __syncthreads();
+    sync(c);
 }
 
 // New Anderson column (src/cache.cuh:863-868): Y_h = r - r_prev, S_h = (z - z_prev) - Y_h.
@@ -927,7 +982,7 @@ __device__ __noinline__ void andersonColumn(const Inst<T> &P, const Tc &c, int h
                 Yh[k] = dr;
                 Sh[k] = (v.c - v.d) - dr;
             });
-        __syncthreads();
+        sync(c);
         return;
     }
     double acc[16];
@@ -953,14 +1008,14 @@ __device__ __noinline__ void andersonColumn(const Inst<T> &P, const Tc &c, int h
             }
         });
     blockSum(acc, ldsd, c);
-    if (c.tid == 0) {
+    if (c.ltid == 0) {
         for (int s2 = 0; s2 < m; s2++) {
             G[head * 8 + s2] = acc[s2];
             G[s2 * 8 + head] = acc[s2];
             gb[s2] = acc[8 + s2];
         }
     }
-    __syncthreads();
+    sync(c);
 }
 
 // Least squares argmin ||Y g - r|| from the Gram matrix: Cholesky of G + eps * max(diag) I in
@@ -1048,7 +1103,7 @@ __device__ __noinline__ void andersonApply(const Inst<T> &P, const Tc &c, const 
                 out[k] = toCand ? v.b + d : d;
             });
     }
-    __syncthreads();
+    sync(c);
 }
 
 // ---------------------------------------------------------------- the solve
@@ -1056,8 +1111,8 @@ __device__ __noinline__ void andersonApply(const Inst<T> &P, const Tc &c, const 
 // The six iterate buffers rotate by pointer instead of being copied: acceptCandidate,
 // backup/restore and saveToPrev are pointer moves.
 template <typename T, int R, int V>
-__device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, double tickRate) {
-    const Tc c = threadCtx();
+__device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, double tickRate, int grid) {
+    const Tc c = threadCtx(grid, P.gpart);
     const long n = P.sIter;
     const int m = P.andBuff;
     const bool gram = P.lsMethod == 1;
@@ -1087,8 +1142,8 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
                 });
         }
     }
-    if (c.tid < 64) G[c.tid] = 0.;
-    __syncthreads();
+    if (c.ltid < 64) G[c.ltid] = 0.;
+    sync(c);
     // a given tolerance (> 0) skips the reference's first-check calibration, so warm starts and
     // polish solves are accepted against the threshold of the cold solve they continue
     bool initFlag = !(P.tolFixed > T(0));
@@ -1116,7 +1171,7 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
         // computeError (src/cache.cuh:910-961): r = z - cand
         stream<8>(
             n, c, [&](long k) { return V2<T>{Z[k], C[k]}; }, [&](long k, const V2<T> &v) { RR[k] = v.a - v.b; });
-        __syncthreads();
+        sync(c);
         clk.mark(PH_RES);
         bool haveDot = false;
         T dotRR = T(0);
@@ -1134,8 +1189,11 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
                 int code = ST_NOTRUN;
                 if (mx <= tol) code = ST_CONVERGED;
                 if (P.maxIters && it >= P.maxIters) code = ST_OUTOFITER;
-                if (P.maxTicks > 0 && it % 1000 == 0 && (double) (wall_clock64() - t0) >= P.maxTicks)
-                    code = ST_OUTOFTIME;
+                if (P.maxTicks > 0 && it % 1000 == 0) {
+                    // threads read the clock at different moments: decide on the latest reading
+                    const T late = (double) (wall_clock64() - t0) >= P.maxTicks ? T(1) : T(0);
+                    if (blockMax(late, lds, c) > T(0)) code = ST_OUTOFTIME;
+                }
                 if (code != ST_NOTRUN) {
                     status = code;
                     iters = it;
@@ -1206,13 +1264,13 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
                     DS[k] = ds;
                     Z[k] = v.b + ds;
                 });
-            __syncthreads();
+            sync(c);
             clk.mark(PH_LINE);
             applyT<T, R, V>(P, c, Z, C, &clk);
             nT++;
             stream<8>(
                 n, c, [&](long k) { return V2<T>{Z[k], C[k]}; }, [&](long k, const V2<T> &v) { RR[k] = v.a - v.b; });
-            __syncthreads();
+            sync(c);
             clk.mark(PH_RES);
             T dot, mx;
             reduceEll<T, R, V>(P, c, RR, RR, lds, dot, mx);
@@ -1241,7 +1299,7 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
                         RR[k] = rk;
                         C[k] = v.b - rk;
                     });
-                __syncthreads();
+                sync(c);
                 k2++;
                 break;
             } else {
@@ -1271,7 +1329,7 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
                 dr[k] = v.b;
             });
     }
-    __syncthreads();
+    sync(c);
     if (c.tid == 0) {
         out.status = status;
         out.iters = iters;
@@ -1344,18 +1402,20 @@ __device__ const Inst<T> &stageInstance(const Inst<T> &G, Inst<T> &L, int *pool,
     return L;
 }
 
+// grid = 0: workgroup k solves instance k.  grid = 1 (cooperative launch, count = 1): every
+// workgroup works on instance 0.
 template <typename T, int R, int V>
 __global__ void __launch_bounds__(SPK_MAX_THREADS) k_solve(const Inst<T> *__restrict__ insts, Out *outs, int count,
-                                                double tickRate, int poolInts) {
+                                                double tickRate, int poolInts, int grid) {
     __shared__ T lds[8 * 32];
     __shared__ double ldsd[16 * 32];
     __shared__ __align__(16) char localBuf[sizeof(Inst<T>)];
     Inst<T> &local = *reinterpret_cast<Inst<T> *>(localBuf);
     extern __shared__ int pool[];
-    const int k = blockIdx.x;
+    const int k = grid ? 0 : (int) blockIdx.x;
     if (k >= count) return;
     const Inst<T> &P = stageInstance(insts[k], local, pool, poolInts);
-    solveInstance<T, R, V>(P, outs[k], lds, ldsd, tickRate);
+    solveInstance<T, R, V>(P, outs[k], lds, ldsd, tickRate, grid);
 }
 
 // ---------------------------------------------------------------- single-operator kernels
@@ -1364,15 +1424,15 @@ enum DebugOp { OP_L = 0, OP_LADJ = 1, OP_DYN = 2, OP_KERNEL = 3, OP_T = 4, OP_EL
 
 template <typename T, int R, int V>
 __global__ void __launch_bounds__(SPK_MAX_THREADS) k_debug(const Inst<T> *__restrict__ insts, Out *outs, int count, int op,
-                                                int poolInts) {
+                                                int poolInts, int grid) {
     __shared__ T lds[8 * 32];
     __shared__ __align__(16) char localBuf[sizeof(Inst<T>)];
     Inst<T> &local = *reinterpret_cast<Inst<T> *>(localBuf);
     extern __shared__ int pool[];
-    const int kInst = blockIdx.x;
+    const int kInst = grid ? 0 : (int) blockIdx.x;
     if (kInst >= count) return;
     const Inst<T> &P = stageInstance(insts[kInst], local, pool, poolInts);
-    const Tc c = threadCtx();
+    const Tc c = threadCtx(grid, P.gpart);
     T *z = P.z, *cand = P.cand;
     if (op == OP_L) {
         StoreL<T, R> h{P, cand + P.sPrim, c.lane};
@@ -1396,12 +1456,12 @@ __global__ void __launch_bounds__(SPK_MAX_THREADS) k_debug(const Inst<T> *__rest
         }
     } else if (op == OP_DYN || op == OP_KERNEL) {
         for (long k = c.tid; k < P.sPrim; k += c.nt) cand[k] = z[k];
-        __syncthreads();
+        sync(c);
         if (op == OP_DYN)
             for (int n = c.wave; n < P.N; n += c.nw)
                 if (n >= P.nl)
                     for (int r = c.lane; r < P.nx; r += 64) P.qd[(long) n * P.nx + r] = -cand[P.oX + (long) n * P.nx + r];
-        __syncthreads();
+        sync(c);
         phaseProjections<T, R, V>(P, c, cand, op == OP_DYN, op == OP_KERNEL);
     } else if (op == OP_T) {
         applyT<T, R, V>(P, c, z, cand);
@@ -1432,8 +1492,8 @@ __global__ void k_fill(T *p, T v, long n) {
 // explicit instantiations: register chunk classes (R, V) = (1,1) for dims <= 64,
 // and (4,4) for dims <= 256
 #define SPK_INST(T, R, V)                                                                  \
-    template __global__ void k_solve<T, R, V>(const Inst<T> *, Out *, int, double, int); \
-    template __global__ void k_debug<T, R, V>(const Inst<T> *, Out *, int, int, int);
+    template __global__ void k_solve<T, R, V>(const Inst<T> *, Out *, int, double, int, int); \
+    template __global__ void k_debug<T, R, V>(const Inst<T> *, Out *, int, int, int, int);
 SPK_INST(double, 1, 1)
 #ifndef SPK_FAST_BUILD
 SPK_INST(double, 4, 4)

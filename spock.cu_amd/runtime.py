@@ -20,7 +20,7 @@ STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
 EXPORTS = [
     "spk_default_options", "spk_ctx_create", "spk_ctx_destroy", "spk_last_error", "spk_ctx_stream",
     "spk_ctx_synchronize", "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
-    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_reset",
+    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_reset",
     "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
     "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve", "spk_batch_resume",
     "spk_batch_results", "spk_batch_debug_op", "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
@@ -88,6 +88,7 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_is_shared.argtypes = [vp]
     lib.spk_batch_destroy.argtypes = [vp]
     lib.spk_batch_count.argtypes = [vp]
+    lib.spk_batch_set_workgroups.argtypes = [vp, C.c_int]
     lib.spk_batch_reset.argtypes = [vp]
     lib.spk_batch_set_x0.argtypes = [vp, _dp]
     lib.spk_batch_set_tolerances.argtypes = [vp, _dp]
@@ -222,6 +223,11 @@ class Batch:
 
     def reset(self):
         self.ctx.check(self.ctx.lib.spk_batch_reset(self.h), "reset")
+
+    def set_workgroups(self, n):
+        """Workgroups that solve the single instance together (0 automatic, 1 one workgroup,
+        n > 1 a cooperative launch of n workgroups)."""
+        self.ctx.check(self.ctx.lib.spk_batch_set_workgroups(self.h, int(n)), "set_workgroups")
 
     def set_x0(self, x0):
         x0 = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in x0])

@@ -1,0 +1,83 @@
+"""The C++ drop-in header (include/spock_amd.hpp): its file readers on the CPU, and the
+reference's timing driver (examples/random/main.cu:24-85, as spock.cu_amd/examples/random_main.cpp)
+built against the header and run on the GPU."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from spock_amd import runtime as rt
+from spock_amd.api import configs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+LIBDIR = os.path.dirname(rt.LIB_PATH)
+ROCM = "/opt/rocm"
+HAVE_GXX = shutil.which("g++") is not None and os.path.isdir(os.path.join(ROCM, "include", "hip"))
+needs_lib = pytest.mark.skipif(not (HAVE_GXX and os.path.exists(rt.LIB_PATH)),
+                               reason="needs g++, the HIP headers and the built library")
+
+READER = r'''
+#include <cstdio>
+#include "spock_amd.hpp"
+using namespace spock_amd;
+int main(int argc, char **argv) {
+    ScenarioTree<double> tree(argv[1]);
+    std::vector<double> x0 = parseFromFile<double>(tree.path() + "initialState" + tree.fpFileExt());
+    std::vector<double> cp = parseFromFile<double>(tree.path() + "conditionalProbabilities" + tree.fpFileExt());
+    std::vector<float> anc = parseFromFile<float>(tree.path() + "ancestors_u.bt");
+    std::printf("%zu %zu %zu %zu %zu %zu\n", tree.numStates(), tree.numInputs(), tree.numNodes(),
+                tree.numNonleafNodes(), tree.numStages(), tree.numEvents());
+    for (double v : x0) std::printf("%.17g ", v);
+    std::printf("\n%.17g %g\n", cp[5], (double) anc[5]);
+    return 0;
+}
+'''
+
+
+def _compile(src, out, tmp_path):
+    path = tmp_path / "prog.cpp"
+    path.write_text(src)
+    cmd = ["g++", "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", f"-I{INC}", f"-I{ROCM}/include", str(path),
+           f"-L{LIBDIR}", "-lspock_amd", f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{LIBDIR}",
+           f"-Wl,-rpath,{ROCM}/lib", "-o", str(out)]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+
+
+def _problem_with_initial_state(folder):
+    prob = configs.generate("small", folder)
+    x0 = configs.initial_states(1, 3, 1, "uniform")[0]
+    prob.tree.write_to_file_fp(folder, "initialState", x0)  # as examples/random/main.py:128
+    return x0
+
+
+@needs_lib
+def test_header_reads_the_problem_folder(tmp_path):
+    folder = str(tmp_path / "p")
+    x0 = _problem_with_initial_state(folder)
+    exe = tmp_path / "reader"
+    _compile(READER, exe, tmp_path)
+    out = subprocess.run([str(exe), folder], check=True, capture_output=True, text=True, timeout=60).stdout.split("\n")
+    assert [int(v) for v in out[0].split()] == [3, 2, 31, 15, 5, 2]
+    assert np.array_equal(np.array([float(v) for v in out[1].split()]), x0)
+    from spock_amd.api import btio
+    cp = btio.read_flat(os.path.join(folder, "conditionalProbabilities_d.bt"))
+    anc = btio.read_flat(os.path.join(folder, "ancestors_u.bt"))
+    assert float(out[2].split()[0]) == cp[5] and float(out[2].split()[1]) == anc[5]
+
+
+@needs_lib
+@pytest.mark.gpu
+def test_reference_driver_runs_on_the_drop_in(tmp_path):
+    folder = str(tmp_path / "data")
+    _problem_with_initial_state(folder)
+    exe = tmp_path / "random_main"
+    src = open(os.path.join(ROOT, "spock.cu_amd", "examples", "random_main.cpp")).read()
+    _compile(src, exe, tmp_path)
+    run = subprocess.run([str(exe), folder + "/"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    print(run.stdout)
+    assert run.returncode == 0 and "Saved" in run.stdout
+    row = (tmp_path / "time.csv").read_text().strip().split(",")
+    assert len(row) == 3 and float(row[0]) > 0.0

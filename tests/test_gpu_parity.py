@@ -30,6 +30,8 @@ def x0_for(name, nx, k=0):
         return np.full(nx, 1.1)
     if name == "small":
         return configs.initial_states(1, nx, k + 1, "uniform")[k]
+    if name == "poly":
+        return configs.initial_states(33, nx, k + 1, "small")[k]
     return configs.initial_states(2, nx, k + 1)[k]
 
 
@@ -42,12 +44,15 @@ def inert_zero(o, z):
     return z
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid"])
-def test_operators_match_oracle(ctx, problem_dirs, name):
+@pytest.mark.parametrize("name,wgs", [("fixture", 1), ("small", 1), ("mid", 1), ("poly", 1),
+                                      ("mid", 7), ("poly", 3), ("mid", 128)])
+def test_operators_match_oracle(ctx, problem_dirs, name, wgs):
+    """wgs > 1: grid mode, the instance spread over a cooperative launch of wgs workgroups."""
     d = problem_dirs[name]
     o = Oracle(d)
     p = rt.Problem(ctx, d)
     b = rt.Batch(ctx, [p], rt.F64)
+    b.set_workgroups(wgs)
     rng = np.random.default_rng(7)
     z = inert_zero(o, 3.0 * rng.standard_normal(o.n_iter))
     x0 = x0_for(name, o.nx)
@@ -73,15 +78,18 @@ def test_operators_match_oracle(ctx, problem_dirs, name):
     assert abs(sc[1] - np.max(np.abs(y / o.gamma - ell))) <= 1e-12 * np.max(np.abs(y / o.gamma))
 
 
-@pytest.mark.parametrize("name,iters,method", [("fixture", 75, "spock"), ("small", 75, "spock"),
-                                               ("small", 250, "cp"), ("fixture", 250, "cp"), ("mid", 50, "spock")])
-def test_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method):
+@pytest.mark.parametrize("name,iters,method,wgs", [
+    ("fixture", 75, "spock", 1), ("small", 75, "spock", 1), ("small", 250, "cp", 1), ("fixture", 250, "cp", 1),
+    ("mid", 50, "spock", 1), ("poly", 75, "spock", 1), ("poly", 250, "cp", 1),
+    ("mid", 50, "spock", 64), ("poly", 75, "spock", 5), ("small", 250, "cp", 16)])
+def test_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method, wgs):
     d = problem_dirs[name]
     o = Oracle(d)
     x0 = x0_for(name, o.nx)
     ref = o.solve(x0, method=method, tol_abs=1e-30, max_iters=iters)
     p = rt.Problem(ctx, d)
     b = rt.Batch(ctx, [p], rt.F64)
+    b.set_workgroups(wgs)
     b.set_x0([x0])
     b.solve(rt.options(tol_abs=1e-30, max_iters=iters, method=rt.SPOCK if method == "spock" else rt.CP,
                        block_size=256))
@@ -106,7 +114,7 @@ def gpu_solve(ctx, problems, x0s, dtype=rt.F64, **opts):
     return b, res
 
 
-@pytest.mark.parametrize("name", ["fixture", "small"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
 def test_converged_solve_matches_oracle(ctx, problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)

@@ -31,6 +31,8 @@ def x0s_for(name, nx, count):
         return [np.full(nx, 1.1) * (1.0 - 0.01 * k) for k in range(count)]
     if name == "small":
         return list(configs.initial_states(1, nx, count, "uniform"))
+    if name == "poly":
+        return list(configs.initial_states(33, nx, count, "small"))
     return list(configs.initial_states(2, nx, count))
 
 
@@ -47,7 +49,7 @@ def shared(ctx, d, count, dtype=rt.F64):
     return p, rt.Batch(ctx, [p] * count, dtype, shared=True)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly"])
 def test_shared_operators_match_oracle(ctx, problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -92,7 +94,7 @@ def test_shared_operators_match_oracle(ctx, problem_dirs, name):
 
 
 @pytest.mark.parametrize("name,iters,method", [("fixture", 50, "spock"), ("small", 50, "spock"),
-                                               ("mid", 50, "spock"), ("small", 250, "cp")])
+                                               ("mid", 50, "spock"), ("small", 250, "cp"), ("poly", 50, "spock")])
 def test_shared_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method):
     d = problem_dirs[name]
     o = Oracle(d)

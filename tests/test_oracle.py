@@ -33,7 +33,7 @@ def inert_zero(o, z):
     return z
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly"])
 def test_adjoint(problem_dirs, name):
     o = Oracle(problem_dirs[name])
     rng = np.random.default_rng(0)
@@ -44,14 +44,14 @@ def test_adjoint(problem_dirs, name):
     assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))
 
 
-@pytest.mark.parametrize("name", ["fixture", "small"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
 def test_step_size_is_099_over_norm(problem_dirs, name):
     o = Oracle(problem_dirs[name])
     Ld = nr.dense_operator(o.L, o.n_prim)
     assert o.gamma * np.linalg.norm(Ld, 2) == pytest.approx(0.99, abs=2e-3)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
 def test_dynamics_projection_is_the_kkt_solution(problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -66,7 +66,7 @@ def test_dynamics_projection_is_the_kkt_solution(problem_dirs, name):
     assert rel_err(uo, uk.reshape(-1)) < 1e-9
 
 
-@pytest.mark.parametrize("name", ["fixture", "small"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
 def test_kernel_projection_is_orthogonal_onto_the_risk_kernel(problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -149,7 +149,7 @@ def test_least_squares_against_numpy():
         assert rel_err(lstsq_gram(Y, r), ref) < 1e-9
 
 
-@pytest.mark.parametrize("name", ["fixture", "small"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
 def test_dotM_definition(problem_dirs, name):
     o = Oracle(problem_dirs[name])
     rng = np.random.default_rng(5)
@@ -160,7 +160,7 @@ def test_dotM_definition(problem_dirs, name):
     assert o.dotM(x, y) == pytest.approx(ref, rel=1e-12, abs=1e-12)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
 def test_T_lands_on_the_dynamics(problem_dirs, name):
     """The primal part of T(z) is a projection onto the dynamics with x_0 = x0."""
     d = problem_dirs[name]
