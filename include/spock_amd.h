@@ -116,12 +116,11 @@ int spk_batch_is_shared(const spk_batch *b);
 int spk_batch_destroy(spk_batch *b);
 int spk_batch_count(const spk_batch *b);
 /* workgroups that solve one instance together (per-instance batches of ONE instance; a batch of
- * several instances always runs one workgroup per instance).  0 (default): one workgroup, or, for
- * an iterate of 200,000 elements or more, one per compute unit; 1: one workgroup; n > 1: n
- * workgroups in a cooperative launch with grid barriers between the stages of the tree sweeps and
- * grid-wide reductions, so one large tree uses the whole GPU.  The count must fit co-resident on
- * the device (the solve fails with a message otherwise).  The reference runs one instance on
- * the whole GPU through its many small launches (src/cache.cuh:1008-1132). */
+ * several instances always runs one workgroup per instance).  0 or 1 (default): one workgroup;
+ * n > 1: n workgroups in a cooperative launch with grid barriers between the stages of the tree
+ * sweeps and grid-wide reductions, so one large tree is spread over the GPU.  The count must fit
+ * co-resident on the device (the solve fails with a message otherwise).  The reference runs one
+ * instance on the whole GPU through its many small launches (src/cache.cuh:1008-1132). */
 int spk_batch_set_workgroups(spk_batch *b, int workgroups);
 /* zero every iterate and workspace (Cache::reset, src/cache.cuh:436-479); the initial states set by
  * spk_batch_set_x0 are inputs and stay, where the reference re-uploads them on every runSpock */

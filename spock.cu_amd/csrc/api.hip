@@ -70,7 +70,7 @@ struct spk_batch {
     size_t coreElems = 0, andElems = 0, x0Elems = 0;
     int mAlloc = 0;
     DevBuf core, anders, x0buf, dInst, dOut;  // x0buf: scaled initial states (inputs, kept by reset)
-    int wgs = 0;    // workgroups per instance (0: automatic, see gridWorkgroups)
+    int wgs = 0;    // workgroups per instance (0 or 1: one; see gridWorkgroups)
     DevBuf gpart;   // grid-mode reduction partials (single-instance batches)
     std::vector<char> hInst;  // array of Inst<T>
     std::vector<double> tolFixed;  // per-instance given tolerance (0 = calibrate at the first check)
@@ -305,16 +305,11 @@ int ldsPoolInts(const spk_batch *b) {
     return mx <= budget ? mx : 0;
 }
 
-// Iterate size from which a single instance is solved by all compute units together by default.
-constexpr size_t GRID_AUTO_ELEMS = 200000;
-
-// Workgroups that cooperate on one instance: 1 for a batch of several instances; for a single
-// instance the count set by spk_batch_set_workgroups, or by default one per compute unit when the
-// iterate has at least GRID_AUTO_ELEMS elements.  Bounded by what can be co-resident, which a
-// cooperative launch requires.
+// Workgroups that cooperate on one instance: 1 for a batch of several instances and by default;
+// for a single instance the count set by spk_batch_set_workgroups.  Bounded by what can be
+// co-resident, which a cooperative launch requires.
 int gridWorkgroups(spk_batch *b, const void *kernel, int bs, size_t shm) {
-    if (b->count != 1 || b->wgs == 1) return 1;
-    if (b->wgs == 0 && b->probs[0]->hp.sIter < GRID_AUTO_ELEMS) return 1;
+    if (b->count != 1 || b->wgs <= 1) return 1;
     const int dev = b->ctx->device;
     int coop = 0, cus = 0, perCU = 0;
     check(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev), "device attribute");
@@ -329,7 +324,7 @@ int gridWorkgroups(spk_batch *b, const void *kernel, int bs, size_t shm) {
     if (b->wgs > most)
         throw std::runtime_error("at most " + std::to_string(most) + " workgroups of " + std::to_string(bs) +
                                  " threads can cooperate on one instance on this device");
-    return b->wgs > 1 ? b->wgs : std::min(cus, most);
+    return b->wgs;
 }
 
 template <typename K, typename... A>

@@ -225,8 +225,8 @@ class Batch:
         self.ctx.check(self.ctx.lib.spk_batch_reset(self.h), "reset")
 
     def set_workgroups(self, n):
-        """Workgroups that solve the single instance together (0 automatic, 1 one workgroup,
-        n > 1 a cooperative launch of n workgroups)."""
+        """Workgroups that solve the single instance together (0 or 1: one workgroup,
+        n > 1: a cooperative launch of n workgroups)."""
         self.ctx.check(self.ctx.lib.spk_batch_set_workgroups(self.h, int(n)), "set_workgroups")
 
     def set_x0(self, x0):
