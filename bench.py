@@ -174,10 +174,12 @@ def reference_costs(rt, ctx, prob, x0s, tol, max_iters):
     b.set_x0(list(x0s))
     b.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=max_iters, block_size=512))
     res, _ = b.results()
+    log("  reference: plain FP64 solves done")
     b32 = rt.Batch(ctx, [prob] * n, rt.F32)
     b32.set_x0(list(x0s))
     b32.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=max_iters, block_size=512))
     b32.results()
+    log("  reference: cold FP32 solves done")
     b64 = rt.Batch(ctx, [prob] * n, rt.F64)
     b64.set_x0(list(x0s))
     b64.copy_iterates_from(b32)
@@ -209,14 +211,19 @@ def cost_check(rt, ctx, prob, x0s, args, accepted, s_batch):
 
 
 def train_warm_start(rt, ctx, prob, configs, args):
-    """Solve a training family (FP32, one workgroup per instance) and fit the MLP on its solutions."""
+    """Solve a training family of initial states cold (FP32 lane engine, chunked) and fit the MLP
+    on the solutions."""
     x_train = configs.initial_states(50, prob.nx, args.train, "normal")
-    b = rt.Batch(ctx, [prob] * args.train, rt.F32)
+    b = rt.Batch(ctx, [prob] * args.train, rt.F32, shared=True)
     b.set_x0(list(x_train))
     t = time.perf_counter()
-    b.solve(rt.options(tol_abs=args.tol, tol_rel=args.tol, max_iters=args.train_iters, block_size=512,
-                       ls_method=rt.LS_GRAM))
-    res, _ = b.results()
+
+    def report(res, sec):
+        done = sum(r["status"] != -1 for r in res)
+        log(f"  training solves: {done}/{len(res)} finished, {time.perf_counter() - t:.1f} s")
+
+    res, _ = b.solve_chunked(rt.options(tol_abs=args.tol, tol_rel=args.tol, max_iters=args.train_iters,
+                                        block_size=1024, chunk_iters=args.chunk), False, report)
     ok = np.array([r["status"] == 0 for r in res])
     log(f"training solves: {ok.sum()}/{args.train} converged in {time.perf_counter() - t:.1f} s")
     mlp = rt.WarmStartMLP(ctx, prob.nx, prob.n_iter, hidden=512, layers=3, seed=11)
@@ -265,9 +272,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16384, help="held-out initial states solved per step")
-    ap.add_argument("--train", type=int, default=512, help="solutions the warm-start MLP is trained on")
-    ap.add_argument("--epochs", type=int, default=100)
-    ap.add_argument("--train-iters", type=int, default=4000, help="iteration cap of the training solves")
+    ap.add_argument("--train", type=int, default=16384, help="solutions the warm-start MLP is trained on")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--train-iters", type=int, default=3000, help="iteration cap of the training solves")
     ap.add_argument("--chunk", type=int, default=400, help="outer iterations per launch of a chunked solve")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold starts)")
     ap.add_argument("--plain", type=int, default=1, help="instances solved by the plain single-instance path")
