@@ -92,9 +92,16 @@ class Pipeline:
         self.b32.set_x0(list(x0s))
         self.b64.set_x0(list(x0s))
         self.kernel_s = []
+        # the lane kernels' schedule (matrix-vector staging, workgroup size), timed on these batches
+        self.schedule = {}
+        for name, b in (("f32", self.b32), ("f64", self.b64)):
+            mv, block, secs = b.autotune(self.opts(), iters=args.tune_iters)
+            self.schedule[name] = {"mv": mv, "block": block,
+                                   "seconds": {f"mv{k[0]}_b{k[1]}": v for k, v in secs.items()}}
+            log(f"autotuned {name}: mv {mv}, block {block}; {self.schedule[name]['seconds']}")
 
     def opts(self, **kw):
-        o = dict(tol_abs=self.a.tol, tol_rel=self.a.tol, block_size=1024)
+        o = dict(tol_abs=self.a.tol, tol_rel=self.a.tol, block_size=0)
         o.update(kw)
         return self.rt.options(**o)
 
@@ -286,6 +293,7 @@ def main():
     ap.add_argument("--kappa", type=float, default=1.0, help="warm-path threshold as a fraction of the cold one")
     ap.add_argument("--fallback", action="store_true", help="solve what the warm path leaves with cold solves")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tune-iters", type=int, default=50, help="outer iterations per schedule the autotuner times")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
@@ -378,6 +386,7 @@ def main():
         "speedup_vs_plain": (value / plain["solves_per_s"]) if plain else None,
         "accepted": int(accepted.sum()), "not_solved": int(B - accepted.sum()), "fallback": n_fb,
         "kappa": args.kappa if mlp is not None else 1.0,
+        "schedule": pipe.schedule,
         "fp32_iters_mean": float(it32.mean()), "fp32_iters_max": int(it32.max()),
         "polish_iters_mean": float(it64.mean()),
         "kernel_seconds": [{"solve": k[5], "dtype": "f32" if k[0] == rt.F32 else "f64", "seconds": k[1],

@@ -154,6 +154,16 @@ int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm);
  * continued iterates, branch decisions and counters are those of one uninterrupted solve.  Results
  * after a chunk report SPK_NOT_RUN (and the iterations so far) for instances still running. */
 int spk_batch_resume(spk_batch *b);
+/* shared batches: the lane kernels' schedule.  mv = 0 reads factor entries with scalar loads,
+ * mv = 8 stages them through LDS in 8-column chunks (vector loads, one chunk prefetched); both give
+ * bitwise identical iterates.  block = workgroup size (0: 1024).  No counterpart in the reference. */
+int spk_batch_set_schedule(spk_batch *b, int mv, int block);
+/* shared batches: time `iters` cold outer iterations (from the batch's initial states) under each
+ * schedule {mv 0, 8} x {block 1024, 512}, keep the fastest for the batch's later solves and return it
+ * (mv, block: nullable; seconds: nullable, 4 kernel times in that order).  The iterates are reset
+ * (initial states and given tolerances are kept).  opts supplies the solver options of the timed
+ * runs (nullable: defaults). */
+int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv, int *block, double *seconds);
 /* wait for the last solve; fills count results and the kernel time in seconds (nullable) */
 int spk_batch_results(spk_batch *b, spk_result *out, double *kernel_seconds);
 /* single operator on every instance: in/out are host arrays concatenated over instances

@@ -29,6 +29,10 @@ template <typename T>
 __global__ void k_lane_solve(const LaneInst<T> *, Out *);
 template <typename T>
 __global__ void k_lane_debug(const LaneInst<T> *, Out *, int);
+template <typename T>
+__global__ void k_lane_solve_x8(const LaneInst<T> *, Out *);
+template <typename T>
+__global__ void k_lane_debug_x8(const LaneInst<T> *, Out *, int);
 template <typename S, typename D>
 __global__ void k_interleave(const S *, long, D *, long, int, long);
 template <typename S, typename D>
@@ -84,6 +88,9 @@ struct spk_batch {
     bool resumable = false;
     size_t lRegion[16] = {};  // element offsets: work[0..8], zio, rio, hS, hY, qd, dd, x0
     static constexpr int LANE_MAXM = 4;
+    int laneMv = 0;      // matrix-vector schedule of the lane kernels: 0 scalar loads, 8 LDS-staged
+    int laneBlockSz = 0; // workgroup size chosen by spk_batch_autotune (0: none)
+    bool tuned = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool pending = false;
     size_t elSize() const { return dtype == SPK_F64 ? 8 : 4; }
@@ -449,9 +456,18 @@ void buildLaneInst(spk_batch *b, const spk_options &o, int warm, int resume = 0)
           "upload lane instance");
 }
 
-int laneBlock(const spk_options &o) {
-    const int bs = o.block_size > 0 ? o.block_size : 1024;
+int laneBlock(const spk_batch *b, const spk_options &o) {
+    const int bs = o.block_size > 0 ? o.block_size : (b->laneBlockSz > 0 ? b->laneBlockSz : 1024);
     return std::max(64, std::min(1024, bs / 64 * 64));
+}
+
+template <typename T>
+void launchLaneKernel(spk_batch *b, const spk_options &o) {
+    const dim3 grid((unsigned) (b->Bpad / 64)), block(laneBlock(b, o));
+    auto *P = static_cast<const spk::LaneInst<T> *>(b->dInst.p);
+    auto *outs = static_cast<spk::Out *>(b->dOut.p);
+    if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_solve_x8<T>, grid, block, 0, b->ctx->stream, P, outs);
+    else hipLaunchKernelGGL(spk::k_lane_solve<T>, grid, block, 0, b->ctx->stream, P, outs);
 }
 
 template <typename T>
@@ -462,8 +478,7 @@ void launchLaneSolve(spk_batch *b, const spk_options &o, int warm, int resume = 
     b->resumable = o.chunk_iters > 0;
     b->dOut.alloc(sizeof(spk::Out) * b->count);
     check(hipEventRecord(b->ev0, b->ctx->stream), "event");
-    hipLaunchKernelGGL(spk::k_lane_solve<T>, dim3((unsigned) (b->Bpad / 64)), dim3(laneBlock(o)), 0, b->ctx->stream,
-                       static_cast<const spk::LaneInst<T> *>(b->dInst.p), static_cast<spk::Out *>(b->dOut.p));
+    launchLaneKernel<T>(b, o);
     check(hipGetLastError(), "launch k_lane_solve");
     check(hipEventRecord(b->ev1, b->ctx->stream), "event");
     b->pending = true;
@@ -1074,6 +1089,67 @@ int spk_batch_resume(spk_batch *b) {
     }
 }
 
+int spk_batch_set_schedule(spk_batch *b, int mv, int block) {
+    try {
+        if (!b->lanes) throw std::runtime_error("schedules apply to shared batches");
+        if (mv != 0 && mv != 8) throw std::runtime_error("matrix-vector schedule must be 0 or 8");
+        if (block != 0 && (block < 64 || block > 1024 || block % 64))
+            throw std::runtime_error("block must be 0 or a multiple of 64 in 64..1024");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        b->laneMv = mv;
+        b->laneBlockSz = block;
+        b->resumable = false;  // a chunked solve continues with the schedule it started with
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv, int *block, double *seconds) {
+    try {
+        if (!b->lanes) throw std::runtime_error("the autotuner tunes shared batches");
+        if (iters < 1) throw std::runtime_error("iters must be >= 1");
+        spk_options o;
+        if (opts) o = *opts;
+        else spk_default_options(&o);
+        o.tol_abs = o.tol_rel = 0.;  // a fixed number of iterations
+        o.max_iters = iters;
+        o.chunk_iters = 0;
+        o.block_size = 0;
+        o.max_time_secs = 0.;
+        const std::vector<double> keepTol = b->tolFixed;
+        b->tolFixed.clear();
+        const int cand[4][2] = {{0, 1024}, {8, 1024}, {0, 512}, {8, 512}};
+        double best = 1e300;
+        int bi = 0;
+        for (int i = 0; i < 4; i++) {
+            b->laneMv = cand[i][0];
+            b->laneBlockSz = cand[i][1];
+            if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
+            if (b->dtype == SPK_F64) launchLaneSolve<double>(b, o, 0);
+            else launchLaneSolve<float>(b, o, 0);
+            double sec = 0.;
+            if (spk_batch_results(b, nullptr, &sec)) throw std::runtime_error(b->ctx->err);
+            if (seconds) seconds[i] = sec;
+            if (sec < best) {
+                best = sec;
+                bi = i;
+            }
+        }
+        b->laneMv = cand[bi][0];
+        b->laneBlockSz = cand[bi][1];
+        b->tuned = true;
+        b->resumable = false;
+        b->tolFixed = keepTol;
+        if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
+        if (mv) *mv = b->laneMv;
+        if (block) *block = b->laneBlockSz;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
 int spk_batch_results(spk_batch *b, spk_result *out, double *kernelSeconds) {
     try {
         check(hipStreamSynchronize(b->ctx->stream), "sync after solve");
@@ -1128,17 +1204,18 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
             spk_default_options(&o);
             b->dOut.alloc(sizeof(spk::Out) * b->count);
             check(hipMemset(b->dOut.p, 0, sizeof(spk::Out) * b->count), "zero out");
-            const dim3 grid((unsigned) (b->Bpad / 64)), block(laneBlock(o));
+            const dim3 grid((unsigned) (b->Bpad / 64)), block(laneBlock(b, o));
+            auto *outs = static_cast<spk::Out *>(b->dOut.p);
             if (f64) {
                 buildLaneInst<double>(b, o, 0);
-                hipLaunchKernelGGL(spk::k_lane_debug<double>, grid, block, 0, b->ctx->stream,
-                                   static_cast<const spk::LaneInst<double> *>(b->dInst.p),
-                                   static_cast<spk::Out *>(b->dOut.p), op);
+                auto *P = static_cast<const spk::LaneInst<double> *>(b->dInst.p);
+                if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_debug_x8<double>, grid, block, 0, b->ctx->stream, P, outs, op);
+                else hipLaunchKernelGGL(spk::k_lane_debug<double>, grid, block, 0, b->ctx->stream, P, outs, op);
             } else {
                 buildLaneInst<float>(b, o, 0);
-                hipLaunchKernelGGL(spk::k_lane_debug<float>, grid, block, 0, b->ctx->stream,
-                                   static_cast<const spk::LaneInst<float> *>(b->dInst.p),
-                                   static_cast<spk::Out *>(b->dOut.p), op);
+                auto *P = static_cast<const spk::LaneInst<float> *>(b->dInst.p);
+                if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_debug_x8<float>, grid, block, 0, b->ctx->stream, P, outs, op);
+                else hipLaunchKernelGGL(spk::k_lane_debug<float>, grid, block, 0, b->ctx->stream, P, outs, op);
             }
             check(hipGetLastError(), "launch k_lane_debug");
             check(hipStreamSynchronize(b->ctx->stream), "sync debug op");

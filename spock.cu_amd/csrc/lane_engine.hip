@@ -23,6 +23,21 @@
 #define SPK_LANE_MAX_THREADS 1024
 #endif
 
+// Matrix-vector schedule of this build: 0 reads the factor entries with scalar loads; 4, 8 or 16
+// stage them through LDS in chunks of that many columns.  The library is built once per schedule;
+// the kernels of a build carry its suffix (k_lane_solve, k_lane_solve_x8, ...), and the autotuner
+// of the host side picks one per batch.
+#ifndef SPK_MV_XB
+#define SPK_MV_XB 0
+#endif
+#if SPK_MV_XB == 0
+#define SPK_LANE_KERNEL(name) name
+#else
+#define SPK_LANE_KERNEL_CAT2(a, b) a##_x##b
+#define SPK_LANE_KERNEL_CAT(a, b) SPK_LANE_KERNEL_CAT2(a, b)
+#define SPK_LANE_KERNEL(name) SPK_LANE_KERNEL_CAT(name, SPK_MV_XB)
+#endif
+
 namespace spk {
 namespace lane {
 
@@ -101,6 +116,7 @@ __device__ __forceinline__ void zeroY(T (&y)[NY]) {
     for (int i = 0; i < NY; i++) y[i] = T(0);
 }
 
+#if SPK_MV_XB == 0
 // y[r] += sum_c M[r + c * ld] * x(c) for r < RR: M is wave-uniform, x(c) the lane's own value.
 // XB vector loads are issued before their FMAs; CB columns of the matrix at a time stay in
 // scalar registers (CB * RR <= 64).  Columns past the end are clamped to the last one and enter
@@ -131,6 +147,63 @@ __device__ __forceinline__ void mvFixed(T (&y)[NY], CP<T> M, int ld, int cols, X
         }
     }
 }
+
+#else
+// Per-wave LDS staging area of the matrix-vector products: XB columns x 16 rows of one matrix.
+constexpr int MV_XB = SPK_MV_XB;
+__shared__ __align__(16) double laneStage[(SPK_LANE_MAX_THREADS / 64) * MV_XB * 16];
+
+// y[r] += sum_c M[r + c * ld] * x(c) for r < RR: M is wave-uniform, x(c) the lane's own value.
+// The matrix streams through the wave's LDS slice in chunks of XB columns: each lane fetches 4
+// of the chunk's XB x 16 entries with vector loads (memory order, so the next chunk's loads are
+// in flight while the current chunk is consumed), the wave writes them to LDS, and every FMA
+// reads its matrix entry as an LDS broadcast.  (Scalar loads of the entries complete out of
+// order, so each use would wait for every outstanding one.)  Columns past the end are clamped to
+// the last one and enter with x = 0, and rows past RR are not read, so every load stays inside
+// the table (which carries 64 elements of padding for the RR rows that may run past a matrix's
+// last column).  The accumulation order over columns is that of a plain column loop.
+template <int RR, typename T, int NY, typename XF>
+__device__ __forceinline__ void mvFixed(T (&y)[NY], CP<T> M, int ld, int cols, XF x) {
+    static_assert(RR <= NY && RR % 4 == 0, "row chunk");
+    constexpr int XB = MV_XB;
+    constexpr int PER = XB * 16 / 64;  // entries each lane stages per chunk
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    T *st = reinterpret_cast<T *>(laneStage) + w * (XB * 16);
+    const int q = lane / (16 / PER), r4 = (lane % (16 / PER)) * PER;  // rows r4.. of column q
+    const SPK_GLOBAL T *Mg = (const SPK_GLOBAL T *) (unsigned long long) M;
+    T nxt[PER];
+    auto fetch = [&](int c0) {
+        const SPK_GLOBAL T *col = Mg + (size_t) min(c0 + q, cols - 1) * ld;
+#pragma unroll
+        for (int k = 0; k < PER; k++) nxt[k] = (r4 + k < RR) ? col[r4 + k] : T(0);
+    };
+    fetch(0);
+    for (int c0 = 0; c0 < cols; c0 += XB) {
+        T xv[XB];
+#pragma unroll
+        for (int qq = 0; qq < XB; qq++) {
+            const int cc = min(c0 + qq, cols - 1);
+            const T v = x(cc);
+            xv[qq] = (c0 + qq < cols) ? v : T(0);
+        }
+        // the previous chunk's LDS reads were issued before these writes (LDS is in order per wave)
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < PER; k++) st[q * 16 + r4 + k] = nxt[k];
+        __builtin_amdgcn_wave_barrier();
+        if (c0 + XB < cols) fetch(c0 + XB);
+#pragma unroll
+        for (int qq = 0; qq < XB; qq++) {
+            if (c0 + qq >= cols) break;
+#pragma unroll
+            for (int r = 0; r < RR; r++) y[r] += st[qq * 16 + r] * xv[qq];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+#endif
 
 // rows <= NY; the row count picks a fixed-size variant so every scalar load is unconditional.
 template <typename T, int NY, typename XF>
@@ -1236,7 +1309,7 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
 }
 
 template <typename T>
-__global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) k_lane_solve(const LaneInst<T> *__restrict__ P, Out *outs) {
+__global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) SPK_LANE_KERNEL(k_lane_solve)(const LaneInst<T> *__restrict__ P, Out *outs) {
     __shared__ double lds[16 * 64];
     __shared__ long phaseAcc[NUM_PHASES];
     laneSolve(P, outs, lds, phaseAcc);
@@ -1244,8 +1317,8 @@ __global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) k_lane_solve(const LaneI
 
 // Single operators on every instance (parity tests): z = zio, cand = work[1].
 template <typename T>
-__global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) k_lane_debug(const LaneInst<T> *__restrict__ Pp, Out *outs,
-                                                                     int op) {
+__global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) SPK_LANE_KERNEL(k_lane_debug)(const LaneInst<T> *__restrict__ Pp,
+                                                                                      Out *outs, int op) {
     using namespace lane;
     __shared__ double lds[16 * 64];
     LP<T> P = *cst(Pp);
@@ -1283,6 +1356,12 @@ __global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) k_lane_debug(const LaneI
     }
 }
 
+template __global__ void SPK_LANE_KERNEL(k_lane_solve)<float>(const LaneInst<float> *, Out *);
+template __global__ void SPK_LANE_KERNEL(k_lane_solve)<double>(const LaneInst<double> *, Out *);
+template __global__ void SPK_LANE_KERNEL(k_lane_debug)<float>(const LaneInst<float> *, Out *, int);
+template __global__ void SPK_LANE_KERNEL(k_lane_debug)<double>(const LaneInst<double> *, Out *, int);
+
+#if SPK_MV_XB == 0
 // [instance][element] rows (leading dimension ld) to the block-interleaved layout of a region of
 // n rows and back, with dtype conversion.
 template <typename S, typename D>
@@ -1303,10 +1382,6 @@ __global__ void k_deinterleave(const S *__restrict__ in, long, D *__restrict__ o
     }
 }
 
-template __global__ void k_lane_solve<float>(const LaneInst<float> *, Out *);
-template __global__ void k_lane_solve<double>(const LaneInst<double> *, Out *);
-template __global__ void k_lane_debug<float>(const LaneInst<float> *, Out *, int);
-template __global__ void k_lane_debug<double>(const LaneInst<double> *, Out *, int);
 template __global__ void k_interleave<float, float>(const float *, long, float *, long, int, long);
 template __global__ void k_interleave<float, double>(const float *, long, double *, long, int, long);
 template __global__ void k_interleave<double, float>(const double *, long, float *, long, int, long);
@@ -1315,5 +1390,6 @@ template __global__ void k_deinterleave<float, float>(const float *, long, float
 template __global__ void k_deinterleave<float, double>(const float *, long, double *, long, long, int);
 template __global__ void k_deinterleave<double, float>(const double *, long, float *, long, long, int);
 template __global__ void k_deinterleave<double, double>(const double *, long, double *, long, long, int);
+#endif
 
 }  // namespace spk

@@ -20,7 +20,7 @@ STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
 EXPORTS = [
     "spk_default_options", "spk_ctx_create", "spk_ctx_destroy", "spk_last_error", "spk_ctx_stream",
     "spk_ctx_synchronize", "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
-    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_reset",
+    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_set_schedule", "spk_batch_autotune", "spk_batch_reset",
     "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
     "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve", "spk_batch_resume",
     "spk_batch_results", "spk_batch_debug_op", "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
@@ -89,6 +89,8 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_destroy.argtypes = [vp]
     lib.spk_batch_count.argtypes = [vp]
     lib.spk_batch_set_workgroups.argtypes = [vp, C.c_int]
+    lib.spk_batch_set_schedule.argtypes = [vp, C.c_int, C.c_int]
+    lib.spk_batch_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
     lib.spk_batch_reset.argtypes = [vp]
     lib.spk_batch_set_x0.argtypes = [vp, _dp]
     lib.spk_batch_set_tolerances.argtypes = [vp, _dp]
@@ -223,6 +225,21 @@ class Batch:
 
     def reset(self):
         self.ctx.check(self.ctx.lib.spk_batch_reset(self.h), "reset")
+
+    def set_schedule(self, mv, block=0):
+        """Lane-kernel schedule of a shared batch: mv 0 (scalar factor loads) or 8 (LDS-staged)."""
+        self.ctx.check(self.ctx.lib.spk_batch_set_schedule(self.h, int(mv), int(block)), "set_schedule")
+
+    def autotune(self, opts=None, iters=50):
+        """Time the lane-kernel schedules on this batch and keep the fastest; returns
+        (mv, block, {(mv, block): seconds})."""
+        mv, blk = C.c_int(), C.c_int()
+        sec = np.zeros(4)
+        o = opts if opts is not None else options()
+        self.ctx.check(self.ctx.lib.spk_batch_autotune(self.h, C.byref(o), int(iters), C.byref(mv), C.byref(blk),
+                                                        _d(sec)), "autotune")
+        cands = [(0, 1024), (8, 1024), (0, 512), (8, 512)]
+        return mv.value, blk.value, dict(zip(cands, sec.tolist()))
 
     def set_workgroups(self, n):
         """Workgroups that solve the single instance together (0 or 1: one workgroup,

@@ -231,3 +231,45 @@ def test_chunked_solve_equals_uninterrupted(ctx, problem_dirs, dtype):
         for f in ("status", "iters", "k0", "k1", "k2", "k2bt", "k3", "num_T"):
             assert ra[k][f] == rb[k][f], (k, f)
     assert np.array_equal(a.iterates(), b.iterates())
+
+
+@pytest.mark.parametrize("name,dtype", [("small", rt.F64), ("mid", rt.F64), ("mid", rt.F32), ("poly", rt.F64)])
+def test_lds_schedule_is_bitwise_identical(ctx, problem_dirs, name, dtype):
+    """The LDS-staged matrix-vector schedule accumulates in the same order as the scalar one."""
+    d = problem_dirs[name]
+    o = Oracle(d)
+    count = 70
+    x0s = x0s_for(name, o.nx, count)
+    out = []
+    for mv in (0, 8):
+        p, b = shared(ctx, d, count, dtype)
+        b.set_schedule(mv)
+        b.set_x0(x0s)
+        b.solve(rt.options(tol_abs=1e-3, tol_rel=1e-3, max_iters=100))
+        res, _ = b.results()
+        out.append((b.iterates(), [(r["status"], r["iters"], r["k0"], r["k1"], r["k2"], r["k3"]) for r in res]))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+def test_autotune_keeps_the_solution(ctx, problem_dirs):
+    """After the autotuner's timed runs the batch solves exactly as a fresh batch given the chosen
+    schedule (the workgroup size sets the order of the per-lane sums, so it is part of the result)."""
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    count = 100
+    x0s = x0s_for("small", o.nx, count)
+    _, b = shared(ctx, d, count)
+    b.set_x0(x0s)
+    mv, block, secs = b.autotune(iters=20)
+    assert (mv, block) in secs and all(s > 0 for s in secs.values())
+    assert secs[(mv, block)] == min(secs.values())
+    b.solve(rt.options(tol_abs=1e-3, tol_rel=1e-3, max_iters=500, block_size=0))
+    res, _ = b.results()
+    _, ref = shared(ctx, d, count)
+    ref.set_schedule(mv, block)
+    ref.set_x0(x0s)
+    ref.solve(rt.options(tol_abs=1e-3, tol_rel=1e-3, max_iters=500, block_size=0))
+    res_ref, _ = ref.results()
+    assert [r["iters"] for r in res] == [r["iters"] for r in res_ref]
+    assert np.array_equal(b.iterates(), ref.iterates())
