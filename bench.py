@@ -136,7 +136,7 @@ class Pipeline:
         r32 = self.solve(self.b32, self.mlp is not None, "FP32 from the warm start", max_iters=max32)
         self.b32.set_tolerances(None)
         self.b64.copy_iterates_from(self.b32)
-        self.b64.set_tolerances(tols * kappa)
+        self.b64.set_tolerances(tols * (self.a.polish_kappa if self.mlp is not None else 1.0))
         r64 = self.solve(self.b64, True, "FP64 polish", max_iters=max64)
         self.b64.set_tolerances(None)
         status = np.array([r["status"] for r in r64])
@@ -291,6 +291,9 @@ def main():
     ap.add_argument("--polish-iters", type=int, default=400)
     ap.add_argument("--ref-sample", type=int, default=16, help="instances checked against plain FP64 cost")
     ap.add_argument("--kappa", type=float, default=1.0, help="warm-path threshold as a fraction of the cold one")
+    ap.add_argument("--polish-kappa", type=float, default=1.0,
+                    help="FP64 polish threshold as a fraction of the cold one (below 1: more accurate costs, "
+                         "more polish iterations)")
     ap.add_argument("--fallback", action="store_true", help="solve what the warm path leaves with cold solves")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tune-iters", type=int, default=50, help="outer iterations per schedule the autotuner times")
@@ -386,6 +389,7 @@ def main():
         "speedup_vs_plain": (value / plain["solves_per_s"]) if plain else None,
         "accepted": int(accepted.sum()), "not_solved": int(B - accepted.sum()), "fallback": n_fb,
         "kappa": args.kappa if mlp is not None else 1.0,
+        "polish_kappa": args.polish_kappa if mlp is not None else 1.0,
         "schedule": pipe.schedule,
         "fp32_iters_mean": float(it32.mean()), "fp32_iters_max": int(it32.max()),
         "polish_iters_mean": float(it64.mean()),
