@@ -365,6 +365,17 @@ def main():
         main32 = [k for k in kernels if k[0] == rt.F32][1]  # [0] is the threshold calibration
         alg = main32[2] * bt + main32[3] * bl
         achieved = alg / main32[1] / 1e9
+        # HBM bytes per launch: FETCH_SIZE + WRITE_SIZE per instance-T measured by rocprofv3 on this
+        # kernel (scaled by a plain copy's counters), times this solve's instance-T per launch
+        traffic, traffic_note = None, None
+        tf = os.path.join(ROOT, "profiles", "k_lane_solve_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as fh:
+                meas = json.load(fh)
+            traffic = meas["hbm_bytes_per_instance_T_and_ell"] * main32[2] / max(1, main32[4])
+            traffic_note = ("bytes per launch: %.2f MB of HBM traffic per instance-T (rocprofv3 FETCH_SIZE + "
+                            "WRITE_SIZE, copy-scaled, %s) x this solve's instance-T per launch"
+                            % (meas["hbm_bytes_per_instance_T_and_ell"] / 1e6, meas["workload"]))
 
         cpu = None
         if not args.no_cpu:
@@ -397,7 +408,8 @@ def main():
                             "launches": k[4]} for k in kernels],
         "cost_check": cost,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_lane_solve<float>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
+                     "kernel": "k_lane_solve<float>",
                      "bytes_per_T": bt, "bytes_per_ell": bl, "T": main32[2], "ell": main32[3],
                      "launches": main32[4], "seconds": main32[1], "avg_launch_ms": 1e3 * main32[1] / max(1, main32[4]),
                      "unit_note": "per instance: T bytes = factors/64 + 2|z| + 2(nx N + nu nl) words, "
