@@ -280,15 +280,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16384, help="held-out initial states solved per step")
     ap.add_argument("--train", type=int, default=16384, help="solutions the warm-start MLP is trained on")
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--train-iters", type=int, default=3000, help="iteration cap of the training solves")
     ap.add_argument("--chunk", type=int, default=400, help="outer iterations per launch of a chunked solve")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold starts)")
     ap.add_argument("--plain", type=int, default=1, help="instances solved by the plain single-instance path")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--max-iters", type=int, default=8000, help="iteration cap of plain cold solves")
-    ap.add_argument("--warm-iters", type=int, default=2000, help="FP32 iteration cap from the warm start")
-    ap.add_argument("--polish-iters", type=int, default=400)
+    ap.add_argument("--warm-iters", type=int, default=800, help="FP32 iteration cap from the warm start")
+    ap.add_argument("--polish-iters", type=int, default=200)
     ap.add_argument("--ref-sample", type=int, default=16, help="instances checked against plain FP64 cost")
     ap.add_argument("--kappa", type=float, default=1.0, help="warm-path threshold as a fraction of the cold one")
     ap.add_argument("--polish-kappa", type=float, default=1.0,
