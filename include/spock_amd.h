@@ -46,7 +46,8 @@ enum spk_status { SPK_NOT_RUN = -1, SPK_CONVERGED = 0, SPK_OUT_OF_ITER = 1, SPK_
 /* Anderson least squares: Householder QR as the reference (geqrf/ormqr/trsm), or the Gram
  * matrix kept incrementally and solved by Cholesky in FP64 (one fused pass per iteration). */
 enum spk_ls_method { SPK_LS_HOUSEHOLDER = 0, SPK_LS_GRAM = 1 };
-enum spk_debug_op { SPK_OP_L = 0, SPK_OP_LADJ = 1, SPK_OP_DYN = 2, SPK_OP_KERNEL = 3, SPK_OP_T = 4, SPK_OP_ELL = 5 };
+enum spk_debug_op { SPK_OP_L = 0, SPK_OP_LADJ = 1, SPK_OP_DYN = 2, SPK_OP_KERNEL = 3, SPK_OP_T = 4, SPK_OP_ELL = 5,
+                    SPK_OP_DUAL = 6 };
 
 /* Builder options (src/cache.cuh:1154-1210).  tol_abs/tol_rel are multiplied by
  * min(scaling) for preconditioned problems (src/cache.cuh:302-306).  max_iters = 0 means
@@ -170,7 +171,9 @@ int spk_batch_results(spk_batch *b, spk_result *out, double *kernel_seconds);
  * (size_iterate_k values each).  L reads the primal part of `in` and writes the dual part of
  * `out`; LADJ the reverse; DYN/KERNEL project the primal part; T writes T(in); ELL writes
  * scalars[2k] = <in, M out_in> and scalars[2k+1] = max|out_in/gamma - ell(out_in)| where out_in
- * is taken from `out` on entry. */
+ * is taken from `out` on entry.  DUAL (per-instance batches): the primal part of `out` on entry is a
+ * primal point p; writes the dual part of `out` as the CP dual step from the iterate `in` with
+ * p in place of the primal prox, i.e. Cache::getDual(p) (src/cache.cuh:371-376, 770-789). */
 int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, double *scalars);
 
 /* ------------------------------------------------------------------ learned warm start

@@ -7,8 +7,9 @@
 //   CacheBuilder<T>(tree, data).tol(...)...     src/cache.cuh:1154-1244 (defaults: tolAbs 1e-3,
 //                                               tolRel 0, maxTime 0, maxIters 0, maxInner 8, m 3)
 //   Cache<T>::runSpock / runCp (x0, previous)   src/cache.cuh:353-355, 967-1132
-//   status / solveIter / solveTime / states / inputs / solution / reset / sizePrimal / sizeDual
-//                                               src/cache.cuh:357-412, 436-479
+//   status / solveIter / solveTime / states / inputs / solution / solutionPrim / solutionDual /
+//   getDual / reset / sizePrimal / sizeDual     src/cache.cuh:357-412, 436-479
+//   CacheBuilder<T>::build / make_unique        src/cache.cuh:1215-1244
 //   enum CacheStatus and toString               src/cache.cuh:68-88
 // Errors are std::invalid_argument with the library's message, as the reference throws.
 // T = double solves in FP64, T = float in FP32 (the reference instantiates both,
@@ -19,6 +20,7 @@
 #include <chrono>
 #include <cstdint>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -180,6 +182,26 @@ public:
         detail::check(spk_batch_get_iterate(m_b, 0, z.data()), "solution");
         return std::vector<T>(z.begin(), z.end());
     }
+    std::vector<T> solutionPrim() {
+        std::vector<T> z = solution();
+        z.resize(sizePrimal());
+        return z;
+    }
+    std::vector<T> solutionDual() {
+        const std::vector<T> z = solution();
+        return std::vector<T>(z.begin() + sizePrimal(), z.end());
+    }
+
+    // Dual point of the CP step from the current iterate with `primal` in place of the primal prox
+    // (src/cache.cuh:371-376): modifyDual + proximalDual.
+    std::vector<T> getDual(const std::vector<T> &primal) {
+        if (primal.size() != sizePrimal()) throw std::invalid_argument("[getDual] primal has the wrong size");
+        std::vector<double> z(sizePrimal() + sizeDual()), out(z.size(), 0.);
+        detail::check(spk_batch_get_iterate(m_b, 0, z.data()), "getDual");
+        std::copy(primal.begin(), primal.end(), out.begin());
+        detail::check(spk_batch_debug_op(m_b, SPK_OP_DUAL, z.data(), out.data(), nullptr), "getDual");
+        return std::vector<T>(out.begin() + sizePrimal(), out.end());
+    }
 
     // states x (numNodes x numStates) and inputs u (numNonleafNodes x numInputs) of the solution,
     // undoing the preconditioning (src/cache.cuh:386-412)
@@ -250,6 +272,7 @@ public:
     CacheBuilder &andersonBuffer(size_t buffer) { m_opts.anderson_buffer = (int) buffer; return *this; }
     CacheBuilder &enableDebug(bool = true) { return *this; }  // per-iteration traces: not provided
     Cache<T> build() { return Cache<T>(m_tree, m_data, m_opts); }
+    std::unique_ptr<Cache<T>> make_unique() { return std::make_unique<Cache<T>>(m_tree, m_data, m_opts); }
 
 private:
     ScenarioTree<T> &m_tree;

@@ -1187,7 +1187,8 @@ int spk_batch_results(spk_batch *b, spk_result *out, double *kernelSeconds) {
 
 int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, double *scalars) {
     try {
-        if (op < SPK_OP_L || op > SPK_OP_ELL) throw std::runtime_error("unknown debug op");
+        if (op < SPK_OP_L || op > SPK_OP_DUAL) throw std::runtime_error("unknown debug op");
+        if (op == SPK_OP_DUAL && b->lanes) throw std::runtime_error("the DUAL op needs a per-instance batch");
         check(hipSetDevice(b->ctx->device), "hipSetDevice");
         check(hipStreamSynchronize(b->ctx->stream), "sync");
         if (b->lanes) {

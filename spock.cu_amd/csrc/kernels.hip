@@ -1420,7 +1420,7 @@ __global__ void __launch_bounds__(SPK_MAX_THREADS) k_solve(const Inst<T> *__rest
 
 // ---------------------------------------------------------------- single-operator kernels
 // Debug entry points used by the parity tests: they read P.z and write P.cand.
-enum DebugOp { OP_L = 0, OP_LADJ = 1, OP_DYN = 2, OP_KERNEL = 3, OP_T = 4, OP_ELL = 5 };
+enum DebugOp { OP_L = 0, OP_LADJ = 1, OP_DYN = 2, OP_KERNEL = 3, OP_T = 4, OP_ELL = 5, OP_DUAL = 6 };
 
 template <typename T, int R, int V>
 __global__ void __launch_bounds__(SPK_MAX_THREADS) k_debug(const Inst<T> *__restrict__ insts, Out *outs, int count, int op,
@@ -1465,6 +1465,8 @@ __global__ void __launch_bounds__(SPK_MAX_THREADS) k_debug(const Inst<T> *__rest
         phaseProjections<T, R, V>(P, c, cand, op == OP_DYN, op == OP_KERNEL);
     } else if (op == OP_T) {
         applyT<T, R, V>(P, c, z, cand);
+    } else if (op == OP_DUAL) {  // Cache::getDual (src/cache.cuh:371-376): dual prox of the given primal
+        phaseDualProx<T, R, V>(P, c, z, cand);
     } else if (op == OP_ELL) {
         T dot, mx;
         reduceEll<T, R, V>(P, c, z, cand, lds, dot, mx);

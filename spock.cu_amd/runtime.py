@@ -14,7 +14,7 @@ LIB_PATH = os.environ.get("SPK_LIB") or os.path.join(HERE, "lib", "libspock_amd.
 F64, F32 = 0, 1
 LS_HOUSEHOLDER, LS_GRAM = 0, 1
 SPOCK, CP = 0, 1
-OP_L, OP_LADJ, OP_DYN, OP_KERNEL, OP_T, OP_ELL = range(6)
+OP_L, OP_LADJ, OP_DYN, OP_KERNEL, OP_T, OP_ELL, OP_DUAL = range(7)
 STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
 
 EXPORTS = [

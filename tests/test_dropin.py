@@ -37,6 +37,31 @@ int main(int argc, char **argv) {
 '''
 
 
+ACCESSORS = r'''
+#include <cmath>
+#include <cstdio>
+#include "spock_amd.hpp"
+using namespace spock_amd;
+int main(int argc, char **argv) {
+    ScenarioTree<double> tree(argv[1]);
+    ProblemData<double> problem(tree);
+    CacheBuilder<double> builder(tree, problem);
+    std::unique_ptr<Cache<double>> cache = builder.tol(1e-3).maxIters(3000).make_unique();
+    std::vector<double> x0 = parseFromFile<double>(tree.path() + "initialState" + tree.fpFileExt());
+    const int status = cache->runSpock(x0);
+    std::vector<double> p = cache->solutionPrim(), d = cache->solutionDual(), z = cache->solution();
+    std::vector<double> g = cache->getDual(p);
+    double diff = 0., scale = 0.;
+    for (size_t k = 0; k < d.size(); k++) {
+        diff = std::fmax(diff, std::fabs(g[k] - d[k]));
+        scale = std::fmax(scale, std::fabs(d[k]));
+    }
+    std::printf("%d %zu %zu %zu %zu %.3e\n", status, p.size(), d.size(), z.size(), cache->states().size(), diff / scale);
+    return 0;
+}
+'''
+
+
 def _compile(src, out, tmp_path):
     path = tmp_path / "prog.cpp"
     path.write_text(src)
@@ -81,3 +106,18 @@ def test_reference_driver_runs_on_the_drop_in(tmp_path):
     assert run.returncode == 0 and "Saved" in run.stdout
     row = (tmp_path / "time.csv").read_text().strip().split(",")
     assert len(row) == 3 and float(row[0]) > 0.0
+
+
+@needs_lib
+@pytest.mark.gpu
+def test_drop_in_accessors(tmp_path):
+    """make_unique, solutionPrim/solutionDual, states and getDual: at a converged solution the dual
+    step from the solution's own primal part returns (nearly) the solution's dual part."""
+    folder = str(tmp_path / "data")
+    _problem_with_initial_state(folder)
+    exe = tmp_path / "accessors"
+    _compile(ACCESSORS, exe, tmp_path)
+    out = subprocess.run([str(exe), folder + "/"], check=True, capture_output=True, text=True, timeout=120).stdout.split()
+    status, n_prim, n_dual, n_iter, n_states, rel = int(out[0]), int(out[1]), int(out[2]), int(out[3]), int(out[4]), float(out[5])
+    assert status == 0 and n_prim + n_dual == n_iter and n_states == 31 * 3
+    assert rel < 1e-2

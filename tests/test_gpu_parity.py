@@ -78,6 +78,23 @@ def test_operators_match_oracle(ctx, problem_dirs, name, wgs):
     assert abs(sc[1] - np.max(np.abs(y / o.gamma - ell))) <= 1e-12 * np.max(np.abs(y / o.gamma))
 
 
+@pytest.mark.parametrize("name", ["fixture", "mid", "poly"])
+def test_get_dual_matches_oracle(ctx, problem_dirs, name):
+    """Cache::getDual (src/cache.cuh:371-376): the dual step from the iterate with a given primal
+    point; with the primal prox of T(z) as that point it is the dual part of T(z)."""
+    d = problem_dirs[name]
+    o = Oracle(d)
+    b = rt.Batch(ctx, [rt.Problem(ctx, d)], rt.F64)
+    x0 = x0_for(name, o.nx)
+    b.set_x0([x0])
+    z = inert_zero(o, 2.0 * np.random.default_rng(11).standard_normal(o.n_iter))
+    tz = o.T(o.scaled_x0(x0), z)
+    out = np.zeros(o.n_iter)
+    out[:o.n_prim] = tz[:o.n_prim]
+    got, _ = b.debug_op(rt.OP_DUAL, z, out)
+    assert rel_err(got[o.n_prim:], tz[o.n_prim:]) < 1e-12
+
+
 @pytest.mark.parametrize("name,iters,method,wgs", [
     ("fixture", 75, "spock", 1), ("small", 75, "spock", 1), ("small", 250, "cp", 1), ("fixture", 250, "cp", 1),
     ("mid", 50, "spock", 1), ("poly", 75, "spock", 1), ("poly", 250, "cp", 1),
