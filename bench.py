@@ -106,9 +106,25 @@ class Pipeline:
         return self.rt.options(**o)
 
     def solve(self, b, warm, label, **kw):
-        """A chunked solve of a shared batch, reporting after every chunk."""
+        """A chunked solve of a shared batch: one HIP graph of its chunk launches (--graph), or
+        launch by launch with a report after every chunk."""
         t = time.perf_counter()
         launches = [0]
+        if self.a.graph:
+            o = self.opts(chunk_iters=self.a.chunk, **kw)
+            n_launch = max(1, -(-o.max_iters // self.a.chunk) + 1)
+            b.solve_graph(o, warm, n_launch)
+            res, sec = b.results()
+            while any(r["status"] == -1 for r in res):  # not reached: the graph covers max_iters
+                b.resume()
+                res, s2 = b.results()
+                sec += s2
+                n_launch += 1
+            log(f"  {label}: {sum(r['status'] != -1 for r in res)}/{len(res)} finished, "
+                f"{time.perf_counter() - t:.1f} s (one graph of {n_launch} launches)")
+            self.kernel_s.append((b.dtype, sec, sum(r["num_T"] for r in res), sum(r["num_ell"] for r in res),
+                                  n_launch, label))
+            return res
 
         def report(res, sec):
             launches[0] += 1
@@ -296,6 +312,7 @@ def main():
                          "more polish iterations)")
     ap.add_argument("--fallback", action="store_true", help="solve what the warm path leaves with cold solves")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--graph", type=int, default=1, help="1: each chunked solve of the step is one HIP graph")
     ap.add_argument("--tune-iters", type=int, default=50, help="outer iterations per schedule the autotuner times")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")

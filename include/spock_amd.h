@@ -165,6 +165,13 @@ int spk_batch_set_schedule(spk_batch *b, int mv, int block);
  * (initial states and given tolerances are kept).  opts supplies the solver options of the timed
  * runs (nullable: defaults). */
 int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv, int *block, double *seconds);
+/* shared batches: a chunked solve (opts->chunk_iters > 0) captured as one HIP graph of `launches`
+ * kernel launches (the first and launches - 1 resumes) and replayed with no host round trip between
+ * chunks; workgroups whose instances have all finished return at once from the remaining launches.
+ * The iterates are those of one uninterrupted solve.  If launches x chunk_iters is too few, results
+ * report SPK_NOT_RUN for unfinished instances and spk_batch_resume continues.  No counterpart in the
+ * reference. */
+int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int launches);
 /* wait for the last solve; fills count results and the kernel time in seconds (nullable) */
 int spk_batch_results(spk_batch *b, spk_result *out, double *kernel_seconds);
 /* single operator on every instance: in/out are host arrays concatenated over instances

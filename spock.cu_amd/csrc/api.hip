@@ -89,6 +89,9 @@ struct spk_batch {
     size_t lRegion[16] = {};  // element offsets: work[0..8], zio, rio, hS, hY, qd, dd, x0
     static constexpr int LANE_MAXM = 4;
     int laneMv = 0;      // matrix-vector schedule of the lane kernels: 0 scalar loads, 8 LDS-staged
+    DevBuf dInstResume;  // the lane instance of the resume launches of a captured chunked solve
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
     int laneBlockSz = 0; // workgroup size chosen by spk_batch_autotune (0: none)
     bool tuned = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -462,9 +465,9 @@ int laneBlock(const spk_batch *b, const spk_options &o) {
 }
 
 template <typename T>
-void launchLaneKernel(spk_batch *b, const spk_options &o) {
+void launchLaneKernel(spk_batch *b, const spk_options &o, const void *inst = nullptr) {
     const dim3 grid((unsigned) (b->Bpad / 64)), block(laneBlock(b, o));
-    auto *P = static_cast<const spk::LaneInst<T> *>(b->dInst.p);
+    auto *P = static_cast<const spk::LaneInst<T> *>(inst ? inst : b->dInst.p);
     auto *outs = static_cast<spk::Out *>(b->dOut.p);
     if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_solve_x8<T>, grid, block, 0, b->ctx->stream, P, outs);
     else hipLaunchKernelGGL(spk::k_lane_solve<T>, grid, block, 0, b->ctx->stream, P, outs);
@@ -476,11 +479,42 @@ void launchLaneSolve(spk_batch *b, const spk_options &o, int warm, int resume = 
     b->lastOpts = o;
     b->lastWarm = warm;
     b->resumable = o.chunk_iters > 0;
-    b->dOut.alloc(sizeof(spk::Out) * b->count);
+    // kept across launches: a resume of finished workgroups leaves their results in place
+    if (b->dOut.bytes != sizeof(spk::Out) * b->count) b->dOut.alloc(sizeof(spk::Out) * b->count);
     check(hipEventRecord(b->ev0, b->ctx->stream), "event");
     launchLaneKernel<T>(b, o);
     check(hipGetLastError(), "launch k_lane_solve");
     check(hipEventRecord(b->ev1, b->ctx->stream), "event");
+    b->pending = true;
+}
+
+// A chunked solve of a shared batch as one HIP graph: the first launch and launches - 1 resumes,
+// with no host round trip between them (workgroups whose solve has ended return at once).
+template <typename T>
+void launchLaneGraph(spk_batch *b, const spk_options &o, int warm, int launches) {
+    hipStream_t st = b->ctx->stream;
+    buildLaneInst<T>(b, o, warm, 1);
+    b->dInstResume.alloc(sizeof(spk::LaneInst<T>));
+    check(hipMemcpyAsync(b->dInstResume.p, b->dInst.p, sizeof(spk::LaneInst<T>), hipMemcpyDeviceToDevice, st),
+          "resume instance");
+    buildLaneInst<T>(b, o, warm, 0);
+    if (b->dOut.bytes != sizeof(spk::Out) * b->count) b->dOut.alloc(sizeof(spk::Out) * b->count);
+    check(hipStreamSynchronize(st), "sync before capture");
+    if (b->graphExec) check(hipGraphExecDestroy(b->graphExec), "graph exec destroy");
+    if (b->graph) check(hipGraphDestroy(b->graph), "graph destroy");
+    b->graphExec = nullptr;
+    b->graph = nullptr;
+    check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+    launchLaneKernel<T>(b, o);
+    for (int i = 1; i < launches; i++) launchLaneKernel<T>(b, o, b->dInstResume.p);
+    check(hipStreamEndCapture(st, &b->graph), "end capture");
+    check(hipGraphInstantiate(&b->graphExec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+    b->lastOpts = o;
+    b->lastWarm = warm;
+    b->resumable = true;
+    check(hipEventRecord(b->ev0, st), "event");
+    check(hipGraphLaunch(b->graphExec, st), "graph launch");
+    check(hipEventRecord(b->ev1, st), "event");
     b->pending = true;
 }
 
@@ -725,6 +759,8 @@ int spk_batch_is_shared(const spk_batch *b) { return b && b->lanes ? 1 : 0; }
 int spk_batch_destroy(spk_batch *b) {
     if (!b) return 0;
     if (b->ctx) (void) hipStreamSynchronize(b->ctx->stream);
+    if (b->graphExec) (void) hipGraphExecDestroy(b->graphExec);
+    if (b->graph) (void) hipGraphDestroy(b->graph);
     if (b->ev0) (void) hipEventDestroy(b->ev0);
     if (b->ev1) (void) hipEventDestroy(b->ev1);
     delete b;
@@ -1083,6 +1119,22 @@ int spk_batch_resume(spk_batch *b) {
         check(hipStreamSynchronize(b->ctx->stream), "sync");
         if (b->dtype == SPK_F64) launchLaneSolve<double>(b, b->lastOpts, b->lastWarm, 1);
         else launchLaneSolve<float>(b, b->lastOpts, b->lastWarm, 1);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int launches) {
+    try {
+        if (!b->lanes) throw std::runtime_error("graph-captured chunked solves need a shared batch");
+        if (!opts || opts->chunk_iters <= 0) throw std::runtime_error("a graph-captured solve needs chunk_iters > 0");
+        if (launches < 1) throw std::runtime_error("launches must be >= 1");
+        if (opts->anderson_buffer < 1 || opts->anderson_buffer > spk_batch::LANE_MAXM)
+            throw std::runtime_error("shared batches support anderson_buffer 1..4");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        if (b->dtype == SPK_F64) launchLaneGraph<double>(b, *opts, warm, launches);
+        else launchLaneGraph<float>(b, *opts, warm, launches);
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);

@@ -57,10 +57,11 @@ struct LaneSave {
 };
 
 // Per-workgroup state of a chunked solve: buffer roles (indices of z, cand, zPrev, res, resPrev,
-// backup into work[0..5]), Anderson ring head, next outer iteration and device ticks spent.
+// backup into work[0..5]), Anderson ring head, whether the workgroup's solve has ended (its
+// results are final and a resume returns at once), next outer iteration and device ticks spent.
 struct LaneGroupSave {
     int roles[6];
-    int head, pad;
+    int head, finished;
     long it, ticks;
 };
 

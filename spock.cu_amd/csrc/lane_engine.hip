@@ -899,6 +899,8 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
     LaneGroupSave *gs = P.gsave + blockIdx.x;
     long itStart = 0, ticksBefore = 0;
     int headSaved = m - 1;
+    // a resume of a workgroup whose solve has ended leaves its results as they are
+    if (resume && __builtin_amdgcn_readfirstlane(gs->finished)) return;
     if (resume) {  // roles of the six rotating buffers, ring head, next iteration (uniform)
         T **roles[6] = {&Z, &C, &ZP, &RR, &RP, &BK};
 #pragma unroll
@@ -1245,6 +1247,7 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
     }
     if (chunkExit) {  // keep everything a resume needs; results report the progress so far
         if (threadIdx.x == 0) {
+            gs->finished = 0;
             T *cur[6] = {Z, C, ZP, RR, RP, BK};
             for (int i = 0; i < 6; i++)
                 for (int j = 0; j < 6; j++)
@@ -1273,6 +1276,7 @@ __device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *p
         }
         if (active) iters = it;
     }
+    if (!chunkExit && threadIdx.x == 0) gs->finished = 1;
     const bool capped = active && !chunkExit;
     if (anyLane(capped)) {  // iteration cap of the launch: the current iterate is the output
         const LV<T> zio = V(P.zio), rio = V(P.rio), z = V(Z), rr = V(RR);

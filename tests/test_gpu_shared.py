@@ -273,3 +273,36 @@ def test_autotune_keeps_the_solution(ctx, problem_dirs):
     res_ref, _ = ref.results()
     assert [r["iters"] for r in res] == [r["iters"] for r in res_ref]
     assert np.array_equal(b.iterates(), ref.iterates())
+
+
+@pytest.mark.parametrize("dtype", [rt.F64, rt.F32])
+def test_graph_captured_chunked_solve_equals_uninterrupted(ctx, problem_dirs, dtype):
+    """A chunked solve captured as one HIP graph: too few launches leave instances running (a
+    resume finishes them), enough launches finish in one replay; both equal one uninterrupted
+    solve bit for bit, and extra launches after the end change nothing."""
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    count = 100
+    x0s = x0s_for("small", o.nx, count)
+    opts = dict(tol_abs=1e-3, tol_rel=1e-3, max_iters=600)
+    _, ref = shared(ctx, d, count, dtype)
+    ref.set_x0(x0s)
+    ref.solve(rt.options(**opts))
+    res_ref, _ = ref.results()
+    _, g = shared(ctx, d, count, dtype)
+    g.set_x0(x0s)
+    g.solve_graph(rt.options(chunk_iters=50, **opts), launches=3)
+    res, _ = g.results()
+    assert any(r["status"] == -1 for r in res)
+    while any(r["status"] == -1 for r in res):
+        g.resume()
+        res, _ = g.results()
+    assert [(r["status"], r["iters"]) for r in res] == [(r["status"], r["iters"]) for r in res_ref]
+    assert np.array_equal(g.iterates(), ref.iterates())
+    for launches in (None, 40):
+        _, g2 = shared(ctx, d, count, dtype)
+        g2.set_x0(x0s)
+        g2.solve_graph(rt.options(chunk_iters=50, **opts), launches=launches)
+        res2, _ = g2.results()
+        assert [(r["status"], r["iters"]) for r in res2] == [(r["status"], r["iters"]) for r in res_ref]
+        assert np.array_equal(g2.iterates(), ref.iterates())
