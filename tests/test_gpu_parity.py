@@ -189,3 +189,67 @@ def test_fp32_solve_tracks_fp64(ctx, problem_dirs):
     s0 = z[o.n_prim - o.sS]
     s0_ref = ref["z"][o.n_prim - o.sS]
     assert abs(s0 - s0_ref) <= 2e-2 * max(1.0, abs(s0_ref))
+
+
+@pytest.fixture(scope="module")
+def uneven_family(tmp_path_factory):
+    """Instances 0..11 of the uneven family (horizon 5-12, 2-4 events, branching stops at 1-3,
+    nx 10-40, nu 4-16; 19 to 256 nodes)."""
+    base = tmp_path_factory.mktemp("uneven")
+    out = []
+    for k in range(12):
+        d = str(base / f"i{k}")
+        configs.uneven_instance(d, k)
+        out.append(d)
+    return out
+
+
+def test_uneven_family_batch_matches_oracle(ctx, uneven_family):
+    """One persistent launch over 12 differently shaped trees: every instance's iterate after 50
+    SPOCK iterations matches the oracle's to 1e-9 with the same branch counts."""
+    oracles = [Oracle(d) for d in uneven_family]
+    probs = [rt.Problem(ctx, d) for d in uneven_family]
+    x0s = [configs.initial_states(4000 + k, o.nx, 1)[0] for k, o in enumerate(oracles)]
+    b = rt.Batch(ctx, probs, rt.F64)
+    b.set_x0(x0s)
+    b.solve(rt.options(tol_abs=1e-30, max_iters=50))
+    res, _ = b.results()
+    for k, o in enumerate(oracles):
+        ref = o.solve(x0s[k], tol_abs=1e-30, max_iters=50)
+        r = res[k]
+        assert (r["iters"], r["k0"], r["k1"], r["k2"], r["k2bt"], r["k3"]) == \
+            (ref["iters"], ref["k0"], ref["k1"], ref["k2"], ref["k2bt"], ref["k3"])
+        assert rel_err(b.iterate(k), ref["z"]) < 1e-9
+
+
+def test_uneven_family_fp32_bulk_and_fp64_polish(ctx, uneven_family):
+    """FP32 solve of an uneven batch (the ten instances of at most 85 nodes), then an FP64 polish
+    warm-started from it against the plain solves' thresholds: every instance the plain FP64
+    solve converges on passes the FP64 residual test after the polish, at the plain solve's cost."""
+    sel = [k for k in range(len(uneven_family)) if k not in (0, 5)]
+    dirs = [uneven_family[k] for k in sel]
+    probs = [rt.Problem(ctx, d) for d in dirs]
+    x0s = [configs.initial_states(4000 + k, p.nx, 1)[0] for k, p in zip(sel, probs)]
+    opts = dict(tol_abs=1e-3, tol_rel=1e-3, max_iters=4000)
+    plain = rt.Batch(ctx, probs, rt.F64)
+    plain.set_x0(x0s)
+    plain.solve(rt.options(**opts))
+    res_p, _ = plain.results()
+    b32 = rt.Batch(ctx, probs, rt.F32)
+    b32.set_x0(x0s)
+    b32.solve(rt.options(**opts))
+    b32.results()
+    b64 = rt.Batch(ctx, probs, rt.F64)
+    b64.set_x0(x0s)
+    b64.copy_iterates_from(b32)
+    b64.set_tolerances([r["tol"] for r in res_p])
+    b64.solve(rt.options(**opts), warm=True)
+    res64, _ = b64.results()
+    ok = [i for i in range(len(probs)) if res_p[i]["status"] == 0]
+    assert len(ok) >= 4  # 5 of these 10 random trees converge within 4,000 plain iterations
+    for i in ok:
+        assert res64[i]["status"] == 0
+        o = Oracle(dirs[i])
+        s = o.n_prim - o.sS
+        s_p, s_64 = plain.iterate(i)[s], b64.iterate(i)[s]
+        assert abs(s_64 - s_p) <= 2e-2 * max(1.0, abs(s_p))
