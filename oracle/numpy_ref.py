@@ -26,8 +26,9 @@ def dense_operator(apply, n_in):
     return np.array(cols).T
 
 
-def kkt_dynamics_projection(AB_nodes, anc, nl, nx, nu, x0, xbar, ubar):
-    """AB_nodes: (N, nx, nx+nu) per-node [A B] (node 0 unused); xbar (N, nx), ubar (nl, nu)."""
+def kkt_dynamics_projection(AB_nodes, anc, nl, nx, nu, x0, xbar, ubar, e_nodes=None):
+    """AB_nodes: (N, nx, nx+nu) per-node [A B] (node 0 unused); xbar (N, nx), ubar (nl, nu);
+    e_nodes: (N, nx) constants of affine dynamics x_j = A_j x_anc + B_j u_anc + e_j (or None)."""
     N = len(anc)
     nvar = N * nx + nl * nu
 
@@ -50,7 +51,7 @@ def kkt_dynamics_projection(AB_nodes, anc, nl, nx, nu, x0, xbar, ubar):
         c[:, xi(a)] -= AB_nodes[j][:, :nx]
         c[:, ui(a)] -= AB_nodes[j][:, nx:]
         rows.append(c)
-        rhs.append(np.zeros(nx))
+        rhs.append(np.zeros(nx) if e_nodes is None else np.asarray(e_nodes[j], dtype=np.float64))
     Aeq = np.vstack(rows)
     beq = np.concatenate(rhs)
     target = np.concatenate([xbar.reshape(-1), ubar.reshape(-1)])

@@ -81,7 +81,8 @@ class Oracle:
         ints = np.array([info["numNodes"], info["numNonleafNodes"], info["numStages"], info["numEvents"],
                          info["numStates"], info["numInputs"], _COST[info["cost"]["nonleaf"]],
                          _COST[info["cost"]["leaf"]], _CON[info["constraint"]["nonleaf"]],
-                         _CON[info["constraint"]["leaf"]], int(bool(info["preconditioned"]))], dtype=np.int64)
+                         _CON[info["constraint"]["leaf"]], int(bool(info["preconditioned"])),
+                         int(info["dynamics"]["type"] == "affine")], dtype=np.int64)
         err = C.create_string_buffer(512)
         path = folder if folder.endswith("/") else folder + "/"
         self.h = lib().orc_load(path.encode(), ints.ctypes.data_as(_lp), float(info["stepSize"]), err, 512)

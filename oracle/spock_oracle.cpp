@@ -101,6 +101,8 @@ struct Problem {
     double gamma, gammaRecip;
     std::vector<long> anc, chFrom, numCh, stageFrom, stageTo;
     Vec BTr, AB, K, ABKTr, APB, lowChol, NNtr, b, scaling;
+    bool affine = false;
+    Vec e, Pe;  // affine dynamics: constant e_j and P_j e_j per node (src/dynamics.cuh:229-237)
     Vec sqrtQ, sqrtR, trans, q, r, lsqrtQ, ltrans, lq;
     Vec nLB, nUB, nG, lLB, lUB, lG;  // per-node bounds (replicated) and matrices
     size_t sU, sX, sY, sT, sS, sPrim, sI, sII, sIII, sIV, sV, sVI, sDual, sIter;
@@ -131,6 +133,15 @@ struct Problem {
         ABKTr = readFp(dir, "dynamics_ABKTr");
         APB = readFp(dir, "dynamics_APB");
         lowChol = readFp(dir, "dynamics_lowCholesky");
+        affine = ints[11] != 0;
+        if (affine) {  // src/dynamics.cuh:232-236: e from dynamics_c, Pe = P e per node
+            e = readFp(dir, "dynamics_c");
+            const Vec P = readFp(dir, "dynamics_P");
+            Pe.assign((size_t) nx * N, 0.);
+            for (int n = 0; n < N; n++)
+                gemv(P.data() + (size_t) n * nx * nx, nx, nx, e.data() + (size_t) n * nx, Pe.data() + (size_t) n * nx,
+                     false);
+        }
         NNtr = readFp(dir, "risk_NNtr");
         b = readFp(dir, "risk_b");
         if (precond) scaling = readFp(dir, "scaling");
@@ -335,6 +346,12 @@ void dynProject(const Problem &p, const double *x0, double *x, double *u) {
     double w[1024], acc[1024];
     for (int stage = p.S - 2; stage >= 0; stage--) {
         for (long i = p.stageFrom[stage]; i <= p.stageTo[stage]; i++) {
+            // affine dynamics: the children's q carry + P_j e_j from here on (src/dynamics.cuh:93-97)
+            if (p.affine)
+                for (long c = 0; c < p.numCh[i]; c++) {
+                    const long j = p.chFrom[i] + c;
+                    for (int k = 0; k < nx; k++) q[(size_t) j * nx + k] += p.Pe[(size_t) j * nx + k];
+                }
             // d_i = R~^-1 (u_i - sum_j B_j' q_j)
             for (int k = 0; k < nu; k++) acc[k] = 0.;
             for (long c = 0; c < p.numCh[i]; c++) {
@@ -382,6 +399,8 @@ void dynProject(const Problem &p, const double *x0, double *x, double *u) {
             for (long c = 0; c < p.numCh[i]; c++) {
                 const long j = p.chFrom[i] + c;
                 gemv(p.AB.data() + (size_t) j * nx * (nx + nu), nx, nx + nu, xu, x + (size_t) j * nx, false);
+                if (p.affine)  // src/dynamics.cuh:168-171
+                    for (int k = 0; k < nx; k++) x[(size_t) j * nx + k] += p.e[(size_t) j * nx + k];
             }
         }
     }

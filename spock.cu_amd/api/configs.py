@@ -131,6 +131,38 @@ def polyhedral_small(folder, seed=33, dt="d"):
     return polyhedral_iid(folder, seed=seed, dt=dt, events=3, horizon=3, nx=4, nu=2)
 
 
+def unconstrained_markov(folder, seed=7, dt="d"):
+    """The 31-node family (2 events, horizon 4; nx=3, nu=2; AV@R 0.95) with no state or input
+    constraints at any node (the reference's ``no`` constraint)."""
+    rng = np.random.default_rng(seed)
+    tr = _markov(rng, 2, 4, 4, dt)
+    nx, nu = 3, 2
+    dyn = [build.Dynamics(np.eye(nx) + 0.1 * rng.standard_normal((nx, nx)), 0.5 * rng.standard_normal((nx, nu)))
+           for _ in range(tr.num_events)]
+    costs = [build.CostQuadratic(np.eye(nx), 0.1 * np.eye(nu)) for _ in range(tr.num_events)]
+    leaf = build.CostQuadratic(np.eye(nx), leaf=True)
+    return (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(build.No()).with_constraint_leaf(build.No())
+            .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
+
+
+def affine_markov(folder, seed=8, dt="d"):
+    """The 31-node box-constrained family with affine dynamics x+ = A x + B u + c, c ~ N(0, 0.1^2)
+    per event (the reference's ``affine`` dynamics, src/dynamics.cuh:223-238)."""
+    rng = np.random.default_rng(seed)
+    tr = _markov(rng, 2, 4, 4, dt)
+    nx, nu = 3, 2
+    dyn = [build.Dynamics(np.eye(nx) + 0.1 * rng.standard_normal((nx, nx)), 0.5 * rng.standard_normal((nx, nu)),
+                          0.1 * rng.standard_normal(nx)) for _ in range(tr.num_events)]
+    costs = [build.CostQuadratic(np.eye(nx), 0.1 * np.eye(nu)) for _ in range(tr.num_events)]
+    leaf = build.CostQuadratic(np.eye(nx), leaf=True)
+    lim = np.vstack((5.0 * np.ones((nx, 1)), 2.0 * np.ones((nu, 1))))
+    return (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(build.Rectangle(-lim, lim))
+            .with_constraint_leaf(build.Rectangle(-5.0 * np.ones((nx, 1)), 5.0 * np.ones((nx, 1))))
+            .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
+
+
 def initial_states(seed, nx, count, kind="normal"):
     """Initial states: ``normal`` ~ N(0, 0.5^2 I), ``small`` ~ N(0, 0.05^2 I), ``uniform`` ~ U[-1, 1]."""
     rng = np.random.default_rng([seed, 1])
@@ -143,5 +175,5 @@ def initial_states(seed, nx, count, kind="normal"):
 
 def generate(name, folder, **kw):
     os.makedirs(folder, exist_ok=True)
-    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov, "poly": polyhedral_small,
+    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov, "poly": polyhedral_small, "nocon": unconstrained_markov, "affine": affine_markov,
             "large": polyhedral_iid}[name](folder, **kw)

@@ -133,7 +133,7 @@ void uploadTable(spk_problem *p, const std::string &name, const Table &t) {
 
 #define SPK_TABLES(X) \
     X(BTr) X(AB) X(K) X(KTr) X(ABKTr) X(APB) X(cholL) X(cholU) X(RtInv) X(NN) X(b) X(sQ) X(sR) X(q) X(r) X(trans) \
-    X(lsQ) X(lq) X(ltrans) X(nG) X(nLB) X(nUB) X(lG) X(lLB) X(lUB) X(nGT) X(lGT)
+    X(lsQ) X(lq) X(ltrans) X(nG) X(nLB) X(nUB) X(lG) X(lLB) X(lUB) X(nGT) X(lGT) X(e) X(Pe)
 
 template <typename T>
 void ensureUploaded(spk_problem *p) {
@@ -174,6 +174,7 @@ void fillProblem(I &I_, spk_problem *p) {
     I_.N = h.N; I_.nl = h.nl; I_.nlf = h.nlf; I_.S = h.S; I_.E = h.E; I_.nx = h.nx; I_.nu = h.nu;
     I_.yD = h.yD; I_.nD = h.nD; I_.ncost = h.ncost; I_.lcost = h.lcost; I_.ncon = h.ncon; I_.lcon = h.lcon;
     I_.con3 = h.con3; I_.con5 = h.con5;
+    I_.affine = h.affine ? 1 : 0;
     I_.oU = 0; I_.oX = h.sU; I_.oY = I_.oX + h.sX; I_.oT = I_.oY + h.sY; I_.oS = I_.oT + h.sT; I_.sPrim = h.sPrim;
     I_.oI = 0; I_.oII = h.sI; I_.oIII = I_.oII + h.sII; I_.oIV = I_.oIII + h.sIII; I_.oV = I_.oIV + h.sIV;
     I_.oVI = I_.oV + h.sV; I_.sDual = h.sDual; I_.sIter = h.sIter;
@@ -207,6 +208,7 @@ void buildInsts(spk_batch *b, const spk_options &o, int warm) {
         I.N = h.N; I.nl = h.nl; I.nlf = h.nlf; I.S = h.S; I.E = h.E; I.nx = h.nx; I.nu = h.nu;
         I.yD = h.yD; I.nD = h.nD; I.ncost = h.ncost; I.lcost = h.lcost; I.ncon = h.ncon; I.lcon = h.lcon;
         I.con3 = h.con3; I.con5 = h.con5;
+        I.affine = h.affine ? 1 : 0;
         I.oU = 0; I.oX = h.sU; I.oY = I.oX + h.sX; I.oT = I.oY + h.sY; I.oS = I.oT + h.sT; I.sPrim = h.sPrim;
         I.oI = 0; I.oII = h.sI; I.oIII = I.oII + h.sII; I.oIV = I.oIII + h.sIII; I.oV = I.oIV + h.sIV;
         I.oVI = I.oV + h.sV; I.sDual = h.sDual; I.sIter = h.sIter;
@@ -308,7 +310,7 @@ int ldsPoolInts(const spk_batch *b) {
     int mx = 0;
     for (auto *p : b->probs) {
         const HostProblem &h = p->hp;
-        const int need = 10 * h.N + 15 * h.nl + 7 * h.nlf + 2 * h.S;
+        const int need = 12 * h.N + 15 * h.nl + 7 * h.nlf + 2 * h.S;
         mx = std::max(mx, need);
     }
     const int budget = 48 * 1024 / 4;

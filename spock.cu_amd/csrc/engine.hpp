@@ -22,6 +22,8 @@ struct Inst {
     const int *anc, *chFrom, *numCh, *stageFrom, *stageTo;
     DTab<T> BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, RtInv, NN, b, sQ, sR, q, r, trans, lsQ, lq, ltrans;
     DTab<T> nG, nLB, nUB, lG, lLB, lUB, nGT, lGT;
+    DTab<T> e, Pe;  // affine dynamics (zero tables when linear)
+    int affine;
     // state (iterate-sized unless noted)
     T *z, *zPrev, *cand, *backup, *res, *resPrev, *dir, *dirS;
     T *hS, *hY;    // Anderson histories (iterate and residual differences), andBuff columns each
@@ -77,6 +79,8 @@ struct LaneInst {
     const int *anc, *chFrom, *numCh, *stageFrom, *stageTo;
     DTab<T> BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, RtInv, NN, b, sQ, sR, q, r, trans, lsQ, lq, ltrans;
     DTab<T> nG, nLB, nUB, lG, lLB, lUB, nGT, lGT;
+    DTab<T> e, Pe;  // affine dynamics (zero tables when linear)
+    int affine;
     long B;     // padded instance count (multiple of 64); a region of r rows holds r * B words
     int count;  // instances in use
     T *work[9];   // rotating roles z, cand, zPrev, res, resPrev, backup + cand2, dir, dirS

@@ -622,7 +622,8 @@ __device__ __noinline__ void phaseAdjAxpy(const Inst<T> &P, const Tc &c, const T
     }
 }
 
-// Backward sweep step for node i (src/dynamics.cuh:85-139).
+// Backward sweep step for node i (src/dynamics.cuh:85-139).  With affine dynamics the children's
+// q enter as q_j + P_j e_j (src/dynamics.cuh:93-97).
 template <typename T, int R, int V>
 __device__ __forceinline__ void dynBackwardNode(const Inst<T> &P, T *cand, int i, int stage, int lane) {
     const int nx = P.nx, nu = P.nu;
@@ -631,9 +632,18 @@ __device__ __forceinline__ void dynBackwardNode(const Inst<T> &P, T *cand, int i
     T v[V], w[R], acc[R];
     zero(acc);
     const int c0 = P.chFrom[i], nc = P.numCh[i];
+    auto loadQ = [&](int j) {
+        const T *qj = P.qd + (long) j * nx;
+        if (P.affine) {
+            const T *pe = P.Pe.at(j);
+            loadV(v, nx, lane, [&](int k) { return qj[k] + pe[k]; });
+        } else {
+            loadPtr(v, qj, nx, lane);
+        }
+    };
     for (int ci = 0; ci < nc; ci++) {
         const int j = c0 + ci;
-        loadPtr(v, P.qd + (long) j * nx, nx, lane);
+        loadQ(j);
         zero(w);
         mvAcc(w, P.BTr.at(j), nu, nu, nx, v, lane);
 #pragma unroll
@@ -674,7 +684,7 @@ __device__ __forceinline__ void dynBackwardNode(const Inst<T> &P, T *cand, int i
         const int j = c0 + ci;
         zero(w);
         mvAcc(w, P.APB.at(j), nx, nx, nu, dv, lane);
-        loadPtr(v, P.qd + (long) j * nx, nx, lane);
+        loadQ(j);
         mvAcc(w, P.ABKTr.at(j), nx, nx, nx, v, lane);
 #pragma unroll
         for (int rc = 0; rc < R; rc++) acc[rc] = (ci == 0) ? w[rc] : acc[rc] + w[rc];
@@ -713,7 +723,7 @@ __device__ __forceinline__ void kernelNode(const Inst<T> &P, T *cand, int i, int
     }
 }
 
-// Forward rollout for node i (src/dynamics.cuh:144-173).
+// Forward rollout for node i (src/dynamics.cuh:144-173); affine dynamics add e_j to each child.
 template <typename T, int R, int V>
 __device__ __forceinline__ void dynForwardNode(const Inst<T> &P, T *cand, int i, int lane) {
     const int nx = P.nx, nu = P.nu;
@@ -741,10 +751,11 @@ __device__ __forceinline__ void dynForwardNode(const Inst<T> &P, T *cand, int i,
         zero(w);
         mvAcc(w, AB, nx, nx, nx, xv, lane);
         mvAcc(w, AB + (size_t) nx * nx, nx, nx, nu, uv, lane);
+        const T *ej = P.e.at(j);
 #pragma unroll
         for (int rc = 0; rc < R; rc++) {
             const int r = lane + 64 * rc;
-            if (r < nx) cand[P.oX + (long) j * nx + r] = w[rc];
+            if (r < nx) cand[P.oX + (long) j * nx + r] = P.affine ? w[rc] + ej[r] : w[rc];
         }
     }
 }
@@ -1399,6 +1410,8 @@ __device__ const Inst<T> &stageInstance(const Inst<T> &G, Inst<T> &L, int *pool,
     take(L.lUB.idx, nlf);
     take(L.lG.idx, nlf);
     take(L.lGT.idx, nlf);
+    take(L.e.idx, N);
+    take(L.Pe.idx, N);
     return L;
 }
 

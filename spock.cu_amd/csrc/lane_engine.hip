@@ -545,12 +545,15 @@ __device__ __forceinline__ void dynBackward(LP<T> P, const LV<T> &cand, const LV
     const int nx = P.nx, nu = P.nu;  // nu <= 32 (checked on the host)
     const int c0 = cst(P.chFrom)[i], nc = cst(P.numCh)[i];
     T y0[NY], y1[NY];
+    // affine dynamics: the children's q enter as q_j + P_j e_j (src/dynamics.cuh:93-97)
+    const bool affine = P.affine != 0;
+    auto qv = [&](int j, int cc) { return affine ? qd[(long) j * nx + cc] + tab(P.Pe, j)[cc] : qd[(long) j * nx + cc]; };
     // e = ubar - sum_j B_j' q_j, stored in dd, then d = R~^-1 e in place
     zeroY(y0);
     zeroY(y1);
     for (int ci = 0; ci < nc; ci++) {
         const int j = c0 + ci;
-        mv32(y0, y1, tab(P.BTr, j), nu, nu, nx, [&](int cc) { return qd[(long) j * nx + cc]; });
+        mv32(y0, y1, tab(P.BTr, j), nu, nu, nx, [&](int cc) { return qv(j, cc); });
     }
     auto storeE = [&](int r, T v) { dd[(long) i * nu + r] = -v + cand[P.oU + (long) i * nu + r]; };
     each(y0, min(nu, 16), storeE);
@@ -569,7 +572,7 @@ __device__ __forceinline__ void dynBackward(LP<T> P, const LV<T> &cand, const LV
         for (int ci = 0; ci < nc; ci++) {
             const int j = c0 + ci;
             mv(y, tab(P.APB, j) + r0, rr, nx, nu, [&](int cc) { return dd[(long) i * nu + cc]; });
-            mv(y, tab(P.ABKTr, j) + r0, rr, nx, nx, [&](int cc) { return qd[(long) j * nx + cc]; });
+            mv(y, tab(P.ABKTr, j) + r0, rr, nx, nx, [&](int cc) { return qv(j, cc); });
         }
         mv(y, tab(P.KTr, i) + r0, rr, nx, nu,
            [&](int cc) { return dd[(long) i * nu + cc] - cand[P.oU + (long) i * nu + cc]; });
@@ -603,7 +606,7 @@ __device__ __forceinline__ void kernelProj(LP<T> P, const LV<T> &cand, int i) {
     each(y1, P.nD - 16, [&](int k, T v) { put(16 + k, v); });
 }
 
-// Forward rollout for node i (src/dynamics.cuh:144-173).
+// Forward rollout for node i (src/dynamics.cuh:144-173); affine dynamics add e_j to each child.
 template <typename T>
 __device__ __forceinline__ void dynForward(LP<T> P, const LV<T> &cand, const LV<T> &x0s,
                                            const LV<T> &dd, int i) {
@@ -627,7 +630,12 @@ __device__ __forceinline__ void dynForward(LP<T> P, const LV<T> &cand, const LV<
             zeroY(y);
             mv(y, AB + r0, rr, nx, nx, [&](int cc) { return cand[P.oX + (long) i * nx + cc]; });
             mv(y, AB + (size_t) nx * nx + r0, rr, nx, nu, [&](int cc) { return cand[P.oU + (long) i * nu + cc]; });
-            each(y, rr, [&](int r, T v) { cand[P.oX + (long) j * nx + r0 + r] = v; });
+            if (P.affine) {
+                CP<T> ej = tab(P.e, j) + r0;
+                each(y, rr, [&](int r, T v) { cand[P.oX + (long) j * nx + r0 + r] = v + ej[r]; });
+            } else {
+                each(y, rr, [&](int r, T v) { cand[P.oX + (long) j * nx + r0 + r] = v; });
+            }
         }
     }
 }

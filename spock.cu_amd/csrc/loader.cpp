@@ -254,7 +254,9 @@ HostProblem loadProblem(const std::string &folderIn) {
     p.stepSize = std::stod(need(js, "stepSize"));
     p.precond = need(js, "preconditioned") == "true";
     std::string dyn = need(js, "dynamics.type");
-    if (dyn != "linear") throw std::runtime_error("dynamics type '" + dyn + "' is not supported by this solver");
+    if (dyn != "linear" && dyn != "affine")
+        throw std::runtime_error("dynamics type '" + dyn + "' is not supported (linear, affine)");
+    p.affine = dyn == "affine";
     auto costKind = [](const std::string &s) {
         if (s == "quadratic") return (int) COST_QUAD;
         if (s == "quadraticPlusLinear") return (int) COST_QUADLIN;
@@ -314,6 +316,22 @@ HostProblem loadProblem(const std::string &folderIn) {
     p.cholL = stackTable("dynamics_lowCholesky", nu, nu, nl);
     p.cholU = transposeTable(p.cholL);
     p.RtInv = cholInverseTable(p.cholL);
+    if (p.affine) {  // src/dynamics.cuh:229-237: e from dynamics_c, P e formed once at load
+        auto e = readFp(dir, "dynamics_c");
+        expectSize(e, (size_t) nx * N, "dynamics_c");
+        auto P = readFp(dir, "dynamics_P");
+        expectSize(P, (size_t) nx * nx * N, "dynamics_P");
+        std::vector<double> pe((size_t) nx * N, 0.);
+        for (int n = 0; n < N; n++)
+            for (int c = 0; c < nx; c++)
+                for (int r = 0; r < nx; r++)
+                    pe[(size_t) n * nx + r] += P[((size_t) n * nx + c) * nx + r] * e[(size_t) n * nx + c];
+        p.e = dedup(e, nx, 1, N);
+        p.Pe = dedup(pe, nx, 1, N);
+    } else {
+        p.e = single(std::vector<double>(nx, 0.), nx, 1, N);
+        p.Pe = single(std::vector<double>(nx, 0.), nx, 1, N);
+    }
     // risk (src/risks.cuh:26-33)
     p.NN = stackTable("risk_NNtr", p.nD, p.nD, nl);
     p.b = stackTable("risk_b", p.yD, 1, nl);

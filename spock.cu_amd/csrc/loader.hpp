@@ -40,6 +40,8 @@ struct HostProblem {
     std::vector<double> prob, condProb;
     // dynamics factors
     Table BTr, AB, K, KTr, ABKTr, APB, cholL, cholU, RtInv, P;  // RtInv = (L L')^-1
+    bool affine = false;
+    Table e, Pe;  // affine dynamics: e_j and P_j e_j per node (zero tables for linear dynamics)
     // risk
     Table NN, b;
     // costs (nonleaf tables are indexed per node 0..N-1, leaf tables per leaf)

@@ -35,7 +35,7 @@ def problem_dirs(tmp_path_factory):
     from spock_amd.api import configs
     base = tmp_path_factory.mktemp("problems")
     out = {}
-    for name in ("fixture", "small", "mid", "poly"):
+    for name in ("fixture", "small", "mid", "poly", "nocon", "affine"):
         d = str(base / name)
         configs.generate(name, d)
         out[name] = d

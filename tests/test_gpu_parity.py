@@ -32,6 +32,10 @@ def x0_for(name, nx, k=0):
         return configs.initial_states(1, nx, k + 1, "uniform")[k]
     if name == "poly":
         return configs.initial_states(33, nx, k + 1, "small")[k]
+    if name == "nocon":
+        return configs.initial_states(7, nx, k + 1, "uniform")[k]
+    if name == "affine":
+        return configs.initial_states(8, nx, k + 1, "uniform")[k]
     return configs.initial_states(2, nx, k + 1)[k]
 
 
@@ -45,7 +49,8 @@ def inert_zero(o, z):
 
 
 @pytest.mark.parametrize("name,wgs", [("fixture", 1), ("small", 1), ("mid", 1), ("poly", 1),
-                                      ("mid", 7), ("poly", 3), ("mid", 128)])
+                                      ("mid", 7), ("poly", 3), ("mid", 128), ("nocon", 1), ("affine", 1),
+                                      ("affine", 5)])
 def test_operators_match_oracle(ctx, problem_dirs, name, wgs):
     """wgs > 1: grid mode, the instance spread over a cooperative launch of wgs workgroups."""
     d = problem_dirs[name]
@@ -98,7 +103,8 @@ def test_get_dual_matches_oracle(ctx, problem_dirs, name):
 @pytest.mark.parametrize("name,iters,method,wgs", [
     ("fixture", 75, "spock", 1), ("small", 75, "spock", 1), ("small", 250, "cp", 1), ("fixture", 250, "cp", 1),
     ("mid", 50, "spock", 1), ("poly", 75, "spock", 1), ("poly", 250, "cp", 1),
-    ("mid", 50, "spock", 64), ("poly", 75, "spock", 5), ("small", 250, "cp", 16)])
+    ("mid", 50, "spock", 64), ("poly", 75, "spock", 5), ("small", 250, "cp", 16), ("nocon", 75, "spock", 1),
+    ("affine", 75, "spock", 1), ("affine", 250, "cp", 1)])
 def test_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method, wgs):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -131,7 +137,7 @@ def gpu_solve(ctx, problems, x0s, dtype=rt.F64, **opts):
     return b, res
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly", "affine"])
 def test_converged_solve_matches_oracle(ctx, problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)

@@ -33,6 +33,10 @@ def x0s_for(name, nx, count):
         return list(configs.initial_states(1, nx, count, "uniform"))
     if name == "poly":
         return list(configs.initial_states(33, nx, count, "small"))
+    if name == "nocon":
+        return list(configs.initial_states(7, nx, count, "uniform"))
+    if name == "affine":
+        return list(configs.initial_states(8, nx, count, "uniform"))
     return list(configs.initial_states(2, nx, count))
 
 
@@ -49,7 +53,7 @@ def shared(ctx, d, count, dtype=rt.F64):
     return p, rt.Batch(ctx, [p] * count, dtype, shared=True)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "affine"])
 def test_shared_operators_match_oracle(ctx, problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -94,7 +98,8 @@ def test_shared_operators_match_oracle(ctx, problem_dirs, name):
 
 
 @pytest.mark.parametrize("name,iters,method", [("fixture", 50, "spock"), ("small", 50, "spock"),
-                                               ("mid", 50, "spock"), ("small", 250, "cp"), ("poly", 50, "spock")])
+                                               ("mid", 50, "spock"), ("small", 250, "cp"), ("poly", 50, "spock"),
+                                               ("nocon", 50, "spock"), ("affine", 50, "spock")])
 def test_shared_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method):
     d = problem_dirs[name]
     o = Oracle(d)

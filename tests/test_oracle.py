@@ -25,6 +25,14 @@ from spock_amd.api import btio, build, configs
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+def affine_constants(d, o):
+    """Per-node constants e_j of affine dynamics (zeros for linear dynamics), shape (N, nx)."""
+    N = o.info["numNodes"]
+    if o.info["dynamics"]["type"] != "affine":
+        return np.zeros((N, o.nx))
+    return btio.read_bt(os.path.join(d, "dynamics_c_d.bt")).transpose(2, 0, 1).reshape(N, o.nx)
+
+
 def inert_zero(o, z):
     z = z.copy()
     z[o.sU + o.sX + o.sY] = 0.0
@@ -33,7 +41,7 @@ def inert_zero(o, z):
     return z
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine"])
 def test_adjoint(problem_dirs, name):
     o = Oracle(problem_dirs[name])
     rng = np.random.default_rng(0)
@@ -44,14 +52,14 @@ def test_adjoint(problem_dirs, name):
     assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly", "nocon", "affine"])
 def test_step_size_is_099_over_norm(problem_dirs, name):
     o = Oracle(problem_dirs[name])
     Ld = nr.dense_operator(o.L, o.n_prim)
     assert o.gamma * np.linalg.norm(Ld, 2) == pytest.approx(0.99, abs=2e-3)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly", "nocon", "affine"])
 def test_dynamics_projection_is_the_kkt_solution(problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -60,7 +68,7 @@ def test_dynamics_projection_is_the_kkt_solution(problem_dirs, name):
     anc = btio.read_flat(os.path.join(d, "ancestors_u.bt")).astype(np.int64)
     rng = np.random.default_rng(1)
     xb, ub, x0 = 10 * rng.standard_normal((N, nx)), 10 * rng.standard_normal((nl, nu)), rng.standard_normal(nx)
-    xk, uk = nr.kkt_dynamics_projection(AB, anc, nl, nx, nu, x0, xb, ub)
+    xk, uk = nr.kkt_dynamics_projection(AB, anc, nl, nx, nu, x0, xb, ub, affine_constants(d, o))
     xo, uo = o.dyn_project(x0, xb.reshape(-1), ub.reshape(-1))
     assert rel_err(xo, xk.reshape(-1)) < 1e-9
     assert rel_err(uo, uk.reshape(-1)) < 1e-9
@@ -160,7 +168,7 @@ def test_dotM_definition(problem_dirs, name):
     assert o.dotM(x, y) == pytest.approx(ref, rel=1e-12, abs=1e-12)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "poly"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly", "nocon", "affine"])
 def test_T_lands_on_the_dynamics(problem_dirs, name):
     """The primal part of T(z) is a projection onto the dynamics with x_0 = x0."""
     d = problem_dirs[name]
@@ -174,10 +182,12 @@ def test_T_lands_on_the_dynamics(problem_dirs, name):
     tz = o.T(o.scaled_x0(x0), z)
     x = tz[o.sU:o.sU + o.sX].reshape(N, nx)
     u = tz[:o.sU].reshape(nl, nu)
+    e = affine_constants(d, o)
     assert rel_err(x[0], o.scaled_x0(x0)) < 1e-14
     for j in range(1, N):
         a = anc[j]
-        assert np.max(np.abs(x[j] - AB[j][:, :nx] @ x[a] - AB[j][:, nx:] @ u[a])) < 1e-10 * max(1.0, np.abs(x).max())
+        res = x[j] - AB[j][:, :nx] @ x[a] - AB[j][:, nx:] @ u[a] - e[j]
+        assert np.max(np.abs(res)) < 1e-10 * max(1.0, np.abs(x).max())
 
 
 def test_converged_solve_is_a_fixed_point(problem_dirs):
