@@ -219,3 +219,38 @@ class IidProcess:
         if folder is not None:
             tree.write_files(folder)
         return tree
+
+
+class FromStructure:
+    """Tree from an explicit structure (``api/python/spock/tree.py:902-951``): the stage, ancestor
+    (-1 at the root) and unconditional probability of every node; events are all 0 below the root.
+    ``data`` (node x dimension) is accepted for interface parity and not used by the solver."""
+
+    def __init__(self, stages, ancestors, probabilities, data=None, dt="d"):
+        self.stages = np.asarray(stages)
+        self.ancestors = np.asarray(ancestors)
+        self.probs = np.asarray(probabilities, dtype=np.float64)
+        n = self.stages.size
+        if data is not None and not isinstance(data, np.ndarray):
+            raise ValueError("[FromStructure] Data must be a numpy array!")
+        self.data = np.atleast_2d(data) if data is not None else np.zeros((n, 1))
+        self.dt = dt
+        if n != self.ancestors.size or n != self.probs.size:
+            raise ValueError("[FromStructure] Arguments have differing sizes!")
+        if self.stages[0] != 0:
+            raise ValueError("[FromStructure] No root node!")
+        if self.ancestors[0] != -1:
+            raise ValueError("[FromStructure] Ancestor of root node must be (-1)!")
+        for stage in range(int(self.stages.max())):
+            if not np.isclose(self.probs[self.stages == stage].sum(), 1.0):
+                raise ValueError(f"[FromStructure] Probabilities of nodes at stage ({stage}) do not sum to (1.0)!")
+        if self.data.shape[0] != n:
+            self.data = self.data.T
+        if self.data.shape[0] != n:
+            raise ValueError("[FromStructure] Data does not fit tree!")
+
+    def build(self, folder=None):
+        tree = Tree(self.stages, self.ancestors, self.probs, dt=self.dt)
+        if folder is not None:
+            tree.write_files(folder)
+        return tree

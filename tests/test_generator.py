@@ -173,3 +173,23 @@ def test_workload_sizes(tmp_path):
     configs.generate("small", d)
     st = btio.read_flat(os.path.join(d, "stages_u.bt"))
     assert len(st) == 31 and int((st < st.max()).sum()) == 15
+
+
+def test_from_structure_matches_the_factory_tree(tmp_path):
+    """FromStructure (api/python/spock/tree.py:902-951) rebuilds a Markov tree from its stage,
+    ancestor and probability arrays: same tree files apart from the events (all 0 there)."""
+    from spock_amd.api import btio
+    from spock_amd.api.tree import FromStructure
+    t = MarkovChain(np.array([[0.3, 0.7], [0.4, 0.6]]), np.array([0.5, 0.5]), 3, 2).build()
+    a, b = tmp_path / "a", tmp_path / "b"
+    t.write_files(str(a))
+    FromStructure(t.stages_arr, t.ancestors_arr, t.prob).build(str(b))
+    for name in ("stages_u", "ancestors_u", "childrenFrom_u", "childrenTo_u", "numChildren_u", "stageFrom_u",
+                 "stageTo_u", "probabilities_d", "conditionalProbabilities_d"):
+        assert np.array_equal(btio.read_flat(str(a / f"{name}.bt")), btio.read_flat(str(b / f"{name}.bt")))
+    ev = btio.read_flat(str(b / "events_u.bt")).astype(np.int64)
+    assert ev[0] == -1 and np.all(ev[1:] == 0)
+    with pytest.raises(ValueError, match="do not sum"):
+        FromStructure([0, 1, 1, 2, 2], [-1, 0, 0, 1, 2], [1.0, 0.5, 0.4, 0.5, 0.4]).build()
+    with pytest.raises(ValueError, match="No root"):
+        FromStructure([1, 1], [-1, 0], [1.0, 1.0]).build()
