@@ -164,7 +164,8 @@ def test_avar_data_gives_the_textbook_risk(alpha):
 
 
 def test_workload_sizes(tmp_path):
-    """The 229-node cfg2 tree (3 events, horizon 10, branching stops at stage 3) and the 31-node cfg1."""
+    """The 229-node tree (3 events, horizon 10, branching stops at stage 3) and the 31-node tree
+    (2 events, horizon 4)."""
     d = str(tmp_path / "mid")
     configs.generate("mid", d)
     st = btio.read_flat(os.path.join(d, "stages_u.bt"))

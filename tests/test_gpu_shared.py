@@ -184,7 +184,7 @@ def test_shared_fp32_bulk_and_fp64_polish(ctx, problem_dirs):
     z = b64.iterates().reshape(count, -1)
     assert sum(r["status"] == 0 for r in r32) >= count - 2
     for k in range(count):
-        if r32[k]["status"] != 0:  # an instance the FP32 bulk did not finish is not judged here
+        if r32[k]["status"] != 0:  # an instance the FP32 bulk did not finish is not checked here
             continue
         assert r64[k]["status"] == 0
         assert r64[k]["tol"] == pytest.approx(r32[k]["tol"], rel=1e-6)
