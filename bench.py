@@ -111,17 +111,24 @@ class Pipeline:
         t = time.perf_counter()
         launches = [0]
         if self.a.graph:
+            # graphs of at most `graph` chunk launches each, with a report after every graph
             o = self.opts(chunk_iters=self.a.chunk, **kw)
-            n_launch = max(1, -(-o.max_iters // self.a.chunk) + 1)
-            b.solve_graph(o, warm, n_launch)
+            total = max(1, -(-o.max_iters // self.a.chunk) + 1)
+            per = max(1, min(self.a.graph, total))
+            b.solve_graph(o, warm, per)
             res, sec = b.results()
-            while any(r["status"] == -1 for r in res):  # not reached: the graph covers max_iters
-                b.resume()
+            n_launch, graphs = per, 1
+            while any(r["status"] == -1 for r in res):
+                log(f"  {label}: {sum(r['status'] != -1 for r in res)}/{len(res)} finished, "
+                    f"{time.perf_counter() - t:.1f} s")
+                more = max(1, min(per, total - n_launch))
+                b.resume_graph(more)
                 res, s2 = b.results()
                 sec += s2
-                n_launch += 1
+                n_launch += more
+                graphs += 1
             log(f"  {label}: {sum(r['status'] != -1 for r in res)}/{len(res)} finished, "
-                f"{time.perf_counter() - t:.1f} s (one graph of {n_launch} launches)")
+                f"{time.perf_counter() - t:.1f} s ({graphs} graph(s), {n_launch} launches)")
             self.kernel_s.append((b.dtype, sec, sum(r["num_T"] for r in res), sum(r["num_ell"] for r in res),
                                   n_launch, label))
             return res
@@ -172,9 +179,11 @@ class Pipeline:
         return r32, r64, result
 
 
-def plain_single(rt, ctx, prob, x0s, tol, max_iters):
-    """Plain FP64 SPOCK, one instance per launch, cold start (per-instance engine)."""
+def plain_single(rt, ctx, prob, x0s, tol, max_iters, workgroups=1):
+    """Plain FP64 SPOCK, one instance per launch, cold start (per-instance engine; workgroups > 1:
+    grid mode, the instance spread over that many cooperating workgroups)."""
     b = rt.Batch(ctx, [prob], rt.F64)
+    b.set_workgroups(workgroups)
     out = []
     for k, x0 in enumerate(x0s):
         b.reset()
@@ -184,7 +193,8 @@ def plain_single(rt, ctx, prob, x0s, tol, max_iters):
         res, _ = b.results()
         dt = time.perf_counter() - t
         out.append((dt, res[0]["iters"], res[0]["status"], s0_of(prob, b.iterate(0)[None, :])[0]))
-        log(f"plain FP64 single x0#{k}: {dt:.2f} s, {res[0]['iters']} iterations, status {res[0]['status']}")
+        log(f"plain FP64 single x0#{k} ({workgroups} workgroups): {dt:.2f} s, {res[0]['iters']} iterations, "
+            f"status {res[0]['status']}")
     return out
 
 
@@ -301,6 +311,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=400, help="outer iterations per launch of a chunked solve")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold starts)")
     ap.add_argument("--plain", type=int, default=1, help="instances solved by the plain single-instance path")
+    ap.add_argument("--plain-workgroups", type=int, default=64,
+                    help="workgroups of the grid-mode variant of the plain single-instance solve")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--max-iters", type=int, default=8000, help="iteration cap of plain cold solves")
     ap.add_argument("--warm-iters", type=int, default=800, help="FP32 iteration cap from the warm start")
@@ -312,7 +324,8 @@ def main():
                          "more polish iterations)")
     ap.add_argument("--fallback", action="store_true", help="solve what the warm path leaves with cold solves")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--graph", type=int, default=1, help="1: each chunked solve of the step is one HIP graph")
+    ap.add_argument("--graph", type=int, default=3,
+                    help="chunk launches per HIP graph of a solve in the step (0: launch by launch)")
     ap.add_argument("--tune-iters", type=int, default=50, help="outer iterations per schedule the autotuner times")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -332,10 +345,17 @@ def main():
 
         plain = None
         if args.plain > 0:
-            ps = plain_single(rt, ctx, prob, x0s[:args.plain], args.tol, 20000)
-            secs = [p[0] for p in ps]
-            plain = {"solves_per_s": len(secs) / sum(secs), "seconds": secs, "iters": [p[1] for p in ps],
-                     "status": [p[2] for p in ps]}
+            # the plain method at its fastest single-instance launch shape: one workgroup, or the
+            # instance spread over 64 workgroups (grid mode)
+            variants = {}
+            for wgs in (1, args.plain_workgroups):
+                ps = plain_single(rt, ctx, prob, x0s[:args.plain], args.tol, 20000, wgs)
+                secs = [p[0] for p in ps]
+                variants[wgs] = {"solves_per_s": len(secs) / sum(secs), "seconds": secs,
+                                 "iters": [p[1] for p in ps], "status": [p[2] for p in ps], "workgroups": wgs}
+            best = max(variants.values(), key=lambda v: v["solves_per_s"])
+            plain = dict(best)
+            plain["variants"] = list(variants.values())
 
         mlp, n_train, train_loss = None, 0, None
         if not args.cold:

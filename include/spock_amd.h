@@ -172,6 +172,9 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
  * report SPK_NOT_RUN for unfinished instances and spk_batch_resume continues.  No counterpart in the
  * reference. */
 int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int launches);
+/* continue a chunked solve of a shared batch with `launches` resume launches captured as one graph
+ * (as spk_batch_resume, without host round trips between the launches). */
+int spk_batch_resume_graph(spk_batch *b, int launches);
 /* wait for the last solve; fills count results and the kernel time in seconds (nullable) */
 int spk_batch_results(spk_batch *b, spk_result *out, double *kernel_seconds);
 /* single operator on every instance: in/out are host arrays concatenated over instances

@@ -493,13 +493,13 @@ void launchLaneSolve(spk_batch *b, const spk_options &o, int warm, int resume = 
 // A chunked solve of a shared batch as one HIP graph: the first launch and launches - 1 resumes,
 // with no host round trip between them (workgroups whose solve has ended return at once).
 template <typename T>
-void launchLaneGraph(spk_batch *b, const spk_options &o, int warm, int launches) {
+void launchLaneGraph(spk_batch *b, const spk_options &o, int warm, int launches, bool resumeOnly = false) {
     hipStream_t st = b->ctx->stream;
     buildLaneInst<T>(b, o, warm, 1);
     b->dInstResume.alloc(sizeof(spk::LaneInst<T>));
     check(hipMemcpyAsync(b->dInstResume.p, b->dInst.p, sizeof(spk::LaneInst<T>), hipMemcpyDeviceToDevice, st),
           "resume instance");
-    buildLaneInst<T>(b, o, warm, 0);
+    if (!resumeOnly) buildLaneInst<T>(b, o, warm, 0);
     if (b->dOut.bytes != sizeof(spk::Out) * b->count) b->dOut.alloc(sizeof(spk::Out) * b->count);
     check(hipStreamSynchronize(st), "sync before capture");
     if (b->graphExec) check(hipGraphExecDestroy(b->graphExec), "graph exec destroy");
@@ -507,8 +507,8 @@ void launchLaneGraph(spk_batch *b, const spk_options &o, int warm, int launches)
     b->graphExec = nullptr;
     b->graph = nullptr;
     check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
-    launchLaneKernel<T>(b, o);
-    for (int i = 1; i < launches; i++) launchLaneKernel<T>(b, o, b->dInstResume.p);
+    if (!resumeOnly) launchLaneKernel<T>(b, o);
+    for (int i = resumeOnly ? 0 : 1; i < launches; i++) launchLaneKernel<T>(b, o, b->dInstResume.p);
     check(hipStreamEndCapture(st, &b->graph), "end capture");
     check(hipGraphInstantiate(&b->graphExec, b->graph, nullptr, nullptr, 0), "graph instantiate");
     b->lastOpts = o;
@@ -1137,6 +1137,19 @@ int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int l
         check(hipSetDevice(b->ctx->device), "hipSetDevice");
         if (b->dtype == SPK_F64) launchLaneGraph<double>(b, *opts, warm, launches);
         else launchLaneGraph<float>(b, *opts, warm, launches);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_resume_graph(spk_batch *b, int launches) {
+    try {
+        if (!b->lanes || !b->resumable) throw std::runtime_error("no chunked solve of a shared batch to resume");
+        if (launches < 1) throw std::runtime_error("launches must be >= 1");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        if (b->dtype == SPK_F64) launchLaneGraph<double>(b, b->lastOpts, b->lastWarm, launches, true);
+        else launchLaneGraph<float>(b, b->lastOpts, b->lastWarm, launches, true);
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);

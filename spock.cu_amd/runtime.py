@@ -20,7 +20,7 @@ STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
 EXPORTS = [
     "spk_default_options", "spk_ctx_create", "spk_ctx_destroy", "spk_last_error", "spk_ctx_stream",
     "spk_ctx_synchronize", "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
-    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_set_schedule", "spk_batch_autotune", "spk_batch_solve_graph", "spk_batch_reset",
+    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_set_schedule", "spk_batch_autotune", "spk_batch_solve_graph", "spk_batch_resume_graph", "spk_batch_reset",
     "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
     "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve", "spk_batch_resume",
     "spk_batch_results", "spk_batch_debug_op", "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
@@ -91,6 +91,7 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_set_workgroups.argtypes = [vp, C.c_int]
     lib.spk_batch_set_schedule.argtypes = [vp, C.c_int, C.c_int]
     lib.spk_batch_solve_graph.argtypes = [vp, vp, C.c_int, C.c_int]
+    lib.spk_batch_resume_graph.argtypes = [vp, C.c_int]
     lib.spk_batch_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
     lib.spk_batch_reset.argtypes = [vp]
     lib.spk_batch_set_x0.argtypes = [vp, _dp]
@@ -301,6 +302,10 @@ class Batch:
             launches = max(1, -(-max(1, opts.max_iters) // max(1, opts.chunk_iters)) + 1)
         self.ctx.check(self.ctx.lib.spk_batch_solve_graph(self.h, C.byref(opts), 1 if warm else 0, int(launches)),
                        "solve_graph")
+
+    def resume_graph(self, launches):
+        """Continue a chunked solve with `launches` resume launches captured as one HIP graph."""
+        self.ctx.check(self.ctx.lib.spk_batch_resume_graph(self.h, int(launches)), "resume_graph")
 
     def solve_chunked(self, opts, warm=False, on_chunk=None):
         """Run a chunked solve to its end; on_chunk(results, kernel_seconds) after every chunk.

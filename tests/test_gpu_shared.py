@@ -299,6 +299,8 @@ def test_graph_captured_chunked_solve_equals_uninterrupted(ctx, problem_dirs, dt
     g.solve_graph(rt.options(chunk_iters=50, **opts), launches=3)
     res, _ = g.results()
     assert any(r["status"] == -1 for r in res)
+    g.resume_graph(2)  # two more chunks as one graph, then launch by launch
+    res, _ = g.results()
     while any(r["status"] == -1 for r in res):
         g.resume()
         res, _ = g.results()
