@@ -35,3 +35,40 @@ def test_default_options_are_the_reference_builder_defaults():
     o = rt.options()
     assert (o.max_inner, o.anderson_buffer, o.check_period, o.method) == (8, 3, 25, rt.SPOCK)
     assert o.tol_abs == pytest.approx(1e-3) and o.tol_rel == 0.0 and o.max_iters == 0
+
+
+def _struct_fields(name):
+    """(field, C type) of `typedef struct name {...} name;` in the header, arrays as 'type[n]'."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, flags=re.S).group(1)
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(double|long|int)\s+(.*)", decl)
+        ctype, names = m.group(1), m.group(2)
+        for nm in names.split(","):
+            nm = nm.strip()
+            arr = re.match(r"(\w+)\[(\d+)\]", nm)
+            out.append((arr.group(1), f"{ctype}[{arr.group(2)}]") if arr else (nm, ctype))
+    return out
+
+
+_CTYPES = {C.c_double: "double", C.c_long: "long", C.c_int: "int"}
+
+
+def _binding_fields(cls):
+    out = []
+    for nm, t in cls._fields_:
+        if hasattr(t, "_length_"):
+            out.append((nm, f"{_CTYPES[t._type_]}[{t._length_}]"))
+        else:
+            out.append((nm, _CTYPES[t]))
+    return out
+
+
+@pytest.mark.parametrize("cname,cls", [("spk_options", rt.Options), ("spk_result", rt.Result)])
+def test_binding_structs_match_the_header(cname, cls):
+    """The ctypes mirrors of spk_options / spk_result have the header's fields, order and types."""
+    assert _binding_fields(cls) == _struct_fields(cname)
