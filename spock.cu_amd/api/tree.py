@@ -254,3 +254,110 @@ class FromStructure:
         if folder is not None:
             tree.write_files(folder)
         return tree
+
+
+class FromData:
+    """Tree from sampled data by forward scenario reduction (``api/python/spock/tree.py:608-899``).
+
+    ``samples_dim_time`` is [samples x dimension x time]; ``branching_per_stage[k]`` caps the
+    children of every node at stage k.  Each node's cluster of samples is reduced at the next
+    stage to at most that many kept samples by greedy forward selection under the
+    probability-weighted l2 transport cost; every sample then joins its closest kept one, which
+    takes over the probability.  Samples start equally likely.  The trial set of a candidate
+    follows the reference literally: it is every sample except those still eliminated, where the
+    candidate's position in the eliminated list stands in as its label, so trees built from the
+    same data match the reference's.  Nodes are then numbered stage by stage along the scenarios
+    in lexicographic order.  Events are all 0 below the root (no per-event data)."""
+
+    def __init__(self, samples_dim_time, branching_per_stage, dt="d", sw_scaling=False):
+        data = np.asarray(samples_dim_time, dtype=np.float64)
+        if data.ndim != 3:
+            raise ValueError("[FromData] Data provided is not a 3D array.")
+        self.data = data
+        self.branching = np.asarray(branching_per_stage, dtype=np.int64)
+        self.dt = dt
+        self.sw_scaling = sw_scaling
+
+    @staticmethod
+    def _reduce(values, keep, probs):
+        """Greedy forward selection of `keep` of the rows of `values`; returns (probabilities of
+        all rows after redistribution, kept row indices, closest kept row of every row)."""
+        n = values.shape[0]
+        if keep > n:
+            raise ValueError("[FromData] Number of desired scenarios too large.")
+        dist = np.linalg.norm(values[:, None, :] - values[None, :, :], axis=2)
+        everyone = set(range(n))
+        remaining = list(range(n))
+        kept = []
+        for _ in range(keep):
+            best, best_pos = np.inf, 0
+            for pos in range(len(remaining)):
+                trial = sorted(everyone - (set(remaining) - {pos}))
+                cost = 0.0
+                for jpos, j in enumerate(remaining):
+                    if jpos != pos:
+                        cost += probs[j] * dist[j, trial].min()
+                if cost < best:
+                    best, best_pos = cost, pos
+            kept.append(remaining.pop(best_pos))
+        kept = np.asarray(kept, dtype=np.int64)
+        closest = kept[np.argmin(dist[:, kept], axis=1)]
+        new_probs = np.zeros(n)
+        for k in kept:
+            new_probs[k] = probs[closest == k].sum()
+        return new_probs, kept, closest
+
+    def build(self, folder=None):
+        data = self.data
+        n_samples, n_vars, n_time = data.shape
+        horizon = min(self.branching.size, n_time)
+        scaled = data
+        if self.sw_scaling:
+            scaled = np.empty_like(data)
+            for v in range(n_vars):
+                x = data[:, v, :]
+                scaled[:, v, :] = (x - x.mean()) / x.std()
+        stage, anc, prob = [0], [-1], [1.0]
+        cluster = [np.arange(n_samples)]
+        leaves = [0]
+        while any(stage[lf] < horizon for lf in leaves):
+            for lf in list(leaves):
+                k = stage[lf]
+                if k >= horizon:
+                    continue
+                leaves.remove(lf)
+                members = cluster[lf]
+                p = np.full(members.size, 1.0 / members.size)
+                if members.size > self.branching[k]:
+                    rp, kept, closest = self._reduce(scaled[members, :, k], int(self.branching[k]), p)
+                else:
+                    rp, kept, closest = p, np.arange(members.size), np.arange(members.size)
+                for kk in kept:
+                    node = len(stage)
+                    stage.append(k + 1)
+                    anc.append(lf)
+                    prob.append(prob[lf] * rp[kk])
+                    cluster.append(members[closest == kk])
+                    leaves.append(node)
+        stage, anc, prob = np.asarray(stage), np.asarray(anc), np.asarray(prob)
+        # number the nodes stage by stage along the scenarios sorted lexicographically by path
+        paths = []
+        for lf in leaves:
+            path = [lf]
+            while anc[path[-1]] >= 0:
+                path.append(anc[path[-1]])
+            paths.append(path[::-1])
+        paths.sort()
+        order, seen = [], set()
+        for col in range(int(stage.max()) + 1):
+            for path in paths:
+                if col < len(path) and path[col] not in seen:
+                    seen.add(path[col])
+                    order.append(path[col])
+        relabel = {old: new for new, old in enumerate(order)}
+        order = np.asarray(order)
+        new_anc = np.array([-1] + [relabel[a] for a in anc[order][1:]])
+        tree = Tree(stage[order], new_anc, prob[order], dt=self.dt)
+        if folder is not None:
+            tree.write_files(folder)
+        return tree

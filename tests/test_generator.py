@@ -194,3 +194,34 @@ def test_from_structure_matches_the_factory_tree(tmp_path):
         FromStructure([0, 1, 1, 2, 2], [-1, 0, 0, 1, 2], [1.0, 0.5, 0.4, 0.5, 0.4]).build()
     with pytest.raises(ValueError, match="No root"):
         FromStructure([1, 1], [-1, 0], [1.0, 1.0]).build()
+
+
+def test_from_data_known_answers():
+    """FromData on the reference's test data (tests/python/testTreeFactory.py:30-37): 10 samples of
+    a 1-D variable over 4 stages, branching [3, 2, 1, 1].  Known answers from :404-512."""
+    from spock_amd.api.tree import FromData
+    col = 0.1 * np.arange(10.0).reshape(10, 1)
+    data = np.dstack((0.1 * col, 0.2 * col, 0.3 * col, 0.4 * col))
+    t = FromData(data, [3, 2, 1, 1]).build()
+    assert (t.num_nodes, t.num_nonleaf_nodes, t.num_stages) == (22, 16, 5)
+    assert list(t.ancestors_arr[:10]) == [-1, 0, 0, 0, 1, 1, 2, 2, 3, 3]
+    assert all(t.ancestors_arr[i] == i - 6 for i in range(10, 22))
+    assert list(t.num_children) == [3, 2, 2, 2] + [1] * 12
+    for s, (lo, hi) in enumerate([(0, 1), (1, 4), (4, 10), (10, 16), (16, 22)]):
+        assert list(t.nodes_of_stage(s)) == list(range(lo, hi))
+    np.testing.assert_allclose(t.prob[:10], [1.0, 0.4, 0.3, 0.3, 0.3, 0.1, 0.2, 0.1, 0.2, 0.1], atol=1e-10)
+    for i in range(10, 22):
+        assert abs(t.prob[i] - t.prob[t.ancestors_arr[i]]) < 1e-10
+    assert np.all(t.events_arr[1:] == 0)
+
+
+def test_from_data_conditional_probabilities_sum_to_one():
+    """tests/python/testTreeFactory.py:520-531 on a seeded draw: 10 samples over 20 stages,
+    branching 4 for 5 stages then 1."""
+    from spock_amd.api.tree import FromData
+    data = np.random.default_rng(3).random((10, 1, 20))
+    branching = np.ones(20)
+    branching[:5] *= 4
+    t = FromData(data, branching).build()
+    for i in range(t.num_nonleaf_nodes):
+        assert abs(t.cond_prob[t.children[i]].sum() - 1.0) < 1e-10
