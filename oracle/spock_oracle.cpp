@@ -9,7 +9,9 @@
 // zero too (src/costs.cuh:136,179-183), so iterates agree.
 //
 // Parity pins for each function live in tests/test_oracle.py (dense operator
-// matrix, KKT solve, closed-form projectors, textbook cone projections, lstsq).
+// matrix, KKT solve, closed-form projectors, textbook cone projections, lstsq); the
+// whole iteration is pinned there by an independent solve of the same conic program
+// (scipy SLSQP on the generator's own operator) reaching the oracle's optimal cost.
 #include <algorithm>
 #include <chrono>
 #include <cmath>

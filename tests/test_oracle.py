@@ -212,3 +212,135 @@ def test_golden_vectors_reproduce(tmp_path):
     r = o.solve(g["x0"], tol_abs=1e-30, max_iters=75)
     assert rel_err(r["z"], g["spock75"]) < 1e-12
     assert [r[k] for k in ("k0", "k1", "k2", "k2bt", "k3")] == list(g["spock75_counts"])
+
+
+@pytest.mark.parametrize("name", ["small", "poly"])
+def test_converged_solution_solves_the_conic_program(tmp_path, name):
+    """An independent optimality check of the whole iteration (CP step, dual prox with its
+    translations, SuperMann and Anderson).  The RAOCP in conic form is built from the generator's
+    own L (``spock_amd.api.problem``, a separate implementation from the oracle's) and the problem
+    data:
+      min s_0  s.t.  the dynamics with x_0 fixed, (I - NN'_i) [y_i; t_ch(i); s_ch(i)] = 0 for every
+                     nonleaf i, and L z + c in C,
+    with C the product of the AV@R cone (the first 2 #children entries of every y block >= 0), the
+    nonnegative orthant, the boxes (the root's state bounds relaxed, src/constraints.cuh:142-148)
+    and the second-order cones of the costs, c the SOC translations.  The oracle's tightly converged
+    solution must be feasible for it, and SLSQP started there must not find a lower cost."""
+    from scipy.optimize import minimize
+    d = str(tmp_path / name)
+    gen = configs.generate(name, d)
+    o = Oracle(d)
+    x0 = configs.initial_states(1, o.nx, 1, "uniform")[0] if name == "small" else \
+        configs.initial_states(33, o.nx, 1, "small")[0]
+    # SPOCK levels off near an error of 1e-5 on these problems; 20,000 iterations reach it
+    ref = o.solve(x0, tol_abs=1e-7, tol_rel=0.0, max_iters=20000)
+    assert ref["err"] < 1e-4
+    zp = ref["z"][:o.n_prim].copy()
+    info = o.info
+    N, nl, nx, nu, E = info["numNodes"], info["numNonleafNodes"], o.nx, o.nu, info["numEvents"]
+    yD = 2 * E + 1
+    nlf = N - nl
+    Lm = np.column_stack([gen.op(np.eye(o.n_prim)[k]) for k in range(o.n_prim)])
+    # translations and block offsets of the dual
+    c = np.zeros(o.n_dual)
+    c3 = 0 if gen.ncon.kind == "no" else gen.ncon.dim_per_node
+    c5 = 0 if gen.lcon.kind == "no" else gen.lcon.dim_per_node
+    oI, oII = 0, nl * yD
+    oIII = oII + nl
+    oIV = oIII + c3 * nl
+    oV = oIV + (nx + nu + 2) * N
+    oVI = oV + c5 * nlf
+    d4, d6 = nx + nu + 2, nx + 2
+    for j in range(1, N):
+        c[oIV + j * d4:oIV + (j + 1) * d4] = gen.ncost.translation[j].ravel()
+    for l in range(nlf):
+        c[oVI + l * d6:oVI + (l + 1) * d6] = gen.lcost.translation[-1].ravel()
+    numch = gen.tree.num_children
+    # linear inequalities G z + h >= 0 from the orthant and box parts of the cone
+    rows, hs = [], []
+    for i in range(nl):
+        for k in range(2 * numch[i]):
+            rows.append(Lm[oI + i * yD + k]); hs.append(c[oI + i * yD + k])
+        rows.append(Lm[oII + i]); hs.append(c[oII + i])
+
+    def boxes(off, count, dim, lb, ub, skip_root_state):
+        for i in range(count):
+            for k in range(dim):
+                if skip_root_state and i == 0 and k < nx:
+                    continue
+                r = Lm[off + i * dim + k]
+                if np.isfinite(lb[k]):
+                    rows.append(r); hs.append(-lb[k])
+                if np.isfinite(ub[k]):
+                    rows.append(-r); hs.append(ub[k])
+    if gen.ncon.kind != "no":
+        boxes(oIII, nl, c3, gen.ncon.lower_bound.ravel(), gen.ncon.upper_bound.ravel(), gen.ncon.kind == "rectangle")
+    if gen.lcon.kind != "no":
+        boxes(oV, nlf, c5, gen.lcon.lower_bound.ravel(), gen.lcon.upper_bound.ravel(), False)
+    G, h = np.array(rows), np.array(hs)
+    socs = [(oIV + j * d4, d4) for j in range(1, N)] + [(oVI + l * d6, d6) for l in range(nlf)]
+
+    def soc_val(z):
+        w = Lm @ z + c
+        return np.array([w[a + n - 1] - np.sqrt(w[a:a + n - 1] @ w[a:a + n - 1] + 1e-18) for a, n in socs])
+
+    def soc_jac(z):
+        w = Lm @ z + c
+        out = []
+        for a, n in socs:
+            v = w[a:a + n - 1]
+            nv = np.sqrt(v @ v + 1e-18)
+            out.append(Lm[a + n - 1] - (v / nv) @ Lm[a:a + n - 1])
+        return np.array(out)
+    # equalities: dynamics with x_0 = x0 and the risk kernel constraints
+    AB = btio.read_bt(os.path.join(d, "dynamics_AB_d.bt")).transpose(2, 0, 1)
+    anc = gen.tree.ancestors_arr
+    oU, oX = 0, nu * nl
+    oY = oX + nx * N
+    oT = oY + yD * nl
+    oS = oT + N
+    eq, beq = [], []
+    x0s = o.scaled_x0(x0)
+    for k in range(nx):
+        r = np.zeros(o.n_prim); r[oX + k] = 1.0; eq.append(r); beq.append(x0s[k])
+    for j in range(1, N):
+        a = anc[j]
+        for k in range(nx):
+            r = np.zeros(o.n_prim)
+            r[oX + j * nx + k] = 1.0
+            r[oX + a * nx:oX + (a + 1) * nx] -= AB[j][k, :nx]
+            r[oU + a * nu:oU + (a + 1) * nu] -= AB[j][k, nx:]
+            eq.append(r); beq.append(0.0)
+    NN = btio.read_bt(os.path.join(d, "risk_NNtr_d.bt")).transpose(2, 0, 1)
+    nD = 4 * E + 1
+    for i in range(nl):
+        sel = np.zeros((nD, o.n_prim))
+        for k in range(yD):
+            sel[k, oY + i * yD + k] = 1.0
+        for m, ch in enumerate(gen.tree.children[i]):
+            sel[yD + m, oT + ch] = 1.0
+            sel[yD + E + m, oS + ch] = 1.0
+        for r in (np.eye(nD) - NN[i]) @ sel:
+            eq.append(r); beq.append(0.0)
+    Aeq, beq = np.array(eq), np.array(beq)
+    # the kernel rows are dependent (I - NN' is a projector): keep an orthonormal row basis
+    U, sv, Vt = np.linalg.svd(Aeq, full_matrices=False)
+    rank = int(np.sum(sv > 1e-10 * sv[0]))
+    Aeq, beq = Vt[:rank], (U[:, :rank].T @ beq) / sv[:rank]
+    scale = max(1.0, np.abs(zp).max())
+    # the oracle's solution is feasible (to its residual) ...
+    viol = (np.abs(Aeq @ zp - beq).max(), -(G @ zp + h).min(), -soc_val(zp).min())
+    print(f"{name}: s0 {zp[oS]:.8f}, violations eq {viol[0]:.2e} lin {viol[1]:.2e} soc {viol[2]:.2e}")
+    assert viol[0] < 1e-9 * scale and viol[1] < 1e-4 * scale and viol[2] < 1e-4 * scale
+    # ... and SLSQP, started away from it, finds the same optimal cost
+    cons = [{"type": "eq", "fun": lambda z: Aeq @ z - beq, "jac": lambda z: Aeq},
+            {"type": "ineq", "fun": lambda z: G @ z + h, "jac": lambda z: G},
+            {"type": "ineq", "fun": soc_val, "jac": soc_jac}]
+    e_s0 = np.zeros(o.n_prim); e_s0[oS] = 1.0
+    start = zp + 0.05 * scale * np.random.default_rng(4).standard_normal(o.n_prim)
+    res = minimize(lambda z: z[oS], start, jac=lambda z: e_s0, constraints=cons, method="SLSQP",
+                   options={"maxiter": 1000, "ftol": 1e-12})
+    s_star = zp[oS]
+    print(f"{name}: SLSQP s0 {res.fun:.8f} after {res.nit} iterations ({res.message})")
+    assert res.success
+    assert abs(res.fun - s_star) <= 1e-4 * max(1.0, abs(s_star))
