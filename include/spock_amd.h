@@ -109,8 +109,8 @@ int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int 
  * lanes are the 64 instances of one block and the factor tables are read once per wave.  Every spk_batch_* call below accepts such a batch; the
  * Anderson least squares is the FP64 Gram solve (ls_method is ignored), the dynamics sweep
  * applies the precomputed (L L')^-1 (dyn_inverse is ignored), anderson_buffer <= 4,
- * numInputs <= 32, numEvents <= 7.  block_size is the workgroup size (64..1024, default
- * 1024 when <= 0); one workgroup serves 64 instances.  No counterpart in the reference, which
+ * numInputs <= 32, numEvents <= 7.  block_size is the workgroup size (64..1024; when <= 0 the
+ * size chosen by spk_batch_autotune, else 1024); one workgroup serves 64 instances.  No counterpart in the reference, which
  * solves one instance per Cache. */
 int spk_batch_create_shared(spk_ctx *ctx, spk_problem *problem, int count, int dtype, spk_batch **out);
 int spk_batch_is_shared(const spk_batch *b);
