@@ -154,10 +154,15 @@ private:
 template <typename T>
 class Cache {
 public:
-    Cache(ScenarioTree<T> &, ProblemData<T> &data, const spk_options &opts) : m_data(&data), m_opts(opts) {
+    // workgroups: how many workgroups solve the instance together (0: 64 for iterates of 50,000
+    // elements or more, where the cooperative grid mode is several times faster, else 1)
+    Cache(ScenarioTree<T> &, ProblemData<T> &data, const spk_options &opts, int workgroups = 0)
+        : m_data(&data), m_opts(opts) {
         spk_problem *p = data.handle();
         detail::check(spk_batch_create(detail::context(), 1, &p, sizeof(T) == 8 ? SPK_F64 : SPK_F32, &m_b),
                       "Cache");
+        if (workgroups == 0) workgroups = data.sizePrimal() + data.sizeDual() >= 50000 ? 64 : 1;
+        detail::check(spk_batch_set_workgroups(m_b, workgroups), "Cache");
     }
     ~Cache() { spk_batch_destroy(m_b); }
     Cache(Cache &&o) noexcept : m_data(o.m_data), m_opts(o.m_opts), m_b(o.m_b), m_exit(o.m_exit), m_iters(o.m_iters),
@@ -271,13 +276,16 @@ public:
     CacheBuilder &maxItersInner(size_t iters) { m_opts.max_inner = (int) iters; return *this; }
     CacheBuilder &andersonBuffer(size_t buffer) { m_opts.anderson_buffer = (int) buffer; return *this; }
     CacheBuilder &enableDebug(bool = true) { return *this; }  // per-iteration traces: not provided
-    Cache<T> build() { return Cache<T>(m_tree, m_data, m_opts); }
-    std::unique_ptr<Cache<T>> make_unique() { return std::make_unique<Cache<T>>(m_tree, m_data, m_opts); }
+    // workgroups per solve (no counterpart in the reference): 0 automatic, 1, or n > 1 (grid mode)
+    CacheBuilder &workgroups(int n) { m_wgs = n; return *this; }
+    Cache<T> build() { return Cache<T>(m_tree, m_data, m_opts, m_wgs); }
+    std::unique_ptr<Cache<T>> make_unique() { return std::make_unique<Cache<T>>(m_tree, m_data, m_opts, m_wgs); }
 
 private:
     ScenarioTree<T> &m_tree;
     ProblemData<T> &m_data;
     spk_options m_opts;
+    int m_wgs = 0;
 };
 
 }  // namespace spock_amd

@@ -121,3 +121,20 @@ def test_drop_in_accessors(tmp_path):
     status, n_prim, n_dual, n_iter, n_states, rel = int(out[0]), int(out[1]), int(out[2]), int(out[3]), int(out[4]), float(out[5])
     assert status == 0 and n_prim + n_dual == n_iter and n_states == 31 * 3
     assert rel < 1e-2
+
+
+@needs_lib
+@pytest.mark.gpu
+def test_reference_driver_on_the_229_node_problem(tmp_path):
+    """The timing driver on the 229-node problem (52,360-element iterate): the drop-in's Cache
+    spreads the single instance over 64 cooperating workgroups (grid mode) and converges."""
+    folder = str(tmp_path / "data")
+    prob = configs.generate("mid", folder)
+    x0 = configs.initial_states(2, 50, 1)[0]
+    prob.tree.write_to_file_fp(folder, "initialState", x0)
+    exe = tmp_path / "random_main"
+    src = open(os.path.join(ROOT, "spock.cu_amd", "examples", "random_main.cpp")).read()
+    _compile(src, exe, tmp_path)
+    run = subprocess.run([str(exe), folder + "/"], cwd=str(tmp_path), capture_output=True, text=True, timeout=170)
+    print(run.stdout)
+    assert run.returncode == 0 and "Saved" in run.stdout
