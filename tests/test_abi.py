@@ -72,3 +72,11 @@ def _binding_fields(cls):
 def test_binding_structs_match_the_header(cname, cls):
     """The ctypes mirrors of spk_options / spk_result have the header's fields, order and types."""
     assert _binding_fields(cls) == _struct_fields(cname)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd") or not os.path.exists(rt.LIB_PATH),
+                    reason="needs the built library on a machine without a GPU")
+def test_no_device_fails_loudly():
+    """Without a HIP device the library reports an error instead of falling back to the CPU."""
+    with pytest.raises(rt.SpockError):
+        rt.Context(0)
