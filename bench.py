@@ -217,7 +217,7 @@ def reference_costs(rt, ctx, prob, x0s, tol, max_iters):
     b64.set_x0(list(x0s))
     b64.copy_iterates_from(b32)
     b64.set_tolerances([r["tol"] for r in res])
-    b64.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=2000, block_size=512), warm=True)
+    b64.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=1000, block_size=512), warm=True)
     res_alt, _ = b64.results()
     return (np.array([r["status"] for r in res]), s0_of(prob, b.iterates().reshape(n, -1)),
             np.array([r["status"] for r in res_alt]), s0_of(prob, b64.iterates().reshape(n, -1)))
@@ -314,7 +314,8 @@ def main():
     ap.add_argument("--plain-workgroups", type=int, default=64,
                     help="workgroups of the grid-mode variant of the plain single-instance solve")
     ap.add_argument("--tol", type=float, default=1e-3)
-    ap.add_argument("--max-iters", type=int, default=8000, help="iteration cap of plain cold solves")
+    ap.add_argument("--max-iters", type=int, default=5000,
+                    help="iteration cap of the cold solves of the cost sample and the fallback")
     ap.add_argument("--warm-iters", type=int, default=800, help="FP32 iteration cap from the warm start")
     ap.add_argument("--polish-iters", type=int, default=200)
     ap.add_argument("--ref-sample", type=int, default=16, help="instances checked against plain FP64 cost")
