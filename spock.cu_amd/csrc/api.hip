@@ -594,7 +594,7 @@ void spk_default_options(spk_options *o) {
     o->anderson_buffer = 3;
     o->check_period = 25;
     o->method = SPK_SPOCK;
-    o->block_size = 256;
+    o->block_size = 0;  // the engine's own default (see spk_options in the header)
     o->ls_method = SPK_LS_HOUSEHOLDER;
     o->dyn_inverse = 1;
     o->chunk_iters = 0;

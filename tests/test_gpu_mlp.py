@@ -171,3 +171,16 @@ def test_warm_start_saves_iterations(ctx, problem_dirs):
     print(f"cost error vs tight solve: warm mean {err_w.mean():.2e} max {err_w.max():.2e}, "
           f"cold mean {err_c.mean():.2e} max {err_c.max():.2e}")
     assert err_w.mean() <= 2.0 * err_c.mean() + 1e-3
+
+
+def test_warm_start_example_runs():
+    """spock.cu_amd/examples/warm_start_batch.py end to end on the 31-node problem (small sizes)."""
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "spock.cu_amd", "examples",
+                        "warm_start_batch.py")
+    spec = importlib.util.spec_from_file_location("warm_start_batch", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    ok, states = mod.main(states=256, train=256)
+    assert ok >= 0.8 * states
