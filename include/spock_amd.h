@@ -176,6 +176,12 @@ int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int l
 /* continue a chunked solve of a shared batch with `launches` resume launches captured as one graph
  * (as spk_batch_resume, without host round trips between the launches). */
 int spk_batch_resume_graph(spk_batch *b, int launches);
+/* per-instance batches of ONE instance: time `iters` cold outer iterations with 1, 8, 32, 64 and
+ * 128 cooperating workgroups (grid mode above 1), keep the fastest for later solves and return it
+ * (workgroups, seconds: nullable; seconds gets the 5 kernel times in that order).  The iterates are
+ * reset (initial states and given tolerances are kept).  No counterpart in the reference. */
+int spk_batch_autotune_workgroups(spk_batch *b, const spk_options *opts, int iters, int *workgroups,
+                                  double *seconds);
 /* wait for the last solve; fills count results and the kernel time in seconds (nullable) */
 int spk_batch_results(spk_batch *b, spk_result *out, double *kernel_seconds);
 /* single operator on every instance: in/out are host arrays concatenated over instances

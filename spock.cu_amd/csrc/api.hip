@@ -1217,6 +1217,44 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
     }
 }
 
+int spk_batch_autotune_workgroups(spk_batch *b, const spk_options *opts, int iters, int *workgroups,
+                                  double *seconds) {
+    try {
+        if (b->lanes || b->count != 1) throw std::runtime_error("the workgroup autotuner tunes one-instance batches");
+        if (iters < 1) throw std::runtime_error("iters must be >= 1");
+        spk_options o;
+        if (opts) o = *opts;
+        else spk_default_options(&o);
+        o.tol_abs = o.tol_rel = 0.;  // a fixed number of iterations
+        o.max_iters = iters;
+        o.max_time_secs = 0.;
+        const std::vector<double> keepTol = b->tolFixed;
+        b->tolFixed.clear();
+        const int cand[5] = {1, 8, 32, 64, 128};
+        double best = 1e300;
+        int bw = 1;
+        for (int i = 0; i < 5; i++) {
+            b->wgs = cand[i];
+            if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
+            if (spk_batch_solve(b, &o, 0)) throw std::runtime_error(b->ctx->err);
+            double sec = 0.;
+            if (spk_batch_results(b, nullptr, &sec)) throw std::runtime_error(b->ctx->err);
+            if (seconds) seconds[i] = sec;
+            if (sec < best) {
+                best = sec;
+                bw = cand[i];
+            }
+        }
+        b->wgs = bw;
+        b->tolFixed = keepTol;
+        if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
+        if (workgroups) *workgroups = bw;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
 int spk_batch_results(spk_batch *b, spk_result *out, double *kernelSeconds) {
     try {
         check(hipStreamSynchronize(b->ctx->stream), "sync after solve");

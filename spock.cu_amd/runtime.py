@@ -20,7 +20,7 @@ STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
 EXPORTS = [
     "spk_default_options", "spk_ctx_create", "spk_ctx_destroy", "spk_last_error", "spk_ctx_stream",
     "spk_ctx_synchronize", "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
-    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_set_schedule", "spk_batch_autotune", "spk_batch_solve_graph", "spk_batch_resume_graph", "spk_batch_reset",
+    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_set_schedule", "spk_batch_autotune", "spk_batch_autotune_workgroups", "spk_batch_solve_graph", "spk_batch_resume_graph", "spk_batch_reset",
     "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
     "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve", "spk_batch_resume",
     "spk_batch_results", "spk_batch_debug_op", "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
@@ -91,6 +91,7 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_set_workgroups.argtypes = [vp, C.c_int]
     lib.spk_batch_set_schedule.argtypes = [vp, C.c_int, C.c_int]
     lib.spk_batch_solve_graph.argtypes = [vp, vp, C.c_int, C.c_int]
+    lib.spk_batch_autotune_workgroups.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), _dp]
     lib.spk_batch_resume_graph.argtypes = [vp, C.c_int]
     lib.spk_batch_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
     lib.spk_batch_reset.argtypes = [vp]
@@ -242,6 +243,16 @@ class Batch:
                                                         _d(sec)), "autotune")
         cands = [(0, 1024), (8, 1024), (0, 512), (8, 512)]
         return mv.value, blk.value, dict(zip(cands, sec.tolist()))
+
+    def autotune_workgroups(self, opts=None, iters=50):
+        """One-instance batch: time 1/8/32/64/128 cooperating workgroups and keep the fastest;
+        returns (workgroups, {workgroups: seconds})."""
+        w = C.c_int()
+        sec = np.zeros(5)
+        o = opts if opts is not None else options()
+        self.ctx.check(self.ctx.lib.spk_batch_autotune_workgroups(self.h, C.byref(o), int(iters), C.byref(w), _d(sec)),
+                       "autotune_workgroups")
+        return w.value, dict(zip([1, 8, 32, 64, 128], sec.tolist()))
 
     def set_workgroups(self, n):
         """Workgroups that solve the single instance together (0 or 1: one workgroup,

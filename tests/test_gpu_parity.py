@@ -259,3 +259,21 @@ def test_uneven_family_fp32_bulk_and_fp64_polish(ctx, uneven_family):
         s = o.n_prim - o.sS
         s_p, s_64 = plain.iterate(i)[s], b64.iterate(i)[s]
         assert abs(s_64 - s_p) <= 2e-2 * max(1.0, abs(s_p))
+
+
+def test_workgroup_autotune_single_instance(ctx, problem_dirs):
+    """The workgroup autotuner of one-instance batches keeps the fastest count, leaves no state
+    behind, and the tuned solve still matches the oracle over fixed iterations."""
+    d = problem_dirs["mid"]
+    o = Oracle(d)
+    b = rt.Batch(ctx, [rt.Problem(ctx, d)], rt.F64)
+    x0 = x0_for("mid", o.nx)
+    b.set_x0([x0])
+    w, secs = b.autotune_workgroups(iters=25)
+    print(f"workgroups {w}: {secs}")
+    assert w in secs and secs[w] == min(secs.values())
+    b.solve(rt.options(tol_abs=1e-30, max_iters=50))
+    res, _ = b.results()
+    ref = o.solve(x0, tol_abs=1e-30, max_iters=50)
+    assert res[0]["iters"] == ref["iters"]
+    assert rel_err(b.iterate(0), ref["z"]) < 1e-9
