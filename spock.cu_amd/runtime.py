@@ -17,13 +17,19 @@ SPOCK, CP = 0, 1
 OP_L, OP_LADJ, OP_DYN, OP_KERNEL, OP_T, OP_ELL, OP_DUAL = range(7)
 STATUS = {-1: "notRun", 0: "converged", 1: "outOfIter", 2: "outOfTime"}
 
+# Every entry point of include/spock_amd.h (checked against the header by tests/test_abi.py)
 EXPORTS = [
     "spk_default_options", "spk_ctx_create", "spk_ctx_destroy", "spk_last_error", "spk_ctx_stream",
-    "spk_ctx_synchronize", "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
-    "spk_problem_step_size", "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy", "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_set_schedule", "spk_batch_autotune", "spk_batch_autotune_workgroups", "spk_batch_solve_graph", "spk_batch_resume_graph", "spk_batch_reset",
-    "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate", "spk_batch_set_iterates",
-    "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates", "spk_batch_solve", "spk_batch_resume",
-    "spk_batch_results", "spk_batch_debug_op", "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
+    "spk_ctx_synchronize",
+    "spk_problem_load", "spk_problem_destroy", "spk_problem_sizes", "spk_problem_scaling",
+    "spk_problem_step_size",
+    "spk_batch_create", "spk_batch_create_shared", "spk_batch_is_shared", "spk_batch_destroy",
+    "spk_batch_count", "spk_batch_set_workgroups", "spk_batch_set_schedule", "spk_batch_autotune",
+    "spk_batch_autotune_workgroups", "spk_batch_solve_graph", "spk_batch_resume_graph", "spk_batch_reset",
+    "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate",
+    "spk_batch_set_iterates", "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates",
+    "spk_batch_solve", "spk_batch_resume", "spk_batch_results", "spk_batch_debug_op",
+    "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
     "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
     "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
     "spk_mlp_loss_backward", "spk_mlp_adam", "spk_mlp_dataset", "spk_mlp_train", "spk_mlp_predict",
