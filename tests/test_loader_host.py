@@ -1,0 +1,104 @@
+"""The problem-folder loader (spock.cu_amd/csrc/loader.cpp, host C++) built on its own with
+AddressSanitizer and UndefinedBehaviorSanitizer: it reads every generated problem family, and
+truncated or inconsistent files end in an error message, never a crash."""
+import os
+import json
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from spock_amd.api import btio
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "spock.cu_amd", "csrc")
+
+MAIN = r'''
+#include <cstdio>
+#include <exception>
+#include "loader.hpp"
+int main(int argc, char **argv) {
+    try {
+        spk::HostProblem p = spk::loadProblem(argv[1]);
+        std::printf("%d %d %d %d %d %ld %ld %d\n", p.N, p.nl, p.nx, p.nu, p.E, p.sPrim, p.sDual, p.affine ? 1 : 0);
+        return 0;
+    } catch (const std::exception &e) {
+        std::printf("error: %s\n", e.what());
+        return 2;
+    }
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def loader_exe(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    d = tmp_path_factory.mktemp("loader")
+    (d / "main.cpp").write_text(MAIN)
+    exe = d / "load"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    f"-I{CSRC}", str(d / "main.cpp"), os.path.join(CSRC, "loader.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=300)
+    return str(exe)
+
+
+def run(exe, folder):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="print_stacktrace=1")
+    return subprocess.run([exe, folder], capture_output=True, text=True, timeout=120, env=env)
+
+
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine"])
+def test_loader_reads_every_family(loader_exe, problem_dirs, name):
+    r = run(loader_exe, problem_dirs[name])
+    assert r.returncode == 0, r.stdout + r.stderr
+    N, nl, nx, nu, E, sp, sd, aff = (int(v) for v in r.stdout.split())
+
+    info = json.load(open(os.path.join(problem_dirs[name], "data.json")))
+    assert (N, nl, nx, nu, E) == (info["numNodes"], info["numNonleafNodes"], info["numStates"],
+                                  info["numInputs"], info["numEvents"])
+    assert aff == int(info["dynamics"]["type"] == "affine")
+
+
+def _copy(problem_dirs, tmp_path, name="small"):
+    dst = tmp_path / "p"
+    shutil.copytree(problem_dirs[name], dst)
+    return dst
+
+
+def test_truncated_tensor_is_an_error(loader_exe, problem_dirs, tmp_path):
+    d = _copy(problem_dirs, tmp_path)
+    f = d / "dynamics_AB_d.bt"
+    raw = f.read_bytes()
+    f.write_bytes(raw[:len(raw) // 2])
+    r = run(loader_exe, str(d))
+    assert r.returncode == 2 and "error" in r.stdout, r.stdout + r.stderr
+
+
+def test_header_only_tensor_is_an_error(loader_exe, problem_dirs, tmp_path):
+    d = _copy(problem_dirs, tmp_path)
+    (d / "risk_b_d.bt").write_bytes((d / "risk_b_d.bt").read_bytes()[:10])
+    r = run(loader_exe, str(d))
+    assert r.returncode == 2 and "error" in r.stdout, r.stdout + r.stderr
+
+
+def test_wrong_sizes_are_an_error(loader_exe, problem_dirs, tmp_path):
+    d = _copy(problem_dirs, tmp_path)
+    btio.write_bt(str(d / "dynamics_K_d.bt"), np.zeros((2, 2, 3)), "d")
+    r = run(loader_exe, str(d))
+    assert r.returncode == 2 and "error" in r.stdout, r.stdout + r.stderr
+
+
+def test_out_of_range_tree_indices_are_an_error(loader_exe, problem_dirs, tmp_path):
+    d = _copy(problem_dirs, tmp_path)
+    anc = btio.read_flat(str(d / "ancestors_u.bt")).astype(np.int64)
+    anc[5] = 10 ** 6
+    btio.write_bt(str(d / "ancestors_u.bt"), anc, "u")
+    r = run(loader_exe, str(d))
+    assert r.returncode == 2 and "error" in r.stdout, r.stdout + r.stderr
+
+
+def test_missing_json_is_an_error(loader_exe, tmp_path):
+    r = run(loader_exe, str(tmp_path))
+    assert r.returncode == 2 and "error" in r.stdout, r.stdout + r.stderr
