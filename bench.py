@@ -219,26 +219,42 @@ def reference_costs(rt, ctx, prob, x0s, tol, max_iters):
     b64.set_tolerances([r["tol"] for r in res])
     b64.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=1000, block_size=512), warm=True)
     res_alt, _ = b64.results()
-    return (np.array([r["status"] for r in res]), s0_of(prob, b.iterates().reshape(n, -1)),
-            np.array([r["status"] for r in res_alt]), s0_of(prob, b64.iterates().reshape(n, -1)))
+    return (np.array([r["status"] for r in res]), b.iterates().reshape(n, -1),
+            np.array([r["status"] for r in res_alt]), b64.iterates().reshape(n, -1))
 
 
-def cost_check(rt, ctx, prob, x0s, args, accepted, s_batch):
-    """Cost s0 of accepted batch solutions against plain FP64 SPOCK on the same sample, with the
+def cost_gaps(prob, gen, z, z_ref, mask):
+    """Relative gaps of two costs of iterates z against z_ref, over the rows in mask: the root
+    epigraph variable s0, and J, the nested AV@R cost of the states and inputs alone."""
+    def rel(a, b):
+        return (np.abs(a - b) / np.maximum(1e-12, np.abs(b)))[mask]
+
+    def J(zz):
+        return np.array([gen.trajectory_cost(row) for row in zz])
+    out = {}
+    for name, r in (("s0", rel(s0_of(prob, z), s0_of(prob, z_ref))), ("J", rel(J(z), J(z_ref)))):
+        out[name] = {"mean": float(r.mean()) if r.size else None, "max": float(r.max()) if r.size else None,
+                     "within_1e-3": int((r <= 1e-3).sum())}
+    return out
+
+
+def cost_check(rt, ctx, prob, gen, x0s, args, accepted, z_batch):
+    """Costs of accepted batch solutions against plain FP64 SPOCK on the same sample, with the
     spread between two accepted solutions of this tolerance beside it."""
-    ref_status, s_ref, alt_status, s_alt = reference_costs(rt, ctx, prob, x0s, args.tol, args.max_iters)
+    ref_status, z_ref, alt_status, z_alt = reference_costs(rt, ctx, prob, x0s, args.tol, args.max_iters)
     both = accepted & (ref_status == 0)
-    rel = np.abs(s_batch - s_ref) / np.maximum(1e-12, np.abs(s_ref))
     alt = (alt_status == 0) & (ref_status == 0)
-    rel_alt = np.abs(s_alt - s_ref) / np.maximum(1e-12, np.abs(s_ref))
-    return {"compared": int(both.sum()), "max_rel_err_s0": float(rel[both].max()) if both.any() else None,
-            "mean_rel_err_s0": float(rel[both].mean()) if both.any() else None,
-            "within_1e-3": int((rel[both] <= 1e-3).sum()),
+    g, ga = cost_gaps(prob, gen, z_batch, z_ref, both), cost_gaps(prob, gen, z_alt, z_ref, alt)
+    return {"compared": int(both.sum()), "max_rel_err_s0": g["s0"]["max"], "mean_rel_err_s0": g["s0"]["mean"],
+            "within_1e-3": g["s0"]["within_1e-3"],
+            "max_rel_err_J": g["J"]["max"], "mean_rel_err_J": g["J"]["mean"], "within_1e-3_J": g["J"]["within_1e-3"],
             "reference": "plain FP64 SPOCK (per-instance GPU engine), same x0, same tolerance",
+            "costs": "s0: the root epigraph variable of the iterate; J: the nested AV@R cost of its states "
+                     "and inputs alone (Problem.trajectory_cost); the two agree at a solution",
             "yardstick_cold_fp32_polish": {
                 "compared": int(alt.sum()),
-                "mean_rel_diff_s0": float(rel_alt[alt].mean()) if alt.any() else None,
-                "max_rel_diff_s0": float(rel_alt[alt].max()) if alt.any() else None,
+                "mean_rel_diff_s0": ga["s0"]["mean"], "max_rel_diff_s0": ga["s0"]["max"],
+                "mean_rel_diff_J": ga["J"]["mean"], "max_rel_diff_J": ga["J"]["max"],
                 "note": "cold FP32 solve + FP64 polish vs plain FP64, both accepted at the same thresholds: "
                         "how far apart two accepted solutions of this tolerance lie"}}
 
@@ -340,7 +356,7 @@ def main():
     rt, configs = _pkg()
     ctx = rt.Context(0)
     with tempfile.TemporaryDirectory() as folder:
-        configs.generate("mid", folder)
+        gen = configs.generate("mid", folder)
         prob = rt.Problem(ctx, folder)
         x0s = configs.initial_states(51, prob.nx, args.batch, "normal")
 
@@ -379,7 +395,7 @@ def main():
         it32 = np.array([r["iters"] for r in r32])
         it64 = np.array([r["iters"] for r in r64])
         ns = min(args.ref_sample, B)
-        s_batch = s0_of(prob, np.array([pipe.b64.iterate(k) for k in range(ns)])) if ns else np.zeros(0)
+        z_batch = np.array([pipe.b64.iterate(k) for k in range(ns)]) if ns else np.zeros((0, prob.n_iter))
         n_fb = 0
         accepted = st64 == 0
         if fallback is not None:
@@ -389,13 +405,13 @@ def main():
                 if rf[i]["status"] == 0:
                     accepted[k] = True
                     if k < ns:
-                        s_batch[k] = s0_of(prob, f64.iterate(i)[None, :])[0]
+                        z_batch[k] = f64.iterate(i)
         log(f"accepted {accepted.sum()}/{B}, fallback {n_fb}")
 
         # accepted solutions against plain FP64 SPOCK (cold, same tolerance) on a sample
         cost = None
         if ns:
-            cost = cost_check(rt, ctx, prob, x0s[:ns], args, accepted[:ns], s_batch)
+            cost = cost_check(rt, ctx, prob, gen, x0s[:ns], args, accepted[:ns], z_batch)
             log("reference sample done")
 
         # roofline of the dominant kernel: k_lane_solve<float>, the FP32 solve from the warm start

@@ -224,3 +224,19 @@ class AVaR:
         E = np.vstack((self.alpha * np.eye(n), -np.eye(n), np.ones((1, n))))
         b = np.vstack((np.asarray(cond_probs).reshape(-1, 1), np.zeros((n, 1)), np.ones((1, 1))))
         return E, b
+
+    def value(self, cond_probs, Z):
+        """AV@R of the outcomes Z: max mu'Z over the set (E, b) describes, 0 <= alpha mu <= pi with
+        1'mu = 1 (the last row's multiplier is free).  Greedy: the largest outcomes take their
+        full weight pi/alpha until the weights sum to one."""
+        pi, Z = np.asarray(cond_probs, dtype=np.float64).ravel(), np.asarray(Z, dtype=np.float64).ravel()
+        if self.alpha == 0.0:
+            return float(Z.max())
+        left, total = 1.0, 0.0
+        for k in np.argsort(-Z, kind="stable"):
+            w = min(pi[k] / self.alpha, left)
+            total += w * Z[k]
+            left -= w
+            if left <= 0.0:
+                break
+        return float(total)

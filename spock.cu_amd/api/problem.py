@@ -38,6 +38,7 @@ class Problem:
         self.ncost, self.lcost = nonleaf_cost, leaf_cost
         self.ncon, self.lcon = nonleaf_con, leaf_con
         self.risk = risks[0]
+        self.risks = list(risks)
         self.E_list, self.b_list = [], []
         for i in range(tree.num_nonleaf_nodes):
             E, b = risks[i].make(tree.cond_prob_of_children_of_node(i))
@@ -209,6 +210,42 @@ class Problem:
         x[nl:] += np.einsum("nij,nj->ni", st["lsQ"], VI[:, :nx]) - 0.5 * st["lq"] * cdl[:, None]
         s[nl:] = 0.5 * cdl
         return np.concatenate([u.ravel(), x.ravel(), y.ravel(), tt, s])
+
+    def trajectory_cost(self, prim):
+        """Risk-averse cost of the states and inputs of a primal iterate, in the scaled variables
+        the solver iterates in.  A node's stage cost is the smallest epigraph value t that puts its
+        cost block [v; h + c; h + d] = L z + translation (h = t/2 - (linear terms)/2) in the
+        second-order cone, t = |v|^2 / (d - c) - (c + d) - 2 h(0): the cost the conic form
+        encodes (``api/python/spock/build.py:207-216``, ``:522-535``, ``:592-604``); for a
+        quadratic cost that is x'Qx + u'Ru, for the quadratic-plus-linear one it works out to
+        x'Qx + u'Ru + 2 (q'x + r'u) + (q'Q^-1 q + r'R^-1 r) / 2.  A nonleaf node's cost-to-go is
+        the AV@R of its children's stage cost plus cost-to-go.  Unlike the iterate's epigraph
+        variable s_0 it depends on x and u alone; the two agree at a solution."""
+        t, nx, nu = self.tree, self.nx, self.nu
+        N, nl, nlf = t.num_nodes, t.num_nonleaf_nodes, t.num_leaf_nodes
+        p = np.asarray(prim, dtype=np.float64).reshape(-1)[:self.size_prim].copy()
+        p[self.size_prim - 2 * N:] = 0.0  # t = s = 0: the cost blocks hold h(0)
+        w = self.op(p)
+        c3 = 0 if self.ncon.kind == "no" else self.ncon.dim_per_node
+        c5 = 0 if self.lcon.kind == "no" else self.lcon.dim_per_node
+        o4 = self.yD * nl + nl + c3 * nl
+        o6 = o4 + (nx + nu + 2) * N + c5 * nlf
+
+        def smallest_epigraph(block):
+            v, a, b = block[:-2], block[-2], block[-1]
+            return (v @ v) / (b - a) - (a + b)
+        stage = np.zeros(N)
+        for j in range(1, N):
+            blk = w[o4 + j * (nx + nu + 2):o4 + (j + 1) * (nx + nu + 2)] + self.ncost.translation[j].ravel()
+            stage[j] = smallest_epigraph(blk)
+        togo = np.zeros(N)
+        for k in range(nlf):
+            tr = self.lcost.translation[min(k, len(self.lcost.translation) - 1)].ravel()
+            togo[nl + k] = smallest_epigraph(w[o6 + k * (nx + 2):o6 + (k + 1) * (nx + 2)] + tr)
+        for i in range(nl - 1, -1, -1):  # children carry larger indices than their parent
+            ch = np.asarray(t.children[i])
+            togo[i] = self.risks[i].value(t.cond_prob_of_children_of_node(i), stage[ch] + togo[ch])
+        return float(togo[0])
 
     @staticmethod
     def _con_op(con, z):

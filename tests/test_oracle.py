@@ -332,6 +332,8 @@ def test_converged_solution_solves_the_conic_program(tmp_path, name):
     viol = (np.abs(Aeq @ zp - beq).max(), -(G @ zp + h).min(), -soc_val(zp).min())
     print(f"{name}: s0 {zp[oS]:.8f}, violations eq {viol[0]:.2e} lin {viol[1]:.2e} soc {viol[2]:.2e}")
     assert viol[0] < 1e-9 * scale and viol[1] < 1e-4 * scale and viol[2] < 1e-4 * scale
+    # the nested AV@R cost of its states and inputs alone is its s_0
+    assert abs(gen.trajectory_cost(zp) - zp[oS]) <= 1e-4 * max(1.0, abs(zp[oS]))
     # ... and SLSQP, started away from it, finds the same optimal cost
     cons = [{"type": "eq", "fun": lambda z: Aeq @ z - beq, "jac": lambda z: Aeq},
             {"type": "ineq", "fun": lambda z: G @ z + h, "jac": lambda z: G},
@@ -344,3 +346,4 @@ def test_converged_solution_solves_the_conic_program(tmp_path, name):
     print(f"{name}: SLSQP s0 {res.fun:.8f} after {res.nit} iterations ({res.message})")
     assert res.success
     assert abs(res.fun - s_star) <= 1e-4 * max(1.0, abs(s_star))
+    assert abs(gen.trajectory_cost(res.x) - res.fun) <= 1e-4 * max(1.0, abs(res.fun))
