@@ -161,12 +161,15 @@ public:
         spk_problem *p = data.handle();
         detail::check(spk_batch_create(detail::context(), 1, &p, sizeof(T) == 8 ? SPK_F64 : SPK_F32, &m_b),
                       "Cache");
-        if (workgroups == 0) workgroups = data.sizePrimal() + data.sizeDual() >= 50000 ? 64 : 1;
+        if (workgroups == 0) {
+            workgroups = data.sizePrimal() + data.sizeDual() >= 50000 ? 64 : 1;
+            m_autoGrid = workgroups > 1;
+        }
         detail::check(spk_batch_set_workgroups(m_b, workgroups), "Cache");
     }
     ~Cache() { spk_batch_destroy(m_b); }
-    Cache(Cache &&o) noexcept : m_data(o.m_data), m_opts(o.m_opts), m_b(o.m_b), m_exit(o.m_exit), m_iters(o.m_iters),
-                                m_time(o.m_time) {
+    Cache(Cache &&o) noexcept : m_data(o.m_data), m_opts(o.m_opts), m_b(o.m_b), m_autoGrid(o.m_autoGrid),
+                                m_exit(o.m_exit), m_iters(o.m_iters), m_time(o.m_time) {
         o.m_b = nullptr;
     }
     Cache(const Cache &) = delete;
@@ -245,7 +248,16 @@ private:
         }
         spk_options o = m_opts;
         o.method = method;
-        detail::check(spk_batch_solve(m_b, &o, prev ? 1 : 0), method == SPK_SPOCK ? "runSpock" : "runCp");
+        int rc = spk_batch_solve(m_b, &o, prev ? 1 : 0);
+        if (rc != 0 && m_autoGrid) {
+            // the automatic grid mode needs a cooperative launch, which the device can refuse (it
+            // did while another live process had made cooperative launches): from then on the
+            // instance is solved by one workgroup, as when it is below the size threshold
+            m_autoGrid = false;
+            detail::check(spk_batch_set_workgroups(m_b, 1), "Cache");
+            rc = spk_batch_solve(m_b, &o, prev ? 1 : 0);
+        }
+        detail::check(rc, method == SPK_SPOCK ? "runSpock" : "runCp");
         spk_result r;
         detail::check(spk_batch_results(m_b, &r, nullptr), "results");
         m_exit = r.status;
@@ -257,6 +269,7 @@ private:
     ProblemData<T> *m_data;
     spk_options m_opts;
     spk_batch *m_b = nullptr;
+    bool m_autoGrid = false;  // grid mode chosen by size, not asked for
     int m_exit = notRun;
     size_t m_iters = 0;
     T m_time = 0;

@@ -349,8 +349,10 @@ void launchMaybeGrid(spk_batch *b, K kernel, int bs, size_t shm, A... args) {
         return;
     }
     void *argv[] = {static_cast<void *>(&args)..., static_cast<void *>(&grid)};
-    check(hipLaunchCooperativeKernel(fn, dim3(g), dim3(bs), argv, (unsigned) shm, b->ctx->stream),
-          "cooperative launch");
+    const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(g), dim3(bs), argv, (unsigned) shm, b->ctx->stream);
+    // a refused launch queues nothing: read the error back so a later launch check does not see it
+    if (e != hipSuccess) (void) hipGetLastError();
+    check(e, "cooperative launch");
 }
 
 template <typename T>

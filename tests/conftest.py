@@ -29,6 +29,13 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 
 
+def pytest_collection_modifyitems(session, config, items):
+    """GPU tests that run the C++ drop-in driver in a process of its own go first, before this
+    process touches the GPU: the driver's 64-workgroup cooperative launch was refused while this
+    process, still alive, had made cooperative launches of its own.  The order is otherwise kept."""
+    items.sort(key=lambda it: 0 if it.get_closest_marker("gpu") and it.path.name == "test_dropin.py" else 1)
+
+
 @pytest.fixture(scope="session")
 def problem_dirs(tmp_path_factory):
     """Generated problem folders shared by the whole session."""
