@@ -158,6 +158,21 @@ def test_avar_data_gives_the_textbook_risk(alpha):
         assert lp.fun == pytest.approx(build.AVaR(alpha).value(pi, Z), abs=1e-9)
 
 
+def test_avar_value_limits():
+    """AV@R_1 is the expectation; AV@R_alpha tends to the maximum as alpha -> 0 (and is the
+    maximum at 0); an outcome with zero probability never carries weight below the maximum."""
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 4, 7):
+        pi = rng.dirichlet(np.ones(n))
+        Z = rng.standard_normal(n)
+        assert build.AVaR(1.0).value(pi, Z) == pytest.approx(float(pi @ Z), abs=1e-12)
+        assert build.AVaR(0.0).value(pi, Z) == Z.max()
+        assert build.AVaR(min(pi.min(), 1e-3)).value(pi, Z) == pytest.approx(Z.max(), abs=1e-12)
+        vals = [build.AVaR(a).value(pi, Z) for a in (0.9, 0.5, 0.2)]
+        assert vals == sorted(vals)  # more averse, larger
+    assert build.AVaR(0.5).value([0.5, 0.5, 0.0], [1.0, 2.0, 9.0]) == pytest.approx(2.0)
+
+
 def test_workload_sizes(tmp_path):
     """The 229-node tree (3 events, horizon 10, branching stops at stage 3) and the 31-node tree
     (2 events, horizon 4)."""
