@@ -583,6 +583,31 @@ void fillDevice(spk_batch *b, T *p, size_t n) {
     hipLaunchKernelGGL(spk::k_fill<T>, dim3(1024), dim3(256), 0, b->ctx->stream, p, T(0), (long) n);
 }
 
+// What an autotuner changes on a batch while it times: the given tolerances (cleared so every
+// timed run does the same fixed iterations) and the launch choices it varies.  Put back as they
+// were when a timed run fails; on success the caller restores the tolerances and keeps the choice.
+struct TuneGuard {
+    spk_batch *b;
+    std::vector<double> tol;
+    int wgs, mv, block;
+    bool kept = false;
+    explicit TuneGuard(spk_batch *bb)
+        : b(bb), tol(bb->tolFixed), wgs(bb->wgs), mv(bb->laneMv), block(bb->laneBlockSz) {
+        b->tolFixed.clear();
+    }
+    void keep() {
+        b->tolFixed = tol;
+        kept = true;
+    }
+    ~TuneGuard() {
+        if (kept) return;
+        b->tolFixed = tol;
+        b->wgs = wgs;
+        b->laneMv = mv;
+        b->laneBlockSz = block;
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -1186,8 +1211,7 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
         o.chunk_iters = 0;
         o.block_size = 0;
         o.max_time_secs = 0.;
-        const std::vector<double> keepTol = b->tolFixed;
-        b->tolFixed.clear();
+        TuneGuard guard(b);
         const int cand[4][2] = {{0, 1024}, {8, 1024}, {0, 512}, {8, 512}};
         double best = 1e300;
         int bi = 0;
@@ -1209,7 +1233,7 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
         b->laneBlockSz = cand[bi][1];
         b->tuned = true;
         b->resumable = false;
-        b->tolFixed = keepTol;
+        guard.keep();
         if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
         if (mv) *mv = b->laneMv;
         if (block) *block = b->laneBlockSz;
@@ -1230,8 +1254,7 @@ int spk_batch_autotune_workgroups(spk_batch *b, const spk_options *opts, int ite
         o.tol_abs = o.tol_rel = 0.;  // a fixed number of iterations
         o.max_iters = iters;
         o.max_time_secs = 0.;
-        const std::vector<double> keepTol = b->tolFixed;
-        b->tolFixed.clear();
+        TuneGuard guard(b);
         const int cand[5] = {1, 8, 32, 64, 128};
         double best = 1e300;
         int bw = 1;
@@ -1248,7 +1271,7 @@ int spk_batch_autotune_workgroups(spk_batch *b, const spk_options *opts, int ite
             }
         }
         b->wgs = bw;
-        b->tolFixed = keepTol;
+        guard.keep();
         if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
         if (workgroups) *workgroups = bw;
         return 0;
