@@ -102,3 +102,14 @@ def test_out_of_range_tree_indices_are_an_error(loader_exe, problem_dirs, tmp_pa
 def test_missing_json_is_an_error(loader_exe, tmp_path):
     r = run(loader_exe, str(tmp_path))
     assert r.returncode == 2 and "error" in r.stdout, r.stdout + r.stderr
+
+
+def test_unsupported_constraint_kind_is_named(loader_exe, problem_dirs, tmp_path):
+    """The reference's polyhedronWithIdentity constraint (src/constraints.cuh:299) is not on the
+    HIP path: the loader refuses it by name."""
+    d = _copy(problem_dirs, tmp_path)
+    info = json.load(open(d / "data.json"))
+    info["constraint"]["nonleaf"] = "polyhedronWithIdentity"
+    json.dump(info, open(d / "data.json", "w"))
+    r = run(loader_exe, str(d))
+    assert r.returncode == 2 and "polyhedronWithIdentity" in r.stdout and "not supported" in r.stdout, r.stdout
