@@ -6,7 +6,11 @@
 // major per node and node-major across nodes (src/definitions.cu:11-14).  Operator
 // outputs that the reference leaves unwritten (t[0] of L*, the IV block of node 0
 // of L) are defined as zero here; from a zero start the reference keeps them at
-// zero too (src/costs.cuh:136,179-183), so iterates agree.
+// zero too (src/costs.cuh:136,179-183), so iterates agree.  The polyhedron-with-identity
+// constraint's L* is the plain adjoint.  The reference's also adds G' (G part) into the
+// identity part of its input in place (src/constraints.cuh:418-433), but that input is always
+// the work iterate's dual, which the next L rewrites before anything reads it
+// (src/cache.cuh:725-730, 748-751, 916-920), so iterates agree.
 //
 // Parity pins for each function live in tests/test_oracle.py (dense operator
 // matrix, KKT solve, closed-form projectors, textbook cone projections, lstsq); the
@@ -91,7 +95,7 @@ inline void gemvT(const double *M, int rows, int cols, const double *x, double *
     }
 }
 
-enum { CON_NO = 0, CON_RECT = 1, CON_POLY = 2 };
+enum { CON_NO = 0, CON_RECT = 1, CON_POLY = 2, CON_PWI = 3 };
 enum { COST_QUAD = 0, COST_QUADLIN = 1 };
 
 // ------------------------------------------------------------------- problem
@@ -99,6 +103,7 @@ struct Problem {
     int N, nl, nlf, S, E, nx, nu, yD, nD;
     int ncost, lcost, ncon, lcon;
     int con3, con5;  // per-node sizes of dual blocks III and V
+    int id3 = 0, id5 = 0;  // polyhedron with identity: size of the identity part per node
     bool precond;
     double gamma, gammaRecip;
     std::vector<long> anc, chFrom, numCh, stageFrom, stageTo;
@@ -180,6 +185,13 @@ struct Problem {
                     nLB[(size_t) i * con3 + k] = lb[k];
                     nUB[(size_t) i * con3 + k] = ub[k];
                 }
+        } else if (ncon == CON_PWI) {
+            // src/constraints.cuh:316-357: bounds [ilb of every node; glb of every node], the
+            // root state bounds kept
+            nG = readFp(dir, "nonleafConstraintGamma");
+            splitBounds(readFp(dir, "nonleafConstraintILB"), readFp(dir, "nonleafConstraintIUB"),
+                        readFp(dir, "nonleafConstraintGLB"), readFp(dir, "nonleafConstraintGUB"), nl, nxu, id3, con3,
+                        nLB, nUB);
         }
         con5 = 0;
         if (lcon == CON_RECT) {
@@ -201,6 +213,11 @@ struct Problem {
                     lLB[(size_t) i * con5 + k] = lb[k];
                     lUB[(size_t) i * con5 + k] = ub[k];
                 }
+        } else if (lcon == CON_PWI) {
+            lG = readFp(dir, "leafConstraintGamma");
+            splitBounds(readFp(dir, "leafConstraintILB"), readFp(dir, "leafConstraintIUB"),
+                        readFp(dir, "leafConstraintGLB"), readFp(dir, "leafConstraintGUB"), nlf, nx, id5, con5,
+                        lLB, lUB);
         }
         // Iterate layout (src/cache.cuh:482-581)
         sU = (size_t) nu * nl; sX = (size_t) nx * N; sY = (size_t) yD * nl; sT = N; sS = N;
@@ -211,6 +228,29 @@ struct Problem {
         sIter = sPrim + sDual;
         oU = 0; oX = oU + sU; oY = oX + sX; oT = oY + sY; oS = oT + sT;
         oI = 0; oII = oI + sI; oIII = oII + sII; oIV = oIII + sIII; oV = oIV + sIV; oVI = oV + sV;
+    }
+
+    // Polyhedron with identity over n nodes: the identity parts of all nodes come first, then
+    // the G parts (src/constraints.cuh:340-373); bounds laid out the same way.
+    static void splitBounds(const Vec &ilb, const Vec &iub, const Vec &glb, const Vec &gub, int n, int width,
+                            int &idPart, int &perNode, Vec &lo, Vec &hi) {
+        if ((int) ilb.size() != width || iub.size() != ilb.size() || gub.size() != glb.size())
+            throw std::runtime_error("polyhedron with identity: bound sizes do not match");
+        idPart = width;
+        perNode = width + (int) glb.size();
+        lo.resize((size_t) perNode * n);
+        hi.resize((size_t) perNode * n);
+        const size_t g0 = (size_t) width * n, rows = glb.size();
+        for (int i = 0; i < n; i++) {
+            for (int k = 0; k < width; k++) {
+                lo[(size_t) i * width + k] = ilb[k];
+                hi[(size_t) i * width + k] = iub[k];
+            }
+            for (size_t k = 0; k < rows; k++) {
+                lo[g0 + (size_t) i * rows + k] = glb[k];
+                hi[g0 + (size_t) i * rows + k] = gub[k];
+            }
+        }
     }
 };
 
@@ -237,6 +277,12 @@ void opL(const Problem &p, const double *prim, double *dual) {
             for (int k = 0; k < nx; k++) xu[k] = x[(size_t) i * nx + k];
             for (int k = 0; k < nu; k++) xu[nx + k] = u[(size_t) i * nu + k];
             gemv(p.nG.data(), p.con3, nxu, xu, iii, false);
+        } else if (p.ncon == CON_PWI) {  // src/constraints.cuh:386-390
+            const int rows = p.con3 - p.id3;
+            double *idPart = III + (size_t) i * p.id3, *gPart = III + (size_t) p.nl * p.id3 + (size_t) i * rows;
+            for (int k = 0; k < nx; k++) idPart[k] = x[(size_t) i * nx + k];
+            for (int k = 0; k < nu; k++) idPart[nx + k] = u[(size_t) i * nu + k];
+            gemv(p.nG.data(), rows, nxu, idPart, gPart, false);
         }
     }
     const int d4 = nxu + 2;
@@ -262,6 +308,12 @@ void opL(const Problem &p, const double *prim, double *dual) {
         double *v = V + (size_t) l * p.con5;
         if (p.lcon == CON_RECT) for (int k = 0; k < nx; k++) v[k] = xl[k];
         else if (p.lcon == CON_POLY) gemv(p.lG.data(), p.con5, nx, xl, v, false);
+        else if (p.lcon == CON_PWI) {  // src/constraints.cuh:398-401
+            const int rows = p.con5 - p.id5;
+            double *idPart = V + (size_t) l * p.id5, *gPart = V + (size_t) p.nlf * p.id5 + (size_t) l * rows;
+            for (int k = 0; k < nx; k++) idPart[k] = xl[k];
+            gemv(p.lG.data(), rows, nx, xl, gPart, false);
+        }
         double *vi = VI + (size_t) l * (nx + 2);
         gemv(p.lsqrtQ.data(), nx, nx, xl, vi, false);
         double half = s[p.nl + l] * 0.5;
@@ -295,6 +347,16 @@ void opLadj(const Problem &p, const double *dual, double *prim) {
             gemvT(p.nG.data(), p.con3, nxu, iii, xu, false);
             for (int k = 0; k < nx; k++) xi[k] = xu[k];
             for (int k = 0; k < nu; k++) ui[k] = xu[nx + k];
+        } else if (p.ncon == CON_PWI) {
+            // [x; u] = identity part + G' (G part), the adjoint of the operator above.  The
+            // reference adds G' (G part) into its input's identity part in place
+            // (src/constraints.cuh:418-422); this adjoint leaves its input unchanged.
+            const int rows = p.con3 - p.id3;
+            const double *idPart = III + (size_t) i * p.id3, *gPart = III + (size_t) p.nl * p.id3 + (size_t) i * rows;
+            double xu[1024];
+            gemvT(p.nG.data(), rows, nxu, gPart, xu, false);
+            for (int k = 0; k < nx; k++) xi[k] = idPart[k] + xu[k];
+            for (int k = 0; k < nu; k++) ui[k] = idPart[nx + k] + xu[nx + k];
         } else {
             for (int k = 0; k < nx; k++) xi[k] = 0.;
             for (int k = 0; k < nu; k++) ui[k] = 0.;
@@ -326,7 +388,12 @@ void opLadj(const Problem &p, const double *dual, double *prim) {
         const double *v = V + (size_t) l * p.con5;
         if (p.lcon == CON_RECT) for (int k = 0; k < nx; k++) xl[k] = v[k];
         else if (p.lcon == CON_POLY) gemvT(p.lG.data(), p.con5, nx, v, xl, false);
-        else for (int k = 0; k < nx; k++) xl[k] = 0.;
+        else if (p.lcon == CON_PWI) {  // src/constraints.cuh:430-433, without the in-place update
+            const int rows = p.con5 - p.id5;
+            const double *idPart = V + (size_t) l * p.id5, *gPart = V + (size_t) p.nlf * p.id5 + (size_t) l * rows;
+            gemvT(p.lG.data(), rows, nx, gPart, xl, false);
+            for (int k = 0; k < nx; k++) xl[k] += idPart[k];
+        } else for (int k = 0; k < nx; k++) xl[k] = 0.;
         const double *vi = VI + (size_t) l * (nx + 2);
         double w[1024];
         gemv(p.lsqrtQ.data(), nx, nx, vi, w, false);

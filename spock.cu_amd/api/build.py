@@ -7,7 +7,8 @@ Semantics follow ``api/python/spock/build.py``:
 * ``CostQuadratic`` / ``CostQuadraticPlusLinear`` (``:184-701``): per-node
   sqrt(Q), sqrt(R), linear terms and the SOC translation; node 0 of the nonleaf
   cost is the zeroed copy (``:338-341``, ``problem.py:783-791``).
-* ``Rectangle`` / ``Polyhedron`` (``:811-1014``) with their conditioning rules.
+* ``Rectangle`` / ``Polyhedron`` (``:811-1014``) and ``PolyhedronWithIdentity``
+  (``:1025-1103``) with their conditioning rules.
 * ``AVaR`` (``:1160-1202``): E = [alpha I; -I; 1^T], b = [pi; 0; 1].
 """
 import numpy as np
@@ -207,6 +208,29 @@ class Polyhedron:
 
     def condition(self, scale):
         self.matrix = self.G0 @ np.diag(1.0 / np.diagonal(scale))
+        return self
+
+
+class PolyhedronWithIdentity:
+    """lb <= [I; G] z <= ub: a rectangle and a polyhedron on the same vector
+    (``api/python/spock/build.py:1025-1103``).  Its dual block holds the identity parts of all
+    nodes first, then the G parts of all nodes (``src/constraints.cuh:340-373``); the two parts
+    are conditioned as a rectangle and a polyhedron."""
+    kind = "polyhedronWithIdentity"
+
+    def __init__(self, rect, poly):
+        if not isinstance(rect, Rectangle):
+            raise ValueError("PolyhedronWithIdentity constraint - not rectangle")
+        if not isinstance(poly, Polyhedron):
+            raise ValueError("PolyhedronWithIdentity constraint - not polyhedron")
+        if poly.matrix.shape[1] != rect.dim_per_node:
+            raise ValueError("PolyhedronWithIdentity constraint - matrix and rectangle dimensions are not equal")
+        self.rect, self.poly = rect, poly
+        self.dim_per_node = rect.dim_per_node + poly.dim_per_node
+
+    def condition(self, scale):
+        self.rect.condition(scale)
+        self.poly.condition(scale)
         return self
 
 

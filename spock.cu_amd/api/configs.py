@@ -131,6 +131,32 @@ def polyhedral_small(folder, seed=33, dt="d"):
     return polyhedral_iid(folder, seed=seed, dt=dt, events=3, horizon=3, nx=4, nu=2)
 
 
+def polyhedron_identity_small(folder, seed=44, dt="d"):
+    """The reference's polyhedron-with-identity constraint on the 40-node i.i.d. tree (3 events,
+    horizon 3), nx=4, nu=2, quadratic costs, AV@R 0.95: at nonleaf nodes |x| <= 2, |u| <= 0.3 and
+    2 (nx + nu) random rows within +-3.3; at leaves |x| <= 2 and 2 nx rows within +-3.3.  The
+    identity part keeps the root state bounds, so the bounds are loose enough for x0 in [-1, 1]^4;
+    at the solution from such an x0 both parts have active rows."""
+    rng = np.random.default_rng(seed)
+    events, nx, nu = 3, 4, 2
+    tr = tree_mod.IidProcess(rng.dirichlet(np.ones(events)), 3, 3, dt=dt).build()
+    dyn = [build.Dynamics(np.eye(nx) + 0.1 * rng.standard_normal((nx, nx)), 0.5 * rng.standard_normal((nx, nu)))
+           for _ in range(events)]
+    costs = [build.CostQuadratic(np.eye(nx), 0.1 * np.eye(nu)) for _ in range(events)]
+    leaf = build.CostQuadratic(np.eye(nx), leaf=True)
+    lim = np.vstack((2.0 * np.ones((nx, 1)), 0.3 * np.ones((nu, 1))))
+    rn, rl = 2 * (nx + nu), 2 * nx
+    ncon = build.PolyhedronWithIdentity(build.Rectangle(-lim, lim),
+                                        build.Polyhedron(rng.standard_normal((rn, nx + nu)), -3.3 * np.ones(rn),
+                                                         3.3 * np.ones(rn)))
+    lcon = build.PolyhedronWithIdentity(build.Rectangle(-2.0 * np.ones((nx, 1)), 2.0 * np.ones((nx, 1))),
+                                        build.Polyhedron(rng.standard_normal((rl, nx)), -3.3 * np.ones(rl),
+                                                         3.3 * np.ones(rl)))
+    return (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(ncon).with_constraint_leaf(lcon)
+            .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
+
+
 def unconstrained_markov(folder, seed=7, dt="d"):
     """The 31-node family (2 events, horizon 4; nx=3, nu=2; AV@R 0.95) with no state or input
     constraints at any node (the reference's ``no`` constraint)."""
@@ -175,5 +201,5 @@ def initial_states(seed, nx, count, kind="normal"):
 
 def generate(name, folder, **kw):
     os.makedirs(folder, exist_ok=True)
-    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov, "poly": polyhedral_small, "nocon": unconstrained_markov, "affine": affine_markov,
+    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov, "poly": polyhedral_small, "nocon": unconstrained_markov, "affine": affine_markov, "pwi": polyhedron_identity_small,
             "large": polyhedral_iid}[name](folder, **kw)

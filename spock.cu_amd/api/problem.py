@@ -253,6 +253,8 @@ class Problem:
             return np.zeros((z.shape[0], 0))
         if con.kind == "rectangle":
             return z
+        if con.kind == "polyhedronWithIdentity":  # identity parts of all nodes, then G parts
+            return np.concatenate([z.ravel(), (z @ con.poly.matrix.T).ravel()])
         return z @ con.matrix.T
 
     @staticmethod
@@ -261,6 +263,10 @@ class Problem:
             return np.zeros((n, width))
         if con.kind == "rectangle":
             return dual.copy()
+        if con.kind == "polyhedronWithIdentity":
+            flat = np.asarray(dual).reshape(-1)
+            r, g = con.rect.dim_per_node, con.poly.dim_per_node
+            return flat[:n * r].reshape(n, r) + flat[n * r:n * (r + g)].reshape(n, g) @ con.poly.matrix
         return dual @ con.matrix
 
     def _step_size(self):
@@ -329,6 +335,12 @@ class Problem:
                 tens[txt + "Gamma"] = con.matrix
                 tens[txt + "GLB"] = con.lower_bound
                 tens[txt + "GUB"] = con.upper_bound
+            elif con.kind == "polyhedronWithIdentity":  # api/python/spock/problem.py:578-585
+                tens[txt + "ILB"] = con.rect.lower_bound
+                tens[txt + "IUB"] = con.rect.upper_bound
+                tens[txt + "Gamma"] = con.poly.matrix
+                tens[txt + "GLB"] = con.poly.lower_bound
+                tens[txt + "GUB"] = con.poly.upper_bound
         for name, arr in tens.items():
             write_bt(os.path.join(folder, f"{name}_{dt}.bt"), arr, dt)
         return folder

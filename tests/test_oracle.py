@@ -52,6 +52,26 @@ def test_adjoint(problem_dirs, name):
     assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))
 
 
+@pytest.mark.parametrize("name", ["small", "poly", "nocon", "affine", "pwi"])
+def test_operators_match_the_generator(tmp_path, name):
+    """The oracle's L and L* (a C++ transcription of src/operator.cuh) against the generator's
+    (spock_amd.api.problem, written separately from the Python reference) on random vectors, for
+    every constraint kind: box, polyhedron, none, and the polyhedron with identity whose dual block
+    holds all identity parts before all G parts."""
+    d = str(tmp_path / name)
+    gen = configs.generate(name, d)
+    o = Oracle(d)
+    rng = np.random.default_rng(12)
+    p, q = rng.standard_normal(o.n_prim), rng.standard_normal(o.n_dual)
+    assert np.abs(o.L(p) - gen.op(p)).max() <= 1e-12 * max(1.0, np.abs(gen.op(p)).max())
+    assert np.abs(o.Ladj(q) - gen.adj(q)).max() <= 1e-12 * max(1.0, np.abs(gen.adj(q)).max())
+    if name == "pwi":
+        lhs, rhs = p @ o.Ladj(q), o.L(p) @ q
+        assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))
+        Ld = nr.dense_operator(o.L, o.n_prim)
+        assert o.gamma * np.linalg.norm(Ld, 2) == pytest.approx(0.99, abs=2e-3)
+
+
 @pytest.mark.parametrize("name", ["fixture", "small", "poly", "nocon", "affine"])
 def test_step_size_is_099_over_norm(problem_dirs, name):
     o = Oracle(problem_dirs[name])
@@ -214,7 +234,7 @@ def test_golden_vectors_reproduce(tmp_path):
     assert [r[k] for k in ("k0", "k1", "k2", "k2bt", "k3")] == list(g["spock75_counts"])
 
 
-@pytest.mark.parametrize("name", ["small", "poly"])
+@pytest.mark.parametrize("name", ["small", "poly", "pwi"])
 def test_converged_solution_solves_the_conic_program(tmp_path, name):
     """An independent optimality check of the whole iteration (CP step, dual prox with its
     translations, SuperMann and Anderson).  The RAOCP in conic form is built from the generator's
@@ -230,8 +250,9 @@ def test_converged_solution_solves_the_conic_program(tmp_path, name):
     d = str(tmp_path / name)
     gen = configs.generate(name, d)
     o = Oracle(d)
-    x0 = configs.initial_states(1, o.nx, 1, "uniform")[0] if name == "small" else \
-        configs.initial_states(33, o.nx, 1, "small")[0]
+    x0 = {"small": configs.initial_states(1, o.nx, 1, "uniform")[0],
+          "poly": configs.initial_states(33, o.nx, 1, "small")[0],
+          "pwi": configs.initial_states(33, o.nx, 1, "uniform")[0]}[name]
     # SPOCK levels off near an error of 1e-5 on these problems; 20,000 iterations reach it
     ref = o.solve(x0, tol_abs=1e-7, tol_rel=0.0, max_iters=20000)
     assert ref["err"] < 1e-4
@@ -263,20 +284,30 @@ def test_converged_solution_solves_the_conic_program(tmp_path, name):
             rows.append(Lm[oI + i * yD + k]); hs.append(c[oI + i * yD + k])
         rows.append(Lm[oII + i]); hs.append(c[oII + i])
 
-    def boxes(off, count, dim, lb, ub, skip_root_state):
-        for i in range(count):
-            for k in range(dim):
-                if skip_root_state and i == 0 and k < nx:
-                    continue
-                r = Lm[off + i * dim + k]
-                if np.isfinite(lb[k]):
-                    rows.append(r); hs.append(-lb[k])
-                if np.isfinite(ub[k]):
-                    rows.append(-r); hs.append(ub[k])
+    def block_bounds(con, count, relax_root_state):
+        """Bounds of a constraint's dual block in its layout: per node for a rectangle or a
+        polyhedron; identity parts of all nodes, then G parts, for a polyhedron with identity."""
+        if con.kind == "polyhedronWithIdentity":
+            return (np.concatenate([np.tile(con.rect.lower_bound.ravel(), count),
+                                    np.tile(con.poly.lower_bound.ravel(), count)]),
+                    np.concatenate([np.tile(con.rect.upper_bound.ravel(), count),
+                                    np.tile(con.poly.upper_bound.ravel(), count)]))
+        lb, ub = np.tile(con.lower_bound.ravel(), count), np.tile(con.upper_bound.ravel(), count)
+        if relax_root_state:
+            lb[:nx], ub[:nx] = -np.inf, np.inf
+        return lb, ub
+
+    def boxes(off, lb, ub):
+        for k in range(lb.size):
+            r = Lm[off + k]
+            if np.isfinite(lb[k]):
+                rows.append(r); hs.append(-lb[k])
+            if np.isfinite(ub[k]):
+                rows.append(-r); hs.append(ub[k])
     if gen.ncon.kind != "no":
-        boxes(oIII, nl, c3, gen.ncon.lower_bound.ravel(), gen.ncon.upper_bound.ravel(), gen.ncon.kind == "rectangle")
+        boxes(oIII, *block_bounds(gen.ncon, nl, gen.ncon.kind == "rectangle"))
     if gen.lcon.kind != "no":
-        boxes(oV, nlf, c5, gen.lcon.lower_bound.ravel(), gen.lcon.upper_bound.ravel(), False)
+        boxes(oV, *block_bounds(gen.lcon, nlf, False))
     G, h = np.array(rows), np.array(hs)
     socs = [(oIV + j * d4, d4) for j in range(1, N)] + [(oVI + l * d6, d6) for l in range(nlf)]
 
