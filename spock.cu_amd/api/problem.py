@@ -32,8 +32,9 @@ def _stack(mats):
 
 class Problem:
     def __init__(self, tree, nx, nu, dynamics, nonleaf_cost, leaf_cost, nonleaf_con, leaf_con, risks,
-                 precondition=True):
+                 precondition=True, julia=False):
         self.tree, self.nx, self.nu = tree, nx, nu
+        self.julia = julia
         self.dyn = dynamics
         self.ncost, self.lcost = nonleaf_cost, leaf_cost
         self.ncon, self.lcon = nonleaf_con, leaf_con
@@ -276,6 +277,37 @@ class Problem:
         lam = eigsh(op, k=1, which="LM", v0=v0, return_eigenvectors=False, tol=1e-12)
         return 0.99 / np.sqrt(float(np.max(lam)))
 
+    def _unconditioned_tensors(self):
+        """The data before preconditioning under an ``uncond_`` prefix, for solvers that take the
+        problem as posed (``api/python/spock/problem.py:605-684``): dynamics A, B, c of every
+        node; nonleaf and leaf cost matrices, with their linear terms for the
+        quadratic-plus-linear kind; constraint bounds and matrices."""
+        nc, lc = self.ncost, self.lcost
+        out = {
+            "uncond_dynamics_A": _stack([d.A0 for d in self.dyn]),
+            "uncond_dynamics_B": _stack([d.B0 for d in self.dyn]),
+            "uncond_dynamics_c": _stack([d.c0 for d in self.dyn]),
+            "uncond_cost_nonleaf_Q": _stack(nc.Q0),
+            "uncond_cost_nonleaf_R": _stack(nc.R0),
+            "uncond_cost_leaf_Q": _stack([lc.Q0[-1]]),
+        }
+        if nc.is_quadratic_plus_linear:
+            out["uncond_cost_nonleaf_q"] = _stack(nc.q0)
+            out["uncond_cost_nonleaf_r"] = _stack(nc.r0)
+        if lc.is_quadratic_plus_linear:
+            out["uncond_cost_leaf_q"] = _stack([lc.q0[-1]])
+        for con, txt in ((self.ncon, "uncond_nonleafConstraint"), (self.lcon, "uncond_leafConstraint")):
+            rect = con if con.kind == "rectangle" else getattr(con, "rect", None)
+            poly = con if con.kind == "polyhedron" else getattr(con, "poly", None)
+            if rect is not None:
+                out[txt + "ILB"] = rect.lb0
+                out[txt + "IUB"] = rect.ub0
+            if poly is not None:
+                out[txt + "Gamma"] = poly.G0
+                out[txt + "GLB"] = poly.lower_bound
+                out[txt + "GUB"] = poly.upper_bound
+        return out
+
     # ------------------------------------------------------------------ files
     def write(self, folder):
         t = self.tree
@@ -341,6 +373,8 @@ class Problem:
                 tens[txt + "Gamma"] = con.poly.matrix
                 tens[txt + "GLB"] = con.poly.lower_bound
                 tens[txt + "GUB"] = con.poly.upper_bound
+        if self.julia:
+            tens.update(self._unconditioned_tensors())
         for name, arr in tens.items():
             write_bt(os.path.join(folder, f"{name}_{dt}.bt"), arr, dt)
         return folder
@@ -357,6 +391,7 @@ class Factory:
         self.ncon, self.lcon = build.No(), build.No()
         self.risks = None
         self.precondition = True
+        self.julia = False
 
     def with_dynamics(self, dynamics):
         self.dyn = [dynamics.zeros_like()] + [build.Dynamics(dynamics.A0, dynamics.B0, dynamics.c0 if dynamics.is_affine else None)
@@ -411,10 +446,15 @@ class Factory:
         self.precondition = enable
         return self
 
+    def with_julia(self):
+        """Also write the unconditioned data (``api/python/spock/problem.py:854-856``)."""
+        self.julia = True
+        return self
+
     def generate_problem(self, folder=None):
         p = Problem(self.tree, self.nx, self.nu, self.dyn,
                     build.CostSet(self.ncost_list, leaf=False), build.CostSet(self.lcost_list, leaf=True),
-                    self.ncon, self.lcon, self.risks, self.precondition)
+                    self.ncon, self.lcon, self.risks, self.precondition, self.julia)
         if folder is not None:
             p.write(folder)
         return p

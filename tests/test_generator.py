@@ -235,3 +235,49 @@ def test_from_data_conditional_probabilities_sum_to_one():
     t = FromData(data, branching).build()
     for i in range(t.num_nonleaf_nodes):
         assert abs(t.cond_prob[t.children[i]].sum() - 1.0) < 1e-10
+
+
+def test_julia_export_is_the_unconditioned_data(tmp_path):
+    """with_julia() also writes the data before preconditioning (api/python/spock/problem.py:605-684):
+    the conditioned dynamics_AB of every node must be Sx [A B] diag(Sx^-1, Su^-1) of the exported
+    A and B, the exported costs and bounds must be the ones given, and the linear terms appear
+    only for the quadratic-plus-linear kind."""
+    from spock_amd.api import problem as problem_mod
+    rng = np.random.default_rng(9)
+    nx, nu = 3, 2
+    tr = MarkovChain(np.array([[0.6, 0.4], [0.3, 0.7]]), np.array([0.5, 0.5]), 3, 3).build()
+    dyn = [build.Dynamics(np.eye(nx) + 0.1 * rng.standard_normal((nx, nx)), rng.standard_normal((nx, nu)),
+                          0.1 * rng.standard_normal(nx)) for _ in range(2)]
+    Q, R, q, r = 4.0 * np.eye(nx), 0.5 * np.eye(nu), rng.standard_normal(nx), rng.standard_normal(nu)
+    costs = [build.CostQuadraticPlusLinear(Q, q, R, r) for _ in range(2)]
+    leaf = build.CostQuadratic(9.0 * np.eye(nx), leaf=True)
+    lim = np.vstack((2.0 * np.ones((nx, 1)), np.ones((nu, 1))))
+    pwi = build.PolyhedronWithIdentity(build.Rectangle(-lim, lim),
+                                       build.Polyhedron(rng.standard_normal((4, nx + nu)), -3 * np.ones(4), 3 * np.ones(4)))
+    d = str(tmp_path / "julia")
+    gen = (problem_mod.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+           .with_cost_leaf(leaf).with_constraint_nonleaf(pwi)
+           .with_constraint_leaf(build.Rectangle(-2 * np.ones(nx), 2 * np.ones(nx)))
+           .with_risk(build.AVaR(0.9)).with_preconditioning().with_julia().generate_problem(d))
+    assert gen.preconditioned
+
+    def rd(name):
+        return btio.read_bt(os.path.join(d, name + "_d.bt"))
+    A, B, c, AB = rd("uncond_dynamics_A"), rd("uncond_dynamics_B"), rd("uncond_dynamics_c"), rd("dynamics_AB")
+    sx, su = gen.scaling[:nx], gen.scaling[nx:]
+    events = tr.events_arr
+    for j in range(1, tr.num_nodes):
+        e = int(events[j])
+        assert np.allclose(A[:, :, j], dyn[e].A0) and np.allclose(B[:, :, j], dyn[e].B0)
+        assert np.allclose(c[:, 0, j], dyn[e].c0.ravel())
+        cond = np.diag(sx) @ np.hstack((A[:, :, j], B[:, :, j])) @ np.diag(np.concatenate((1 / sx, 1 / su)))
+        assert np.allclose(AB[:, :, j], cond)
+    assert np.allclose(rd("uncond_cost_nonleaf_Q")[:, :, 1], Q) and np.allclose(rd("uncond_cost_nonleaf_R")[:, :, 1], R)
+    assert np.allclose(rd("uncond_cost_nonleaf_q")[:, 0, 1], q) and np.allclose(rd("uncond_cost_nonleaf_r")[:, 0, 1], r)
+    assert np.allclose(rd("uncond_cost_leaf_Q")[:, :, 0], 9.0 * np.eye(nx))
+    assert not os.path.exists(os.path.join(d, "uncond_cost_leaf_q_d.bt"))
+    assert np.allclose(rd("uncond_nonleafConstraintILB").ravel(), -lim.ravel())
+    assert np.allclose(rd("uncond_nonleafConstraintGamma")[:, :, 0], pwi.poly.G0)
+    assert np.allclose(rd("uncond_leafConstraintIUB").ravel(), 2 * np.ones(nx))
+    # the conditioned files hold the scaled data: bounds times the scaling
+    assert np.allclose(rd("nonleafConstraintILB").ravel(), -lim.ravel() * gen.scaling)
