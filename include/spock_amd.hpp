@@ -87,6 +87,24 @@ std::vector<T> parseFromFile(const std::string &file) {
     return out;
 }
 
+// The part of GPUtils' DTensor<T> that the reference's drivers use to read a tensor file
+// (examples/*/main.cu: DTensor<T>::parseFromFile(path), then download(vector)): here a host copy
+// of the file's data, in its stored column-major order.
+template <typename T>
+class DTensor {
+public:
+    static DTensor parseFromFile(const std::string &file) {
+        DTensor d;
+        d.m_data = spock_amd::parseFromFile<T>(file);
+        return d;
+    }
+    void download(std::vector<T> &out) const { out.assign(m_data.begin(), m_data.end()); }
+    size_t numEl() const { return m_data.size(); }
+
+private:
+    std::vector<T> m_data;
+};
+
 template <typename T>
 class ScenarioTree {
 public:
@@ -102,6 +120,7 @@ public:
     std::string fpFileExt() const { return sizeof(T) == 8 ? "_d.bt" : "_f.bt"; }
     size_t numStates() const { return field("numStates"); }
     size_t numInputs() const { return field("numInputs"); }
+    size_t numStatesAndInputs() const { return numStates() + numInputs(); }  // src/tree.cuh:121
     size_t numNodes() const { return field("numNodes"); }
     size_t numNonleafNodes() const { return field("numNonleafNodes"); }
     size_t numStages() const { return field("numStages"); }

@@ -32,6 +32,13 @@ int main(int argc, char **argv) {
                 tree.numNonleafNodes(), tree.numStages(), tree.numEvents());
     for (double v : x0) std::printf("%.17g ", v);
     std::printf("\n%.17g %g\n", cp[5], (double) anc[5]);
+    // the reference drivers' way (examples/random/main.cu:48-50)
+    DTensor<double> d_initState = DTensor<double>::parseFromFile(tree.path() + "initialState" + tree.fpFileExt());
+    std::vector<double> initState(tree.numStates());
+    d_initState.download(initState);
+    std::printf("%zu %zu", tree.numStatesAndInputs(), d_initState.numEl());
+    for (double v : initState) std::printf(" %.17g", v);
+    std::printf("\n");
     return 0;
 }
 '''
@@ -91,6 +98,9 @@ def test_header_reads_the_problem_folder(tmp_path):
     cp = btio.read_flat(os.path.join(folder, "conditionalProbabilities_d.bt"))
     anc = btio.read_flat(os.path.join(folder, "ancestors_u.bt"))
     assert float(out[2].split()[0]) == cp[5] and float(out[2].split()[1]) == anc[5]
+    ref_way = out[3].split()
+    assert [int(v) for v in ref_way[:2]] == [5, 3]
+    assert np.array_equal(np.array([float(v) for v in ref_way[2:]]), x0)
 
 
 @needs_lib
@@ -138,3 +148,13 @@ def test_reference_driver_on_the_229_node_problem(tmp_path):
     run = subprocess.run([str(exe), folder + "/"], cwd=str(tmp_path), capture_output=True, text=True, timeout=170)
     print(run.stdout)
     assert run.returncode == 0 and "Saved" in run.stdout
+
+
+@needs_lib
+def test_anderson_buffer_driver_builds(tmp_path):
+    """spock.cu_amd/examples/anderson_buffer_main.cpp (the reference's examples/andersonBuffer
+    study on the drop-in header) compiles and links against the library; running it needs a GPU."""
+    src = open(os.path.join(ROOT, "spock.cu_amd", "examples", "anderson_buffer_main.cpp")).read()
+    exe = tmp_path / "anderson_buffer_main"
+    _compile(src, exe, tmp_path)
+    assert exe.exists()
