@@ -43,3 +43,11 @@ def test_cost_gaps_of_oracle_solutions(bench, tmp_path):
     assert np.allclose(s_t, J_t, rtol=1e-4, atol=1e-6)
     empty = bench.cost_gaps(prob, gen, loose, tight, np.zeros(3, dtype=bool))
     assert empty["J"] == {"mean": None, "max": None, "within_1e-3": 0}
+
+
+def test_more_than_one_gpu_is_refused():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
