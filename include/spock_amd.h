@@ -20,10 +20,14 @@
  *                            projectPrimalWorkspaceOnKernels, iter(), dotM;
  *                            tests/cuda/testOperator.cu, testProblem.cu, testCache.cu)
  *
- * Conventions: every function returns 0 on success and a negative code on error;
- * spk_last_error(ctx) returns the message.  No exception crosses the ABI.  A context
+ * Conventions: every function returns 0 on success and a negative code on error
+ * (enum spk_error); spk_last_error(ctx) returns the message.  No exception crosses the ABI.  A context
  * owns one HIP stream (default priority); solves are enqueued on it and
  * spk_batch_results() waits for them.  One context per host thread.
+ * Lifetimes: a context lives while the caller's handle or any problem, batch or network made on
+ * it does, and a problem while the caller's handle or any batch holding an instance of it does;
+ * spk_*_destroy drops the caller's handle, so handles may be destroyed in any order.  (The
+ * reference's Cache and ProblemData hold references that must not dangle, src/cache.cuh:97-98.)
  * A batch holds `count` instances; each instance refers to a loaded problem (several
  * instances may share one problem, e.g. many initial states of one tree).
  */
@@ -40,6 +44,10 @@ typedef struct spk_batch spk_batch;
 typedef struct spk_mlp spk_mlp;
 
 enum spk_dtype { SPK_F64 = 0, SPK_F32 = 1 };
+/* error codes: SPK_ERR for every failure but one; SPK_ERR_COOPERATIVE_LAUNCH when the device
+ * refused the cooperative launch of a grid-mode solve (spk_batch_set_workgroups > 1), which
+ * queued nothing and leaves the batch as it was, so the solve can be repeated with one workgroup */
+enum spk_error { SPK_OK = 0, SPK_ERR = -1, SPK_ERR_COOPERATIVE_LAUNCH = -2 };
 enum spk_method { SPK_SPOCK = 0, SPK_CP = 1 };
 /* solve status codes, identical to enum CacheStatus (src/cache.cuh:68-73) */
 enum spk_status { SPK_NOT_RUN = -1, SPK_CONVERGED = 0, SPK_OUT_OF_ITER = 1, SPK_OUT_OF_TIME = 2 };

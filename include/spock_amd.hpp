@@ -13,8 +13,11 @@
 //   enum CacheStatus and toString               src/cache.cuh:68-88
 // Errors are std::invalid_argument with the library's message, as the reference throws.
 // T = double solves in FP64, T = float in FP32 (the reference instantiates both,
-// src/definitions.cu:24-443).  solution() returns the iterate as a host vector (the reference
-// returns its device tensor).
+// src/definitions.cu:24-443).  solution(), solutionPrim() and solutionDual() return a DTensor<T>&
+// and getDual takes and returns DTensor<T>, as the reference does (src/cache.cuh:371-382); here
+// DTensor<T> is a host copy (the iterate stays on the device inside the library), refreshed by
+// each call, with the reference's download(std::vector<T>&) and numEl(), and it converts to
+// std::vector<T>.
 #pragma once
 
 #include <chrono>
@@ -32,13 +35,14 @@ namespace spock_amd {
 
 enum CacheStatus { notRun = -1, converged = 0, outOfIter = 1, outOfTime = 2 };
 
-inline std::string toString(int status) {
+// The reference's status strings, which its drivers print (src/cache.cuh:75-88).
+static inline const char *toString(int status) {
     switch (status) {
-        case notRun: return "notRun";
-        case converged: return "converged";
-        case outOfIter: return "outOfIter";
-        case outOfTime: return "outOfTime";
-        default: return "unknown";
+        case notRun: return "Cache has not been run!";
+        case converged: return "Converged!";
+        case outOfIter: return "Out of iterations!";
+        case outOfTime: return "Out of time!";
+        default: return "Unknown status.";
     }
 }
 
@@ -87,19 +91,23 @@ std::vector<T> parseFromFile(const std::string &file) {
     return out;
 }
 
-// The part of GPUtils' DTensor<T> that the reference's drivers use to read a tensor file
-// (examples/*/main.cu: DTensor<T>::parseFromFile(path), then download(vector)): here a host copy
-// of the file's data, in its stored column-major order.
+// The part of GPUtils' DTensor<T> that the reference's callers use: DTensor<T>::parseFromFile(path)
+// then download(vector) (examples/*/main.cu), and the tensors Cache::solution*() and getDual
+// return.  Here a host copy in column-major order; it converts to std::vector<T>.
 template <typename T>
 class DTensor {
 public:
-    static DTensor parseFromFile(const std::string &file) {
-        DTensor d;
-        d.m_data = spock_amd::parseFromFile<T>(file);
-        return d;
-    }
+    DTensor() = default;
+    explicit DTensor(std::vector<T> data) : m_data(std::move(data)) {}
+    static DTensor parseFromFile(const std::string &file) { return DTensor(spock_amd::parseFromFile<T>(file)); }
     void download(std::vector<T> &out) const { out.assign(m_data.begin(), m_data.end()); }
+    void upload(const std::vector<T> &in) { m_data.assign(in.begin(), in.end()); }
     size_t numEl() const { return m_data.size(); }
+    size_t numRows() const { return m_data.size(); }
+    size_t numCols() const { return 1; }
+    size_t numMats() const { return 1; }
+    operator std::vector<T>() const { return m_data; }
+    const std::vector<T> &values() const { return m_data; }
 
 private:
     std::vector<T> m_data;
@@ -173,22 +181,30 @@ private:
 template <typename T>
 class Cache {
 public:
-    // workgroups: how many workgroups solve the instance together (0: 64 for iterates of 50,000
-    // elements or more, where the cooperative grid mode is several times faster, else 1)
+    // Iterate sizes for which the automatic choice spreads one instance over 64 cooperating
+    // workgroups (grid mode): the range of the 229-node, 52,360-element problem this has been run
+    // and measured on (several times faster than one workgroup there).  Elsewhere one workgroup
+    // solves the instance unless workgroups(n) asks otherwise.
+    static constexpr size_t kAutoGridMin = 50000, kAutoGridMax = 100000;
+
+    // workgroups: how many workgroups solve the instance together (0: automatic, see above)
     Cache(ScenarioTree<T> &, ProblemData<T> &data, const spk_options &opts, int workgroups = 0)
         : m_data(&data), m_opts(opts) {
         spk_problem *p = data.handle();
         detail::check(spk_batch_create(detail::context(), 1, &p, sizeof(T) == 8 ? SPK_F64 : SPK_F32, &m_b),
                       "Cache");
         if (workgroups == 0) {
-            workgroups = data.sizePrimal() + data.sizeDual() >= 50000 ? 64 : 1;
+            const size_t n = data.sizePrimal() + data.sizeDual();
+            workgroups = (n >= kAutoGridMin && n <= kAutoGridMax) ? 64 : 1;
             m_autoGrid = workgroups > 1;
         }
+        m_workgroups = workgroups;
         detail::check(spk_batch_set_workgroups(m_b, workgroups), "Cache");
     }
     ~Cache() { spk_batch_destroy(m_b); }
     Cache(Cache &&o) noexcept : m_data(o.m_data), m_opts(o.m_opts), m_b(o.m_b), m_autoGrid(o.m_autoGrid),
-                                m_exit(o.m_exit), m_iters(o.m_iters), m_time(o.m_time) {
+                                m_workgroups(o.m_workgroups), m_gridFallback(o.m_gridFallback), m_exit(o.m_exit),
+                                m_iters(o.m_iters), m_time(o.m_time) {
         o.m_b = nullptr;
     }
     Cache(const Cache &) = delete;
@@ -204,23 +220,27 @@ public:
     size_t sizePrimal() const { return m_data->sizePrimal(); }
     size_t sizeDual() const { return m_data->sizeDual(); }
 
-    std::vector<T> solution() {
-        std::vector<double> z(sizePrimal() + sizeDual());
-        detail::check(spk_batch_get_iterate(m_b, 0, z.data()), "solution");
-        return std::vector<T>(z.begin(), z.end());
+    // The accepted iterate [prim | dual] and its parts (src/cache.cuh:378-382), copied from the
+    // device by each call.
+    DTensor<T> &solution() {
+        m_solution.upload(iterate());
+        return m_solution;
     }
-    std::vector<T> solutionPrim() {
-        std::vector<T> z = solution();
+    DTensor<T> &solutionPrim() {
+        std::vector<T> z = iterate();
         z.resize(sizePrimal());
-        return z;
+        m_solutionPrim.upload(z);
+        return m_solutionPrim;
     }
-    std::vector<T> solutionDual() {
-        const std::vector<T> z = solution();
-        return std::vector<T>(z.begin() + sizePrimal(), z.end());
+    DTensor<T> &solutionDual() {
+        const std::vector<T> z = iterate();
+        m_solutionDual.upload(std::vector<T>(z.begin() + sizePrimal(), z.end()));
+        return m_solutionDual;
     }
 
     // Dual point of the CP step from the current iterate with `primal` in place of the primal prox
     // (src/cache.cuh:371-376): modifyDual + proximalDual.
+    DTensor<T> getDual(DTensor<T> &primal) { return DTensor<T>(getDual(primal.values())); }
     std::vector<T> getDual(const std::vector<T> &primal) {
         if (primal.size() != sizePrimal()) throw std::invalid_argument("[getDual] primal has the wrong size");
         std::vector<double> z(sizePrimal() + sizeDual()), out(z.size(), 0.);
@@ -230,17 +250,22 @@ public:
         return std::vector<T>(out.begin() + sizePrimal(), out.end());
     }
 
+    // Workgroups the last solve ran with, and whether a refused cooperative launch made the
+    // automatic grid mode fall back to one workgroup (no counterpart in the reference).
+    int workgroups() const { return m_workgroups; }
+    bool gridFallback() const { return m_gridFallback; }
+
     // states x (numNodes x numStates) and inputs u (numNonleafNodes x numInputs) of the solution,
     // undoing the preconditioning (src/cache.cuh:386-412)
     std::vector<T> states() {
-        const std::vector<T> z = solution();
+        const std::vector<T> z = iterate();
         const size_t nx = m_data->numStates(), nu = m_data->numInputs(), oX = m_data->numNonleafNodes() * nu;
         std::vector<T> x(z.begin() + oX, z.begin() + oX + m_data->numNodes() * nx);
         for (size_t k = 0; k < x.size(); k++) x[k] /= (T) m_data->scaling()[k % nx];
         return x;
     }
     std::vector<T> inputs() {
-        const std::vector<T> z = solution();
+        const std::vector<T> z = iterate();
         const size_t nx = m_data->numStates(), nu = m_data->numInputs();
         std::vector<T> u(z.begin(), z.begin() + m_data->numNonleafNodes() * nu);
         for (size_t k = 0; k < u.size(); k++) u[k] /= (T) m_data->scaling()[nx + k % nu];
@@ -253,6 +278,12 @@ public:
     }
 
 private:
+    std::vector<T> iterate() {
+        std::vector<double> z(sizePrimal() + sizeDual());
+        detail::check(spk_batch_get_iterate(m_b, 0, z.data()), "solution");
+        return std::vector<T>(z.begin(), z.end());
+    }
+
     int run(int method, std::vector<T> &x0, std::vector<T> *prev) {
         if (x0.size() != m_data->numStates())
             throw std::invalid_argument("[initialiseState] Error initialising state: problem setup for " +
@@ -268,11 +299,13 @@ private:
         spk_options o = m_opts;
         o.method = method;
         int rc = spk_batch_solve(m_b, &o, prev ? 1 : 0);
-        if (rc != 0 && m_autoGrid) {
+        if (rc == SPK_ERR_COOPERATIVE_LAUNCH && m_autoGrid) {
             // the automatic grid mode needs a cooperative launch, which the device can refuse (it
-            // did while another live process had made cooperative launches): from then on the
-            // instance is solved by one workgroup, as when it is below the size threshold
+            // did while another live process had made cooperative launches); the refused launch
+            // queued nothing, and from then on one workgroup solves the instance
             m_autoGrid = false;
+            m_gridFallback = true;
+            m_workgroups = 1;
             detail::check(spk_batch_set_workgroups(m_b, 1), "Cache");
             rc = spk_batch_solve(m_b, &o, prev ? 1 : 0);
         }
@@ -289,6 +322,9 @@ private:
     spk_options m_opts;
     spk_batch *m_b = nullptr;
     bool m_autoGrid = false;  // grid mode chosen by size, not asked for
+    int m_workgroups = 1;
+    bool m_gridFallback = false;
+    DTensor<T> m_solution, m_solutionPrim, m_solutionDual;
     int m_exit = notRun;
     size_t m_iters = 0;
     T m_time = 0;

@@ -56,8 +56,12 @@ struct DevTableSet {  // device copies of one table: data as T, index as int
 
 }  // namespace
 
-struct spk_problem {
-    spk_ctx *ctx = nullptr;
+struct spk_problem : spk::Counted {
+    spk_ctx *ctx = nullptr;  // holds a reference
+    explicit spk_problem(spk_ctx *c) : ctx(spk::retain(c)) {}
+    ~spk_problem() { spk::releaseCtx(ctx); }
+    spk_problem(const spk_problem &) = delete;
+    spk_problem &operator=(const spk_problem &) = delete;
     HostProblem hp;
     DevBuf anc, chFrom, numCh, stageFrom, stageTo;
     std::map<std::string, DevTableSet<double>> t64;
@@ -66,10 +70,20 @@ struct spk_problem {
 };
 
 struct spk_batch {
-    spk_ctx *ctx = nullptr;
+    spk_ctx *ctx = nullptr;  // holds a reference, as does every entry of heldProbs
+    explicit spk_batch(spk_ctx *c) : ctx(spk::retain(c)) {}
+    ~spk_batch();
+    spk_batch(const spk_batch &) = delete;
+    spk_batch &operator=(const spk_batch &) = delete;
     int dtype = SPK_F64;
     int count = 0;
     std::vector<spk_problem *> probs;
+    std::vector<spk_problem *> heldProbs;  // the distinct problems of probs, one reference each
+    void holdProblems() {
+        for (auto *p : probs)
+            if (p && std::find(heldProbs.begin(), heldProbs.end(), p) == heldProbs.end())
+                heldProbs.push_back(spk::retain(p));
+    }
     std::vector<size_t> coreOff, andOff, x0Off;  // element offsets of each instance's regions
     size_t coreElems = 0, andElems = 0, x0Elems = 0;
     int mAlloc = 0;
@@ -351,8 +365,10 @@ void launchMaybeGrid(spk_batch *b, K kernel, int bs, size_t shm, A... args) {
     void *argv[] = {static_cast<void *>(&args)..., static_cast<void *>(&grid)};
     const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(g), dim3(bs), argv, (unsigned) shm, b->ctx->stream);
     // a refused launch queues nothing: read the error back so a later launch check does not see it
-    if (e != hipSuccess) (void) hipGetLastError();
-    check(e, "cooperative launch");
+    if (e != hipSuccess) {
+        (void) hipGetLastError();
+        throw spk::CoopRefused(std::string("cooperative launch: ") + hipGetErrorString(e));
+    }
 }
 
 template <typename T>
@@ -645,9 +661,7 @@ int spk_ctx_create(int device, spk_ctx **out) {
 }
 
 int spk_ctx_destroy(spk_ctx *ctx) {
-    if (!ctx) return 0;
-    if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
-    delete ctx;
+    spk::releaseCtx(ctx);
     return 0;
 }
 
@@ -670,8 +684,8 @@ int spk_ctx_synchronize(spk_ctx *ctx) {
 int spk_problem_load(spk_ctx *ctx, const char *folder, spk_problem **out) {
     *out = nullptr;
     try {
-        auto p = std::make_unique<spk_problem>();
-        p->ctx = ctx;
+        if (!ctx) throw std::runtime_error("null context");
+        auto p = std::make_unique<spk_problem>(ctx);
         p->hp = spk::loadProblem(folder);
         validateProblem(p->hp);
         check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -688,7 +702,7 @@ int spk_problem_load(spk_ctx *ctx, const char *folder, spk_problem **out) {
 }
 
 int spk_problem_destroy(spk_problem *p) {
-    delete p;
+    spk::release(p, [](spk_problem *q) { delete q; });
     return 0;
 }
 
@@ -713,14 +727,16 @@ int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int 
     try {
         if (count <= 0) throw std::runtime_error("batch count must be positive");
         if (dtype != SPK_F64 && dtype != SPK_F32) throw std::runtime_error("dtype must be SPK_F64 or SPK_F32");
-        auto b = std::make_unique<spk_batch>();
-        b->ctx = ctx;
+        if (!ctx) throw std::runtime_error("null context");
+        auto b = std::make_unique<spk_batch>(ctx);
         b->dtype = dtype;
         b->count = count;
         b->probs.assign(problems, problems + count);
+        for (auto *p : b->probs)
+            if (!p) throw std::runtime_error("null problem in batch");
+        b->holdProblems();
         check(hipSetDevice(ctx->device), "hipSetDevice");
         for (auto *p : b->probs) {
-            if (!p) throw std::runtime_error("null problem in batch");
             if (dtype == SPK_F64) ensureUploaded<double>(p);
             else ensureUploaded<float>(p);
         }
@@ -759,13 +775,14 @@ int spk_batch_create_shared(spk_ctx *ctx, spk_problem *problem, int count, int d
         if (!problem) throw std::runtime_error("null problem");
         if (dtype != SPK_F64 && dtype != SPK_F32) throw std::runtime_error("dtype must be SPK_F64 or SPK_F32");
         validateLanes(problem->hp, dtype);
-        auto b = std::make_unique<spk_batch>();
-        b->ctx = ctx;
+        if (!ctx) throw std::runtime_error("null context");
+        auto b = std::make_unique<spk_batch>(ctx);
         b->dtype = dtype;
         b->count = count;
         b->lanes = true;
         b->Bpad = ((long) count + 63) / 64 * 64;
         b->probs.assign(count, problem);
+        b->holdProblems();
         check(hipSetDevice(ctx->device), "hipSetDevice");
         if (dtype == SPK_F64) ensureUploaded<double>(problem);
         else ensureUploaded<float>(problem);
@@ -785,13 +802,19 @@ int spk_batch_create_shared(spk_ctx *ctx, spk_problem *problem, int count, int d
 
 int spk_batch_is_shared(const spk_batch *b) { return b && b->lanes ? 1 : 0; }
 
+// Waits for the batch's work, frees its graph and events and drops its references; the device
+// buffers (members) are freed after this body.
+spk_batch::~spk_batch() {
+    if (ctx && ctx->stream) (void) hipStreamSynchronize(ctx->stream);
+    if (graphExec) (void) hipGraphExecDestroy(graphExec);
+    if (graph) (void) hipGraphDestroy(graph);
+    if (ev0) (void) hipEventDestroy(ev0);
+    if (ev1) (void) hipEventDestroy(ev1);
+    for (auto *p : heldProbs) spk::release(p, [](spk_problem *q) { delete q; });
+    spk::releaseCtx(ctx);
+}
+
 int spk_batch_destroy(spk_batch *b) {
-    if (!b) return 0;
-    if (b->ctx) (void) hipStreamSynchronize(b->ctx->stream);
-    if (b->graphExec) (void) hipGraphExecDestroy(b->graphExec);
-    if (b->graph) (void) hipGraphDestroy(b->graph);
-    if (b->ev0) (void) hipEventDestroy(b->ev0);
-    if (b->ev1) (void) hipEventDestroy(b->ev1);
     delete b;
     return 0;
 }

@@ -6,7 +6,9 @@
 #include <stdexcept>
 #include <string>
 
-struct spk_ctx {
+#include "handles.hpp"
+
+struct spk_ctx : spk::Counted {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -14,6 +16,17 @@ struct spk_ctx {
 };
 
 namespace spk {
+
+// Frees a context whose last reference was dropped: its stream, then the object.
+inline void freeCtx(spk_ctx *c) {
+    if (c->stream) {
+        (void) hipStreamSynchronize(c->stream);
+        (void) hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+inline void releaseCtx(spk_ctx *c) { release(c, freeCtx); }
 
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -47,9 +60,15 @@ struct DevBuf {
     }
 };
 
+// A cooperative launch the device refused; it queued nothing and can be retried with one
+// workgroup (the C ABI returns SPK_ERR_COOPERATIVE_LAUNCH for it).
+struct CoopRefused : HipError {
+    using HipError::HipError;
+};
+
 inline int fail(spk_ctx *ctx, const std::exception &e) {
     if (ctx) ctx->err = e.what();
-    return -1;
+    return dynamic_cast<const CoopRefused *>(&e) ? -2 : -1;
 }
 
 }  // namespace spk

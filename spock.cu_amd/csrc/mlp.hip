@@ -245,7 +245,14 @@ using spk::fail;
 namespace M = spk::mlp;
 
 struct spk_mlp {
-    spk_ctx *ctx = nullptr;
+    spk_ctx *ctx = nullptr;  // holds a reference
+    explicit spk_mlp(spk_ctx *c) : ctx(spk::retain(c)) {}
+    ~spk_mlp() {
+        if (ctx && ctx->stream) (void) hipStreamSynchronize(ctx->stream);
+        spk::releaseCtx(ctx);
+    }
+    spk_mlp(const spk_mlp &) = delete;
+    spk_mlp &operator=(const spk_mlp &) = delete;
     std::vector<int> dims;          // in, hidden..., out
     std::vector<long> wOff, bOff;   // parameter offsets (weights [d_l][d_l+1] row-major, then bias)
     long nParams = 0;
@@ -408,8 +415,8 @@ extern "C" {
 int spk_mlp_create(spk_ctx *ctx, int in, int hidden, int layers, int out, unsigned long long seed, spk_mlp **res) {
     try {
         if (in <= 0 || hidden <= 0 || layers < 0 || out <= 0) throw std::runtime_error("bad MLP dimensions");
-        auto m = std::make_unique<spk_mlp>();
-        m->ctx = ctx;
+        if (!ctx) throw std::runtime_error("null context");
+        auto m = std::make_unique<spk_mlp>(ctx);
         check(hipSetDevice(ctx->device), "hipSetDevice");
         m->dims.push_back(in);
         for (int l = 0; l < layers; l++) m->dims.push_back(hidden);
@@ -460,8 +467,6 @@ int spk_mlp_create(spk_ctx *ctx, int in, int hidden, int layers, int out, unsign
 }
 
 int spk_mlp_destroy(spk_mlp *m) {
-    if (!m) return 0;
-    if (m->ctx) (void) hipStreamSynchronize(m->ctx->stream);
     delete m;
     return 0;
 }

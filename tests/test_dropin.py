@@ -56,14 +56,28 @@ int main(int argc, char **argv) {
     std::unique_ptr<Cache<double>> cache = builder.tol(1e-3).maxIters(3000).make_unique();
     std::vector<double> x0 = parseFromFile<double>(tree.path() + "initialState" + tree.fpFileExt());
     const int status = cache->runSpock(x0);
-    std::vector<double> p = cache->solutionPrim(), d = cache->solutionDual(), z = cache->solution();
-    std::vector<double> g = cache->getDual(p);
+    // the reference's way: device tensors returned by reference, downloaded (src/cache.cuh:371-382)
+    DTensor<double> &sol = cache->solution();
+    std::vector<double> z;
+    sol.download(z);
+    std::vector<double> p = cache->solutionPrim(), d = cache->solutionDual();
+    DTensor<double> prim(p);
+    DTensor<double> dual = cache->getDual(prim);
+    std::vector<double> g;
+    dual.download(g);
     double diff = 0., scale = 0.;
     for (size_t k = 0; k < d.size(); k++) {
         diff = std::fmax(diff, std::fabs(g[k] - d[k]));
         scale = std::fmax(scale, std::fabs(d[k]));
     }
-    std::printf("%d %zu %zu %zu %zu %.3e\n", status, p.size(), d.size(), z.size(), cache->states().size(), diff / scale);
+    std::vector<double> xs = cache->states(), us = cache->inputs();
+    std::printf("%d %zu %zu %zu %zu %.3e %zu\n%s\n", status, p.size(), d.size(), z.size(), xs.size(), diff / scale,
+                sol.numEl(), toString(status));
+    FILE *f = std::fopen(argv[2], "wb");
+    std::fwrite(z.data(), sizeof(double), z.size(), f);
+    std::fwrite(xs.data(), sizeof(double), xs.size(), f);
+    std::fwrite(us.data(), sizeof(double), us.size(), f);
+    std::fclose(f);
     return 0;
 }
 '''
@@ -127,10 +141,25 @@ def test_drop_in_accessors(tmp_path):
     _problem_with_initial_state(folder)
     exe = tmp_path / "accessors"
     _compile(ACCESSORS, exe, tmp_path)
-    out = subprocess.run([str(exe), folder + "/"], check=True, capture_output=True, text=True, timeout=120).stdout.split()
-    status, n_prim, n_dual, n_iter, n_states, rel = int(out[0]), int(out[1]), int(out[2]), int(out[3]), int(out[4]), float(out[5])
-    assert status == 0 and n_prim + n_dual == n_iter and n_states == 31 * 3
+    dump = tmp_path / "solution.bin"
+    out = subprocess.run([str(exe), folder + "/", str(dump)], check=True, capture_output=True, text=True,
+                         timeout=120).stdout.split("\n")
+    head = out[0].split()
+    status, n_prim, n_dual, n_iter, n_states, rel, n_el = (int(head[0]), int(head[1]), int(head[2]), int(head[3]),
+                                                            int(head[4]), float(head[5]), int(head[6]))
+    assert status == 0 and n_prim + n_dual == n_iter == n_el and n_states == 31 * 3
+    assert out[1] == "Converged!"  # the reference's string (src/cache.cuh:75-88)
     assert rel < 1e-2
+    # states()/inputs(): the solution's x and u divided by the preconditioning scaling
+    # (src/cache.cuh:386-412)
+    from oracle.oracle import Oracle
+    o = Oracle(folder)
+    vals = np.fromfile(str(dump), dtype=np.float64)
+    z, xs, us = vals[:n_iter], vals[n_iter:n_iter + o.sX], vals[n_iter + o.sX:]
+    nx, nu = o.nx, o.info["numInputs"]
+    assert us.size == o.sU and o.scaling is not None
+    assert np.array_equal(xs, z[o.sU:o.sU + o.sX] / np.tile(o.scaling[:nx], o.sX // nx))
+    assert np.array_equal(us, z[:o.sU] / np.tile(o.scaling[nx:nx + nu], o.sU // nu))
 
 
 @needs_lib
@@ -148,6 +177,31 @@ def test_reference_driver_on_the_229_node_problem(tmp_path):
     run = subprocess.run([str(exe), folder + "/"], cwd=str(tmp_path), capture_output=True, text=True, timeout=170)
     print(run.stdout)
     assert run.returncode == 0 and "Saved" in run.stdout
+    # which launch shape the Cache used: the automatic 64-workgroup grid mode, with no fallback
+    exe2 = tmp_path / "gridmode"
+    _compile(GRIDMODE, exe2, tmp_path)
+    out = subprocess.run([str(exe2), folder + "/"], capture_output=True, text=True, timeout=120)
+    print(out.stdout)
+    assert out.returncode == 0
+    assert out.stdout.split() == ["0", "64", "0"]
+
+
+GRIDMODE = r'''
+#include <cstdio>
+#include "spock_amd.hpp"
+using namespace spock_amd;
+int main(int argc, char **argv) {
+    ScenarioTree<double> tree(argv[1]);
+    ProblemData<double> problem(tree);
+    CacheBuilder<double> builder(tree, problem);
+    Cache<double> cache = builder.tol(1e-3).maxIters(400).build();
+    std::vector<double> x0 = parseFromFile<double>(tree.path() + "initialState" + tree.fpFileExt());
+    const int status = cache.runSpock(x0);
+    std::printf("%d %d %d\n", status == converged || status == outOfIter ? 0 : 1, cache.workgroups(),
+                (int) cache.gridFallback());
+    return 0;
+}
+'''
 
 
 @needs_lib

@@ -1,7 +1,9 @@
 """The drop-in Cache's grid-mode fallback (include/spock_amd.hpp), without a GPU: the header is
 compiled against a stub of the C ABI whose solve refuses any launch of more than one workgroup,
-as a refused cooperative launch does.  The size-based grid choice must then solve with one
-workgroup, and an explicitly requested workgroup count must report the refusal."""
+as a refused cooperative launch does (SPK_ERR_COOPERATIVE_LAUNCH).  The size-based grid choice
+must then solve with one workgroup and say so (gridFallback), any other error must not be retried,
+an explicitly requested workgroup count must report the refusal, and iterates outside the size
+range grid mode has run on stay on one workgroup."""
 import os
 import shutil
 import subprocess
@@ -43,11 +45,16 @@ int spk_batch_destroy(spk_batch *) { return 0; }
 int spk_batch_set_workgroups(spk_batch *b, int n) { b->wgs = n; std::printf("workgroups %d\n", n); return 0; }
 int spk_batch_set_x0(spk_batch *, const double *) { return 0; }
 int spk_batch_set_iterate(spk_batch *, int, const double *) { return 0; }
+int g_other_error = 0;  // set by the driver: fail every solve with a plain error
 int spk_batch_solve(spk_batch *b, const spk_options *, int) {
     std::printf("solve %d\n", b->wgs);
+    if (g_other_error) {
+        std::snprintf(b->ctx->err, sizeof(b->ctx->err), "hipMalloc: out of memory");
+        return SPK_ERR;
+    }
     if (b->wgs > 1) {
         std::snprintf(b->ctx->err, sizeof(b->ctx->err), "cooperative launch: the operation cannot be performed in the present state");
-        return 1;
+        return SPK_ERR_COOPERATIVE_LAUNCH;
     }
     return 0;
 }
@@ -64,10 +71,12 @@ MAIN = r'''
 #include <cstdlib>
 #include "spock_amd.hpp"
 extern long g_prim;
+extern int g_other_error;
 using namespace spock_amd;
 int main(int argc, char **argv) {
     g_prim = std::atol(argv[2]);
     const int wgs = std::atoi(argv[3]);
+    g_other_error = argc > 4 ? std::atoi(argv[4]) : 0;
     ScenarioTree<double> tree(argv[1]);
     ProblemData<double> data(tree);
     CacheBuilder<double> builder(tree, data);
@@ -76,7 +85,8 @@ int main(int argc, char **argv) {
     try {
         for (int k = 0; k < 2; k++) {
             const int status = cache.runSpock(x0);
-            std::printf("status %d iters %zu\n", status, cache.solveIter());
+            std::printf("status %d iters %zu workgroups %d fallback %d\n", status, cache.solveIter(),
+                        cache.workgroups(), (int) cache.gridFallback());
         }
     } catch (const std::invalid_argument &e) {
         std::printf("error %s\n", e.what());
@@ -101,21 +111,35 @@ def driver(tmp_path_factory):
     return str(exe), str(d / "data")
 
 
-def run(driver, prim, wgs):
+def run(driver, prim, wgs, other_error=0):
     exe, folder = driver
-    out = subprocess.run([exe, folder, str(prim), str(wgs)], check=True, capture_output=True, text=True, timeout=60)
+    out = subprocess.run([exe, folder, str(prim), str(wgs), str(other_error)], check=True, capture_output=True,
+                         text=True, timeout=60)
     return out.stdout.strip().split("\n")
 
 
 def test_size_based_grid_falls_back_to_one_workgroup(driver):
     # an iterate of 60,000 elements: the Cache picks 64 workgroups, the launch is refused, the
     # solve is repeated with one workgroup, and later solves use one workgroup directly
-    assert run(driver, 30000, 0) == ["workgroups 64", "solve 64", "workgroups 1", "solve 1", "status 0 iters 7",
-                                     "solve 1", "status 0 iters 7"]
+    assert run(driver, 30000, 0) == ["workgroups 64", "solve 64", "workgroups 1", "solve 1",
+                                     "status 0 iters 7 workgroups 1 fallback 1",
+                                     "solve 1", "status 0 iters 7 workgroups 1 fallback 1"]
 
 
 def test_small_iterates_use_one_workgroup(driver):
-    assert run(driver, 1000, 0) == ["workgroups 1", "solve 1", "status 0 iters 7", "solve 1", "status 0 iters 7"]
+    assert run(driver, 1000, 0) == ["workgroups 1", "solve 1", "status 0 iters 7 workgroups 1 fallback 0",
+                                    "solve 1", "status 0 iters 7 workgroups 1 fallback 0"]
+
+
+def test_large_iterates_stay_on_one_workgroup(driver):
+    # 2.9M elements (the 5,461-node tree): outside the range grid mode has run on, so no automatic grid
+    assert run(driver, 1500000, 0) == ["workgroups 1", "solve 1", "status 0 iters 7 workgroups 1 fallback 0",
+                                       "solve 1", "status 0 iters 7 workgroups 1 fallback 0"]
+
+
+def test_other_errors_are_not_retried(driver):
+    lines = run(driver, 30000, 0, other_error=1)
+    assert lines == ["workgroups 64", "solve 64", "error [runSpock] hipMalloc: out of memory"]
 
 
 def test_requested_workgroups_report_the_refusal(driver):

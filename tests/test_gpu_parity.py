@@ -277,3 +277,57 @@ def test_workgroup_autotune_single_instance(ctx, problem_dirs):
     ref = o.solve(x0, tol_abs=1e-30, max_iters=50)
     assert res[0]["iters"] == ref["iters"]
     assert rel_err(b.iterate(0), ref["z"]) < 1e-9
+
+
+@pytest.mark.parametrize("name,shared", [("small", False), ("mid", False), ("poly", False), ("small", True),
+                                         ("mid", True)])
+def test_warm_start_matches_oracle(ctx, problem_dirs, name, shared):
+    """runSpock(x0, &previousSolution) (src/cache.cuh:611-632: the previous solution becomes both
+    the iterate and the previous iterate, saveToPrev): 50 SPOCK iterations from a warm start
+    (the oracle's iterate after 100 iterations from another initial state) against the oracle's
+    warm-started solve, iterates to 1e-9 and identical branch counts.  Shared batches run the
+    Gram least squares and adjoint-only dot products, and the oracle the same way."""
+    d = problem_dirs[name]
+    o = Oracle(d)
+    x0a, x0b = x0_for(name, o.nx, 0), x0_for(name, o.nx, 1)
+    kw = dict(ls="gram", adjoint_dots=True) if shared else {}
+    z_prev = o.solve(x0a, tol_abs=1e-30, max_iters=100, **kw)["z"]
+    ref = o.solve(x0b, warm=z_prev, tol_abs=1e-30, max_iters=50, **kw)
+    p = rt.Problem(ctx, d)
+    b = rt.Batch(ctx, [p] * (2 if shared else 1), rt.F64, shared=shared)
+    b.set_x0([x0b] * (2 if shared else 1))
+    b.set_iterates(np.concatenate([z_prev] * (2 if shared else 1)))
+    b.solve(rt.options(tol_abs=1e-30, max_iters=50, block_size=256 if not shared else 0), warm=True)
+    res, _ = b.results()
+    r = res[0]
+    assert r["status"] == ref["status"] == 1
+    assert r["iters"] == ref["iters"]
+    assert (r["k0"], r["k1"], r["k2"], r["k2bt"], r["k3"]) == (ref["k0"], ref["k1"], ref["k2"], ref["k2bt"], ref["k3"])
+    assert rel_err(b.iterate(0), ref["z"]) < 1e-9
+
+
+def test_gpu_matches_committed_golden_vectors(ctx, tmp_path):
+    """The HIP path against the committed fixture vectors (tests/golden/fixture_golden.npz, the
+    oracle's outputs on the reference's unit-test problem): L, L*, T and 75 SPOCK iterations,
+    without running the oracle."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fixture_golden.npz"),
+                allow_pickle=False)
+    d = str(tmp_path / "fixture")
+    configs.generate("fixture", d)
+    p = rt.Problem(ctx, d)
+    b = rt.Batch(ctx, [p], rt.F64)
+    b.set_x0([g["x0"]])
+    n_prim = g["prim"].size
+    z = np.concatenate([g["prim"], g["dual"]])
+    out, _ = b.debug_op(rt.OP_L, z)
+    assert rel_err(out[n_prim:], g["L_prim"]) < 1e-13
+    out, _ = b.debug_op(rt.OP_LADJ, z)
+    assert rel_err(out[:n_prim], g["Ladj_dual"]) < 1e-13
+    out, _ = b.debug_op(rt.OP_T, g["z"])
+    assert rel_err(out, g["T_z"]) < 1e-12
+    b.solve(rt.options(tol_abs=1e-30, max_iters=75, block_size=256))
+    r = b.results()[0][0]
+    assert r["iters"] == int(g["spock75_iters"])
+    assert [r[k] for k in ("k0", "k1", "k2", "k2bt", "k3")] == [int(v) for v in g["spock75_counts"]]
+    assert rel_err(b.iterate(0), g["spock75"]) < 1e-9
