@@ -112,15 +112,16 @@ int spk_problem_scaling(const spk_problem *p, double *out);
 double spk_problem_step_size(const spk_problem *p);
 
 int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int dtype, spk_batch **out);
-/* `count` instances of ONE problem (e.g. many initial states of one tree) in the
- * block-interleaved layout of the lane engine: instances form blocks of 64 and element e of
- * instance k is stored at ((k / 64) * n + e) * 64 + k % 64 (n = size_iterate); a wave's 64
- * lanes are the 64 instances of one block and the factor tables are read once per wave.  Every spk_batch_* call below accepts such a batch; the
- * Anderson least squares is the FP64 Gram solve (ls_method is ignored), the dynamics sweep
- * applies the precomputed (L L')^-1 (dyn_inverse is ignored), anderson_buffer <= 4,
- * numInputs <= 32, numEvents <= 7.  block_size is the workgroup size (64..1024; when <= 0 the
- * size chosen by spk_batch_autotune, else 1024); one workgroup serves 64 instances.  No counterpart in the reference, which
- * solves one instance per Cache. */
+/* `count` instances of ONE problem (e.g. many initial states of one tree) in a block-interleaved
+ * layout solved in lockstep: instances form blocks of 16 (the default tile engine) or 64 (the lane
+ * engine, see spk_batch_set_schedule) and element e of instance k is stored at
+ * ((k / w) * n + e) * w + k % w (n = size_iterate, w the block width); one workgroup serves one
+ * block and the factor tables are shared by its instances.  Every spk_batch_* call below accepts
+ * such a batch; the Anderson least squares is the FP64 Gram solve (ls_method is ignored), the
+ * dynamics sweep applies the precomputed (L L')^-1 (dyn_inverse is ignored), anderson_buffer <= 4,
+ * numInputs <= 32, numEvents <= 7.  block_size is the workgroup size (when <= 0 the size chosen by
+ * spk_batch_autotune, else the engine's default).  No counterpart in the reference, which solves
+ * one instance per Cache. */
 int spk_batch_create_shared(spk_ctx *ctx, spk_problem *problem, int count, int dtype, spk_batch **out);
 int spk_batch_is_shared(const spk_batch *b);
 int spk_batch_destroy(spk_batch *b);
@@ -154,7 +155,7 @@ int spk_batch_get_iterates(spk_batch *b, double *z);
 int spk_batch_set_iterates_device(spk_batch *b, const float *src, long ld);
 int spk_batch_get_iterates_device(spk_batch *b, float *dst, long ld);
 /* device pointer to instance k's accepted iterate (dtype of the batch); contiguous for
- * per-instance batches, stride 64 elements for shared batches */
+ * per-instance batches, stride w elements (the block width, 16 or 64) for shared batches */
 void *spk_batch_iterate_ptr(spk_batch *b, int k);
 /* copy iterates between batches of the same problems (converting dtype) on the device */
 int spk_batch_copy_iterates(spk_batch *dst, const spk_batch *src);
@@ -164,15 +165,40 @@ int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm);
  * continued iterates, branch decisions and counters are those of one uninterrupted solve.  Results
  * after a chunk report SPK_NOT_RUN (and the iterations so far) for instances still running. */
 int spk_batch_resume(spk_batch *b);
-/* shared batches: the lane kernels' schedule.  mv = 0 reads factor entries with scalar loads,
- * mv = 8 stages them through LDS in 8-column chunks (vector loads, one chunk prefetched); both give
- * bitwise identical iterates.  block = workgroup size (0: 1024).  No counterpart in the reference. */
+/* Continuous batching of a shared batch (no counterpart in the reference, which solves one
+ * instance per Cache): between the launches of a chunked solve, start new solves in `count`
+ * distinct instance slots `lanes` (normally slots whose solve has ended; a running solve in a
+ * listed slot is abandoned).  Each gets the unscaled initial state x0[i] (nx values), a fresh
+ * SuperMann state and its own iteration count, and starts at the next spk_batch_resume while the
+ * other slots continue; its checks, caps and results count from its own first iteration.  The slot's
+ * previous result (spk_batch_get_iterate, spk_batch_results) must be read before the refill.
+ *   spk_batch_refill:            cold starts (the zero iterate; previous iterate and residual ones)
+ *   spk_batch_refill_from_batch: warm starts from the accepted iterates of slots src_lanes of the
+ *                                shared batch src (same problem size, either dtype), e.g. the FP64
+ *                                polish of FP32 solutions
+ *   spk_batch_refill_rows:       warm starts from FP32 device rows z (count rows, leading dimension ld)
+ * (a warm start's previous iterate is the starting iterate and its previous residual zero, as
+ * initialisePrev with a previous solution, src/cache.cuh:611-632). */
+int spk_batch_refill(spk_batch *b, int count, const int *lanes, const double *x0);
+int spk_batch_refill_from_batch(spk_batch *b, int count, const int *lanes, const double *x0, spk_batch *src,
+                                const int *src_lanes);
+int spk_batch_refill_rows(spk_batch *b, int count, const int *lanes, const double *x0, const float *z, long ld);
+/* shared batches: the engine and schedule.  mv = 16 (default): the tile engine, 16 instances per
+ * workgroup in blocks of 16 (element e of instance k at ((k / 16) * n + e) * 16 + k % 16) with the
+ * per-node products on the matrix cores (v_mfma 16x16x4, the 16 instances as the B operand), block
+ * 64..512 (0: 256).  mv = 0 or 8: the lane engine, 64 instances per workgroup in blocks of 64 with
+ * the factor entries read by scalar loads (0) or staged through LDS in 8-column chunks (8), which
+ * give bitwise identical iterates, block 64..1024 (0: 1024).  Switching between the engines re-lays
+ * out the initial states and accepted iterates and clears the workspace.  No counterpart in the
+ * reference. */
 int spk_batch_set_schedule(spk_batch *b, int mv, int block);
-/* shared batches: time `iters` cold outer iterations (from the batch's initial states) under each
- * schedule {mv 0, 8} x {block 1024, 512}, keep the fastest for the batch's later solves and return it
- * (mv, block: nullable; seconds: nullable, 4 kernel times in that order).  The iterates are reset
- * (initial states and given tolerances are kept).  opts supplies the solver options of the timed
- * runs (nullable: defaults). */
+/* shared batches: time `iters` cold outer iterations (from the batch's initial states) under each of
+ * the SPK_AUTOTUNE_CANDIDATES schedules (mv, block) = (0, 1024), (8, 1024), (0, 512), (8, 512),
+ * (16, 256), (16, 512), keep the fastest for the batch's later solves and return it (mv, block:
+ * nullable; seconds: nullable, the kernel times in that order).  The iterates are reset (initial
+ * states and given tolerances are kept).  opts supplies the solver options of the timed runs
+ * (nullable: defaults). */
+enum { SPK_AUTOTUNE_CANDIDATES = 6 };
 int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv, int *block, double *seconds);
 /* shared batches: a chunked solve (opts->chunk_iters > 0) captured as one HIP graph of `launches`
  * kernel launches (the first and launches - 1 resumes) and replayed with no host round trip between
@@ -230,6 +256,9 @@ int spk_mlp_dataset(spk_mlp *m, spk_batch *solved, const double *x0);
 int spk_mlp_train(spk_mlp *m, int epochs, int minibatch, float lr, unsigned long long seed, double *epoch_loss);
 /* warm-start iterates of `rows` initial states (host, rows x in) written into dst (rows = its count) */
 int spk_mlp_predict(spk_mlp *m, const double *x0, int rows, spk_batch *dst);
+/* continuous batching: new solves in slots `lanes` of the shared batch dst (see spk_batch_refill),
+ * warm-started from the network's prediction for their initial states x0 (host, rows x in) */
+int spk_mlp_predict_lanes(spk_mlp *m, const double *x0, int rows, spk_batch *dst, const int *lanes);
 
 #ifdef __cplusplus
 }

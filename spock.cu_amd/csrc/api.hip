@@ -33,6 +33,18 @@ template <typename T>
 __global__ void k_lane_solve_x8(const LaneInst<T> *, Out *);
 template <typename T>
 __global__ void k_lane_debug_x8(const LaneInst<T> *, Out *, int);
+template <typename T>
+__global__ void k_tile_solve(const LaneInst<T> *, Out *);
+template <typename T>
+__global__ void k_tile_debug(const LaneInst<T> *, Out *, int);
+template <typename T>
+__global__ void k_relayout(const T *, long, T *, long, long, long);
+template <typename T, typename Z>
+__global__ void k_lane_refill(const LaneInst<T> *, const int *, int, const Z *, long, const T *, long);
+template <typename T>
+__global__ void k_lane_refill_groups(const LaneInst<T> *, const int *, int, long);
+template <typename S, typename D>
+__global__ void k_gather_lanes(const S *, long, D *, long, long, const int *, int);
 template <typename S, typename D>
 __global__ void k_interleave(const S *, long, D *, long, int, long);
 template <typename S, typename D>
@@ -95,6 +107,7 @@ struct spk_batch {
     // shared-problem batch in the instance-interleaved layout (lane engine)
     bool lanes = false;
     long Bpad = 0;     // count rounded up to 64
+    int laneW = 16;    // instances per block of the interleaved layout: 16 (tile engine) or 64 (lane engine)
     DevBuf lbuf, ltol; // interleaved state; per-instance given tolerances
     DevBuf lsave, lgsave;   // solver state kept between the launches of a chunked solve
     spk_options lastOpts{};
@@ -102,8 +115,9 @@ struct spk_batch {
     bool resumable = false;
     size_t lRegion[16] = {};  // element offsets: work[0..8], zio, rio, hS, hY, qd, dd, x0
     static constexpr int LANE_MAXM = 4;
-    int laneMv = 0;      // matrix-vector schedule of the lane kernels: 0 scalar loads, 8 LDS-staged
+    int laneMv = 16;     // schedule: 16 the tile engine (MFMA); 0, 8 the lane engine (scalar loads, LDS-staged)
     DevBuf dInstResume;  // the lane instance of the resume launches of a captured chunked solve
+    DevBuf rLanes, rX0, rZ;  // staging of a refill: lane indices, scaled initial states, starting iterates
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     int laneBlockSz = 0; // workgroup size chosen by spk_batch_autotune (0: none)
@@ -479,18 +493,57 @@ void buildLaneInst(spk_batch *b, const spk_options &o, int warm, int resume = 0)
           "upload lane instance");
 }
 
+// Workgroup size of the shared-batch kernels: the caller's, else the autotuner's, else the
+// engine's default (tile engine 256, at most 512; lane engine 1024).
 int laneBlock(const spk_batch *b, const spk_options &o) {
-    const int bs = o.block_size > 0 ? o.block_size : (b->laneBlockSz > 0 ? b->laneBlockSz : 1024);
-    return std::max(64, std::min(1024, bs / 64 * 64));
+    const bool tile = b->laneW == 16;
+    const int bs = o.block_size > 0 ? o.block_size : (b->laneBlockSz > 0 ? b->laneBlockSz : (tile ? 256 : 1024));
+    return std::max(64, std::min(tile ? 512 : 1024, bs / 64 * 64));
 }
 
 template <typename T>
 void launchLaneKernel(spk_batch *b, const spk_options &o, const void *inst = nullptr) {
-    const dim3 grid((unsigned) (b->Bpad / 64)), block(laneBlock(b, o));
+    const dim3 grid((unsigned) (b->Bpad / b->laneW)), block(laneBlock(b, o));
     auto *P = static_cast<const spk::LaneInst<T> *>(inst ? inst : b->dInst.p);
     auto *outs = static_cast<spk::Out *>(b->dOut.p);
-    if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_solve_x8<T>, grid, block, 0, b->ctx->stream, P, outs);
+    if (b->laneW == 16) hipLaunchKernelGGL(spk::k_tile_solve<T>, grid, block, 0, b->ctx->stream, P, outs);
+    else if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_solve_x8<T>, grid, block, 0, b->ctx->stream, P, outs);
     else hipLaunchKernelGGL(spk::k_lane_solve<T>, grid, block, 0, b->ctx->stream, P, outs);
+}
+
+// Re-lays the batch's inputs and outputs (initial states, accepted iterates and their residuals)
+// out for blocks of w instances and clears the workspace regions; a chunked solve cannot be
+// resumed across the change.
+template <typename T>
+void relayoutRegions(spk_batch *b, int w) {
+    const HostProblem &h = b->probs[0]->hp;
+    hipStream_t st = b->ctx->stream;
+    const int regions[3] = {LR_ZIO, LR_RIO, LR_X0};
+    const long rows[3] = {(long) h.sIter, (long) h.sIter, (long) h.nx};
+    DevBuf tmp;
+    tmp.alloc((size_t) h.sIter * b->Bpad * sizeof(T));
+    for (int r = 0; r < 3; r++) {
+        T *reg = lanePtr<T>(b, regions[r]);
+        const size_t bytes = (size_t) rows[r] * b->Bpad * sizeof(T);
+        hipLaunchKernelGGL(spk::k_relayout<T>, dim3(1024), dim3(256), 0, st, reg, (long) b->laneW, tmp.as<T>(), (long) w,
+                           rows[r], b->Bpad);
+        check(hipGetLastError(), "launch k_relayout");
+        check(hipMemcpyAsync(reg, tmp.p, bytes, hipMemcpyDeviceToDevice, st), "relayout copy");
+    }
+    for (int r = 0; r < LR_X0; r++)  // regions are contiguous in this order, X0 last
+        if (r != LR_ZIO && r != LR_RIO)
+            check(hipMemsetAsync(lanePtr<T>(b, r), 0, (b->lRegion[r + 1] - b->lRegion[r]) * sizeof(T), st),
+                  "clear workspace");
+    check(hipStreamSynchronize(st), "sync");
+    b->laneW = w;
+    b->resumable = false;
+}
+
+void setLaneWidth(spk_batch *b, int w) {
+    if (b->laneW == w) return;
+    check(hipStreamSynchronize(b->ctx->stream), "sync");
+    if (b->dtype == SPK_F64) relayoutRegions<double>(b, w);
+    else relayoutRegions<float>(b, w);
 }
 
 template <typename T>
@@ -543,7 +596,7 @@ template <typename T>
 void laneUpload(spk_batch *b, int region, const double *rows, size_t n) {
     std::vector<T> tmp(n * b->Bpad, T(0));
     for (int k = 0; k < b->count; k++)
-        for (size_t e = 0; e < n; e++) tmp[spk::laneIndex((long) n, (long) e, k)] = (T) rows[(size_t) k * n + e];
+        for (size_t e = 0; e < n; e++) tmp[spk::laneIndex((long) n, (long) e, k, b->laneW)] = (T) rows[(size_t) k * n + e];
     check(hipMemcpy(lanePtr<T>(b, region), tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
 }
 
@@ -552,13 +605,13 @@ void laneDownload(spk_batch *b, int region, double *rows, size_t n) {
     std::vector<T> tmp(n * b->Bpad);
     check(hipMemcpy(tmp.data(), lanePtr<T>(b, region), tmp.size() * sizeof(T), hipMemcpyDeviceToHost), "download");
     for (int k = 0; k < b->count; k++)
-        for (size_t e = 0; e < n; e++) rows[(size_t) k * n + e] = (double) tmp[spk::laneIndex((long) n, (long) e, k)];
+        for (size_t e = 0; e < n; e++) rows[(size_t) k * n + e] = (double) tmp[spk::laneIndex((long) n, (long) e, k, b->laneW)];
 }
 
 template <typename T>
 void laneColumn(spk_batch *b, int region, int k, double *z, const double *zin, size_t n) {
-    const size_t es = sizeof(T), pitch = 64 * es;
-    T *base = lanePtr<T>(b, region) + spk::laneIndex((long) n, 0, k);
+    const size_t es = sizeof(T), pitch = b->laneW * es;
+    T *base = lanePtr<T>(b, region) + spk::laneIndex((long) n, 0, k, b->laneW);
     std::vector<T> tmp(n);
     if (zin) {
         for (size_t e = 0; e < n; e++) tmp[e] = (T) zin[e];
@@ -583,14 +636,14 @@ void launchInterleave(const spk_batch *src, spk_batch *dst, size_t n) {
     const size_t stride = uniformCoreStride(src);
     hipLaunchKernelGGL((spk::k_interleave<S, D>), dim3(1024), dim3(256), 0, dst->ctx->stream,
                        static_cast<const S *>(src->core.p), (long) stride, lanePtr<D>(dst, LR_ZIO), (long) n,
-                       dst->count, dst->Bpad);
+                       dst->count, (long) dst->laneW);
 }
 
 template <typename S, typename D>
 void launchDeinterleave(const spk_batch *src, spk_batch *dst, size_t n) {
     const size_t stride = uniformCoreStride(dst);
     hipLaunchKernelGGL((spk::k_deinterleave<S, D>), dim3(1024), dim3(256), 0, dst->ctx->stream,
-                       lanePtr<S>(src, LR_ZIO), src->Bpad, static_cast<D *>(dst->core.p), (long) stride, (long) n,
+                       lanePtr<S>(src, LR_ZIO), (long) src->laneW, static_cast<D *>(dst->core.p), (long) stride, (long) n,
                        dst->count);
 }
 
@@ -605,10 +658,10 @@ void fillDevice(spk_batch *b, T *p, size_t n) {
 struct TuneGuard {
     spk_batch *b;
     std::vector<double> tol;
-    int wgs, mv, block;
+    int wgs, mv, block, width;
     bool kept = false;
     explicit TuneGuard(spk_batch *bb)
-        : b(bb), tol(bb->tolFixed), wgs(bb->wgs), mv(bb->laneMv), block(bb->laneBlockSz) {
+        : b(bb), tol(bb->tolFixed), wgs(bb->wgs), mv(bb->laneMv), block(bb->laneBlockSz), width(bb->laneW) {
         b->tolFixed.clear();
     }
     void keep() {
@@ -621,8 +674,62 @@ struct TuneGuard {
         b->wgs = wgs;
         b->laneMv = mv;
         b->laneBlockSz = block;
+        if (b->lanes) {
+            try {
+                setLaneWidth(b, width);
+            } catch (const std::exception &) {
+                // the layout stays as the failed timed run left it; the error being reported names the cause
+            }
+        }
     }
 };
+
+// Lane indices of a refill or gather, checked on the host and staged on the device.
+void stageLanes(spk_batch *b, int count, const int *lanes) {
+    if (!b->lanes) throw std::runtime_error("refills and lane gathers apply to shared batches");
+    if (count < 1 || !lanes) throw std::runtime_error("no lanes given");
+    std::vector<char> seen(b->count, 0);
+    for (int i = 0; i < count; i++) {
+        if (lanes[i] < 0 || lanes[i] >= b->count) throw std::runtime_error("lane index out of range");
+        if (seen[lanes[i]]) throw std::runtime_error("a lane is listed twice");
+        seen[lanes[i]] = 1;
+    }
+    if (b->rLanes.bytes < count * sizeof(int)) b->rLanes.alloc(count * sizeof(int));
+    check(hipMemcpyAsync(b->rLanes.p, lanes, count * sizeof(int), hipMemcpyHostToDevice, b->ctx->stream), "lanes");
+}
+
+// New solves in the staged lanes of a resumable shared batch, from the zero iterate (z == null) or
+// from device rows z (leading dimension ld).
+template <typename T, typename Z = float>
+void refillLanes(spk_batch *b, int count, const double *x0, const Z *z, long ld) {
+    if (!b->resumable) throw std::runtime_error("a refill continues a chunked solve (chunk_iters > 0) of the batch");
+    const HostProblem &h = b->probs[0]->hp;
+    std::vector<T> xs((size_t) count * h.nx);
+    for (int i = 0; i < count; i++)
+        for (int k = 0; k < h.nx; k++)
+            xs[(size_t) i * h.nx + k] = (T) (x0[(size_t) i * h.nx + k] * (h.precond ? h.scaling[k] : 1.));  // src/cache.cuh:599-601
+    hipStream_t st = b->ctx->stream;
+    if (b->rX0.bytes < xs.size() * sizeof(T)) b->rX0.alloc(xs.size() * sizeof(T));
+    check(hipMemcpyAsync(b->rX0.p, xs.data(), xs.size() * sizeof(T), hipMemcpyHostToDevice, st), "x0");
+    buildLaneInst<T>(b, b->lastOpts, b->lastWarm, 1);
+    auto *P = static_cast<const spk::LaneInst<T> *>(b->dInst.p);
+    const int *lanes = b->rLanes.as<int>();
+    hipLaunchKernelGGL((spk::k_lane_refill<T, Z>), dim3(1024), dim3(256), 0, st, P, lanes, count, z, ld,
+                       b->rX0.as<T>(), (long) b->laneW);
+    check(hipGetLastError(), "launch k_lane_refill");
+    hipLaunchKernelGGL(spk::k_lane_refill_groups<T>, dim3(16), dim3(256), 0, st, P, lanes, count, (long) b->laneW);
+    check(hipGetLastError(), "launch k_lane_refill_groups");
+    check(hipStreamSynchronize(st), "sync");
+}
+
+// The accepted iterates of the staged lanes as device rows of the batch's dtype (leading dimension ld).
+template <typename T>
+void gatherLanes(spk_batch *b, int count, T *dst, long ld) {
+    const long n = b->probs[0]->hp.sIter;
+    hipLaunchKernelGGL((spk::k_gather_lanes<T, T>), dim3(1024), dim3(256), 0, b->ctx->stream, lanePtr<T>(b, LR_ZIO),
+                       (long) b->laneW, dst, ld, n, b->rLanes.as<int>(), count);
+    check(hipGetLastError(), "launch k_gather_lanes");
+}
 
 }  // namespace
 
@@ -789,7 +896,7 @@ int spk_batch_create_shared(spk_ctx *ctx, spk_problem *problem, int count, int d
         const size_t total = laneLayout(b.get());
         b->lbuf.alloc(total * b->elSize());
         b->lsave.alloc(sizeof(spk::LaneSave) * b->Bpad);
-        b->lgsave.alloc(sizeof(spk::LaneGroupSave) * (b->Bpad / 64));
+        b->lgsave.alloc(sizeof(spk::LaneGroupSave) * (b->Bpad / 16));  // one per workgroup of either engine
         check(hipMemset(b->lbuf.p, 0, total * b->elSize()), "zero state");
         check(hipEventCreate(&b->ev0), "event");
         check(hipEventCreate(&b->ev1), "event");
@@ -988,11 +1095,11 @@ int spk_batch_set_iterates_device(spk_batch *b, const float *src, long ld) {
         if (b->lanes) {
             if (b->dtype == SPK_F64) {
                 hipLaunchKernelGGL((spk::k_interleave<float, double>), dim3(1024), dim3(256), 0, st, src, ld,
-                                   lanePtr<double>(b, LR_ZIO), (long) n, b->count, b->Bpad);
+                                   lanePtr<double>(b, LR_ZIO), (long) n, b->count, (long) b->laneW);
                 fillDevice<double>(b, lanePtr<double>(b, LR_RIO), n * b->Bpad);
             } else {
                 hipLaunchKernelGGL((spk::k_interleave<float, float>), dim3(1024), dim3(256), 0, st, src, ld,
-                                   lanePtr<float>(b, LR_ZIO), (long) n, b->count, b->Bpad);
+                                   lanePtr<float>(b, LR_ZIO), (long) n, b->count, (long) b->laneW);
                 fillDevice<float>(b, lanePtr<float>(b, LR_RIO), n * b->Bpad);
             }
         } else {
@@ -1028,10 +1135,10 @@ int spk_batch_get_iterates_device(spk_batch *b, float *dst, long ld) {
         if (b->lanes) {
             if (b->dtype == SPK_F64)
                 hipLaunchKernelGGL((spk::k_deinterleave<double, float>), dim3(1024), dim3(256), 0, st,
-                                   lanePtr<double>(b, LR_ZIO), b->Bpad, dst, ld, (long) n, b->count);
+                                   lanePtr<double>(b, LR_ZIO), (long) b->laneW, dst, ld, (long) n, b->count);
             else
                 hipLaunchKernelGGL((spk::k_deinterleave<float, float>), dim3(1024), dim3(256), 0, st,
-                                   lanePtr<float>(b, LR_ZIO), b->Bpad, dst, ld, (long) n, b->count);
+                                   lanePtr<float>(b, LR_ZIO), (long) b->laneW, dst, ld, (long) n, b->count);
         } else {
             for (int k = 0; k < b->count; k++) {
                 if (b->dtype == SPK_F64)
@@ -1054,7 +1161,7 @@ void *spk_batch_iterate_ptr(spk_batch *b, int k) {
     if (k < 0 || k >= b->count) return nullptr;
     if (b->lanes)
         return static_cast<char *>(b->lbuf.p) +
-               (b->lRegion[LR_ZIO] + spk::laneIndex(b->probs[0]->hp.sIter, 0, k)) * b->elSize();
+               (b->lRegion[LR_ZIO] + spk::laneIndex(b->probs[0]->hp.sIter, 0, k, b->laneW)) * b->elSize();
     return static_cast<char *>(b->core.p) + b->coreOff[k] * b->elSize();
 }
 
@@ -1164,6 +1271,62 @@ int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm) {
     }
 }
 
+int spk_batch_refill(spk_batch *b, int count, const int *lanes, const double *x0) {
+    try {
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        stageLanes(b, count, lanes);
+        if (b->dtype == SPK_F64) refillLanes<double, float>(b, count, x0, nullptr, 0);
+        else refillLanes<float, float>(b, count, x0, nullptr, 0);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_refill_from_batch(spk_batch *b, int count, const int *lanes, const double *x0, spk_batch *src,
+                                const int *src_lanes) {
+    try {
+        if (!src || !src->lanes || src->probs[0]->hp.sIter != b->probs[0]->hp.sIter)
+            throw std::runtime_error("the source must be a shared batch of the same problem size");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(src->ctx->stream), "sync");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        // the starting iterates pass through rows of the source's dtype, so an FP64 source loses nothing
+        const long n = b->probs[0]->hp.sIter;
+        stageLanes(src, count, src_lanes);
+        if (b->rZ.bytes < (size_t) count * n * src->elSize()) b->rZ.alloc((size_t) count * n * src->elSize());
+        if (src->dtype == SPK_F64) gatherLanes<double>(src, count, b->rZ.as<double>(), n);
+        else gatherLanes<float>(src, count, b->rZ.as<float>(), n);
+        check(hipStreamSynchronize(src->ctx->stream), "sync");
+        stageLanes(b, count, lanes);
+        if (b->dtype == SPK_F64) {
+            if (src->dtype == SPK_F64) refillLanes<double, double>(b, count, x0, b->rZ.as<double>(), n);
+            else refillLanes<double, float>(b, count, x0, b->rZ.as<float>(), n);
+        } else {
+            if (src->dtype == SPK_F64) refillLanes<float, double>(b, count, x0, b->rZ.as<double>(), n);
+            else refillLanes<float, float>(b, count, x0, b->rZ.as<float>(), n);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_refill_rows(spk_batch *b, int count, const int *lanes, const double *x0, const float *z, long ld) {
+    try {
+        if (!z || ld < b->probs[0]->hp.sIter) throw std::runtime_error("starting iterates need rows of the iterate size");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        stageLanes(b, count, lanes);
+        if (b->dtype == SPK_F64) refillLanes<double>(b, count, x0, z, ld);
+        else refillLanes<float>(b, count, x0, z, ld);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
 int spk_batch_resume(spk_batch *b) {
     try {
         if (!b->lanes || !b->resumable) throw std::runtime_error("no chunked solve of a shared batch to resume");
@@ -1209,10 +1372,12 @@ int spk_batch_resume_graph(spk_batch *b, int launches) {
 int spk_batch_set_schedule(spk_batch *b, int mv, int block) {
     try {
         if (!b->lanes) throw std::runtime_error("schedules apply to shared batches");
-        if (mv != 0 && mv != 8) throw std::runtime_error("matrix-vector schedule must be 0 or 8");
-        if (block != 0 && (block < 64 || block > 1024 || block % 64))
-            throw std::runtime_error("block must be 0 or a multiple of 64 in 64..1024");
+        if (mv != 0 && mv != 8 && mv != 16) throw std::runtime_error("schedule must be 0, 8 or 16");
+        const int most = mv == 16 ? 512 : 1024;
+        if (block != 0 && (block < 64 || block > most || block % 64))
+            throw std::runtime_error("block must be 0 or a multiple of 64 in 64.." + std::to_string(most));
         check(hipStreamSynchronize(b->ctx->stream), "sync");
+        setLaneWidth(b, mv == 16 ? 16 : 64);
         b->laneMv = mv;
         b->laneBlockSz = block;
         b->resumable = false;  // a chunked solve continues with the schedule it started with
@@ -1235,10 +1400,11 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
         o.block_size = 0;
         o.max_time_secs = 0.;
         TuneGuard guard(b);
-        const int cand[4][2] = {{0, 1024}, {8, 1024}, {0, 512}, {8, 512}};
+        const int cand[SPK_AUTOTUNE_CANDIDATES][2] = {{0, 1024}, {8, 1024}, {0, 512}, {8, 512}, {16, 256}, {16, 512}};
         double best = 1e300;
         int bi = 0;
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < SPK_AUTOTUNE_CANDIDATES; i++) {
+            setLaneWidth(b, cand[i][0] == 16 ? 16 : 64);
             b->laneMv = cand[i][0];
             b->laneBlockSz = cand[i][1];
             if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
@@ -1252,6 +1418,7 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
                 bi = i;
             }
         }
+        setLaneWidth(b, cand[bi][0] == 16 ? 16 : 64);
         b->laneMv = cand[bi][0];
         b->laneBlockSz = cand[bi][1];
         b->tuned = true;
@@ -1358,17 +1525,20 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
             spk_default_options(&o);
             b->dOut.alloc(sizeof(spk::Out) * b->count);
             check(hipMemset(b->dOut.p, 0, sizeof(spk::Out) * b->count), "zero out");
-            const dim3 grid((unsigned) (b->Bpad / 64)), block(laneBlock(b, o));
+            const dim3 grid((unsigned) (b->Bpad / b->laneW)), block(laneBlock(b, o));
             auto *outs = static_cast<spk::Out *>(b->dOut.p);
+            const bool tile = b->laneW == 16;
             if (f64) {
                 buildLaneInst<double>(b, o, 0);
                 auto *P = static_cast<const spk::LaneInst<double> *>(b->dInst.p);
-                if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_debug_x8<double>, grid, block, 0, b->ctx->stream, P, outs, op);
+                if (tile) hipLaunchKernelGGL(spk::k_tile_debug<double>, grid, block, 0, b->ctx->stream, P, outs, op);
+                else if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_debug_x8<double>, grid, block, 0, b->ctx->stream, P, outs, op);
                 else hipLaunchKernelGGL(spk::k_lane_debug<double>, grid, block, 0, b->ctx->stream, P, outs, op);
             } else {
                 buildLaneInst<float>(b, o, 0);
                 auto *P = static_cast<const spk::LaneInst<float> *>(b->dInst.p);
-                if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_debug_x8<float>, grid, block, 0, b->ctx->stream, P, outs, op);
+                if (tile) hipLaunchKernelGGL(spk::k_tile_debug<float>, grid, block, 0, b->ctx->stream, P, outs, op);
+                else if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_debug_x8<float>, grid, block, 0, b->ctx->stream, P, outs, op);
                 else hipLaunchKernelGGL(spk::k_lane_debug<float>, grid, block, 0, b->ctx->stream, P, outs, op);
             }
             check(hipGetLastError(), "launch k_lane_debug");

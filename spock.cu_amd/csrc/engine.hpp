@@ -43,11 +43,11 @@ struct Inst {
 constexpr int GRID_MAXK = 16;
 constexpr int GRID_MAXWG = 1024;
 
-// Block-interleaved layout of a region of `rows` rows: instances are grouped in blocks of 64 and
-// each block stores rows x 64 words contiguously, element e of instance b at
-// ((b / 64) * rows + e) * 64 + b % 64.
-__host__ __device__ inline size_t laneIndex(long rows, long e, long b) {
-    return ((size_t) (b >> 6) * rows + e) * 64 + (b & 63);
+// Block-interleaved layout of a region of `rows` rows: instances are grouped in blocks of w (64
+// for the lane engine, 16 for the tile engine) and each block stores rows x w words contiguously,
+// element e of instance b at ((b / w) * rows + e) * w + b % w.
+__host__ __device__ inline size_t laneIndex(long rows, long e, long b, long w = 64) {
+    return ((size_t) (b / w) * rows + e) * w + (b % w);
 }
 
 // SuperMann state of one lane kept between the launches of a chunked solve.
@@ -55,6 +55,7 @@ struct LaneSave {
     double G[4][4], gb[4];
     double tol, lastErr, wLast, zeta, w, wSafe;
     long iters, k0, k1, k2, k2bt, k3, nT, nEll;
+    long it0;  // workgroup iteration at which this lane's current solve started
     int status, active, initFlag, pad;
 };
 

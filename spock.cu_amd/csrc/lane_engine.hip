@@ -15,6 +15,7 @@
 // dynamics sweep always applies the precomputed (L L')^-1 and the Anderson least squares is the
 // FP64 Gram/Cholesky solve (the same arithmetic as the single-instance engine's Gram option).
 #include "engine.hpp"
+#include "lockstep.hpp"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -40,12 +41,12 @@
 
 namespace spk {
 namespace lane {
-
-constexpr int MAXM = 4;  // Anderson memory supported by the lane engine
+using namespace ls;
 
 struct Ctx {
     int lane, wave, nw, tid;
-    int gl;  // this lane's instance index
+    int gl;       // this lane's instance index
+    bool writer;  // every lane writes its own instance's records
 };
 
 __device__ __forceinline__ Ctx ctx() {
@@ -55,37 +56,8 @@ __device__ __forceinline__ Ctx ctx() {
     c.wave = __builtin_amdgcn_readfirstlane(c.tid >> 6);
     c.nw = blockDim.x >> 6;
     c.gl = blockIdx.x * 64 + c.lane;
+    c.writer = true;
     return c;
-}
-
-// A pointer every lane holds the same value of, moved to scalar registers: loads through it
-// become scalar loads (needed inside non-inlined functions, whose arguments arrive per lane).
-template <typename T>
-__device__ __forceinline__ T *uniformPtr(T *p) {
-    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned) v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned) (v >> 32));
-    return reinterpret_cast<T *>(((unsigned long long) hi << 32) | lo);
-}
-
-// Constant address space: the descriptor, the tree arrays and the factor tables are never written
-// while a solve runs, and wave-uniform reads through such pointers become scalar loads.
-#define SPK_CONST __attribute__((address_space(4)))
-#define SPK_GLOBAL __attribute__((address_space(1)))
-template <typename T>
-using CP = const SPK_CONST T *;
-template <typename T>
-using LP = const SPK_CONST LaneInst<T> &;
-
-template <typename T>
-__device__ __forceinline__ CP<T> cst(const T *p) {
-    return (CP<T>) p;
-}
-
-// Node n's entry of a per-node table, as a constant-space pointer.
-template <typename T>
-__device__ __forceinline__ CP<T> tab(const SPK_CONST DTab<T> &t, int n) {
-    return cst(t.data) + (size_t) cst(t.idx)[n] * t.rows * t.cols;
 }
 
 // Lane view of a region in the block-interleaved layout (each 64-instance block holds its rows
@@ -672,18 +644,6 @@ __device__ __forceinline__ void phaseDual(LP<T> P, const Ctx &c, const LV<T> &z,
     for (int n = c.wave; n < P.N; n += c.nw) opNode(P, n, pv, h);
 }
 
-// Device wall-clock per phase, accumulated by thread 0 into LDS (uniform control flow, so
-// thread 0's view is the workgroup's).
-struct LaneClock {
-    long last;
-    long *acc;
-    __device__ __forceinline__ void mark(int ph) {
-        const long t = wall_clock64();
-        if (threadIdx.x == 0) acc[ph] += t - last;
-        last = t;
-    }
-    __device__ __forceinline__ void skip() { last = wall_clock64(); }
-};
 
 // The CP operator T (src/cache.cuh:796-801): cand = T(z).
 template <typename T>
@@ -725,10 +685,6 @@ __device__ __forceinline__ double laneMax(double v, double *lds, const Ctx &c) {
     return s;
 }
 
-template <typename T>
-struct DotMax {
-    T dot, mx, cross;  // cross = sum over primal elements of x_p . (L* y_d)
-};
 
 // sum x.(y - gamma ell(y)) and max |y/gamma - ell(y)| per lane.
 template <typename T>
@@ -760,10 +716,6 @@ __device__ __noinline__ DotMax<T> reduceEll(const LaneInst<T> *Pp, T *x, T *y, d
     return r;
 }
 
-template <typename T>
-struct Cross2 {
-    T c1, c2;
-};
 
 // The adjoint-only terms of <x, M x> and <x, M y> (M = [[I, -gamma L*], [-gamma L, I]], and
 // x_d' L y_p = (L* x_d)' y_p):  c1 = sum_p x_p . (L* x_d),  c2 = sum_p x_p . (L* y_d) + (L* x_d) . y_p.
@@ -808,523 +760,37 @@ __device__ __forceinline__ void pass(long n, const Ctx &c, Ld ld, St st) {
     for (; e < e1; e++) st(e, ld(e));
 }
 
-template <typename T>
-struct V2 {
-    T a, b;
-};
-template <typename T>
-struct V4 {
-    T a, b, c, d;
-};
-template <typename T, int M>
-struct VH {  // two vector values and M history values of one element
-    T a, b, h[M];
-};
-template <typename T, int M>
-struct V4H {
-    T a, b, c, d, h[M];
-};
 
-__device__ __forceinline__ bool anyLane(bool v) { return __syncthreads_or(v ? 1 : 0) != 0; }
 
-// Least squares argmin ||Y g - r|| from the Gram matrix: Cholesky of G + 1e-12 max(diag) I in FP64.
-__device__ __forceinline__ bool gramSolve(const double (&G)[MAXM][MAXM], const double (&gb)[MAXM], int m,
-                                          double (&g)[MAXM]) {
-    double Lm[MAXM][MAXM];
-    double dmax = 0.;
-#pragma unroll
-    for (int i = 0; i < MAXM; i++)
-        if (i < m) dmax = fmax(dmax, G[i][i]);
-    if (!(dmax > 0.)) return false;
-    const double ridge = 1e-12 * dmax;
-    bool ok = true;
-#pragma unroll
-    for (int i = 0; i < MAXM; i++) {
-#pragma unroll
-        for (int j = 0; j < MAXM; j++) {
-            Lm[i][j] = 0.;
-            if (i < m && j <= i) {
-                double v = G[i][j] + (i == j ? ridge : 0.);
-#pragma unroll
-                for (int k = 0; k < MAXM; k++)
-                    if (k < j) v -= Lm[i][k] * Lm[j][k];
-                if (i == j) {
-                    ok = ok && (v > 0.);
-                    Lm[i][i] = sqrt(fmax(v, 0.));
-                } else {
-                    Lm[i][j] = v / Lm[j][j];
-                }
-            }
-        }
+
+// The lane engine as a policy of the lockstep solve: lane = instance, 64 per wave.
+struct Policy {
+    template <typename T>
+    using View = LV<T>;
+    __device__ static Ctx ctx() { return lane::ctx(); }
+    template <typename T>
+    __device__ static LV<T> view(T *region, long rows, const Ctx &c) { return lview(region, rows, c.lane); }
+    template <int U, typename Ld, typename St>
+    __device__ static void pass(long n, const Ctx &c, Ld ld, St st) { lane::pass<U>(n, c, ld, st); }
+    __device__ static double sum(double v, double *lds, const Ctx &c) { return laneSum(v, lds, c); }
+    __device__ static double max(double v, double *lds, const Ctx &c) { return laneMax(v, lds, c); }
+    template <typename T>
+    __device__ static void applyT(const LaneInst<T> *P, T *z, T *cand, long *phaseAcc) { lane::applyT(P, z, cand, phaseAcc); }
+    template <typename T>
+    __device__ static DotMax<T> reduceEll(const LaneInst<T> *P, T *x, T *y, double *lds) { return lane::reduceEll(P, x, y, lds); }
+    template <typename T>
+    __device__ static Cross2<T> adjDots(const LaneInst<T> *P, T *x, T *y, double *lds, bool withY) {
+        return lane::adjDots(P, x, y, lds, withY);
     }
-    if (!ok) return false;
-    double y[MAXM];
-#pragma unroll
-    for (int i = 0; i < MAXM; i++) {
-        y[i] = 0.;
-        if (i < m) {
-            double v = gb[i];
-#pragma unroll
-            for (int k = 0; k < MAXM; k++)
-                if (k < i) v -= Lm[i][k] * y[k];
-            y[i] = v / Lm[i][i];
-        }
-    }
-#pragma unroll
-    for (int i = MAXM - 1; i >= 0; i--) {
-        g[i] = 0.;
-        if (i < m) {
-            double v = y[i];
-#pragma unroll
-            for (int k = 0; k < MAXM; k++)
-                if (k > i && k < m) v -= Lm[k][i] * g[k];
-            g[i] = v / Lm[i][i];
-        }
-    }
-    return true;
-}
+};
 
 }  // namespace lane
-
-// ---------------------------------------------------------------- the lockstep solve
-template <typename T>
-__device__ void laneSolve(const LaneInst<T> *Pp, Out *outs, double *lds, long *phaseAcc) {
-    using namespace lane;
-    LP<T> P = *cst(Pp);
-    if (threadIdx.x < NUM_PHASES) phaseAcc[threadIdx.x] = 0;
-    __syncthreads();
-    LaneClock clk{wall_clock64(), phaseAcc};
-    const Ctx c = ctx();
-    const long n = P.sIter, B = P.B;
-    const int gl = c.gl;
-    const bool real = gl < P.count;
-    const int m = P.andBuff;
-    auto V = [&](T *p) { return lview(p, n, c.lane); };
-    T *Z = P.work[0], *C = P.work[1], *ZP = P.work[2], *RR = P.work[3], *RP = P.work[4], *BK = P.work[5];
-    T *C2 = P.work[6], *D = P.work[7], *DS = P.work[8];
-    const long t0 = wall_clock64();
-    const bool resume = P.resume != 0;
-    LaneGroupSave *gs = P.gsave + blockIdx.x;
-    long itStart = 0, ticksBefore = 0;
-    int headSaved = m - 1;
-    // a resume of a workgroup whose solve has ended leaves its results as they are
-    if (resume && __builtin_amdgcn_readfirstlane(gs->finished)) return;
-    if (resume) {  // roles of the six rotating buffers, ring head, next iteration (uniform)
-        T **roles[6] = {&Z, &C, &ZP, &RR, &RP, &BK};
-#pragma unroll
-        for (int i = 0; i < 6; i++) *roles[i] = P.work[__builtin_amdgcn_readfirstlane(gs->roles[i])];
-        headSaved = __builtin_amdgcn_readfirstlane(gs->head);
-        const long itv = gs->it;
-        itStart = ((long) __builtin_amdgcn_readfirstlane((int) (itv >> 32)) << 32) |
-                  (unsigned) __builtin_amdgcn_readfirstlane((int) itv);
-        ticksBefore = gs->ticks;
-    }
-    // initialiseIter / initialisePrev (src/cache.cuh:611-632)
-    if (!resume) {
-        const LV<T> zio = V(P.zio), rio = V(P.rio), z = V(Z), zp = V(ZP), rp = V(RP);
-        const bool spock = P.method == METHOD_SPOCK, warm = P.warm != 0;
-        pass<16>(
-            n, c, [&](long e) { return V2<T>{zio[e], rio[e]}; },
-            [&](long e, const V2<T> &v) {
-                z[e] = v.a;
-                if (spock) {
-                    zp[e] = warm ? v.a : T(1);
-                    rp[e] = warm ? v.b : T(1);
-                }
-            });
-    }
-    __syncthreads();
-    double G[MAXM][MAXM], gb[MAXM];
-#pragma unroll
-    for (int i = 0; i < MAXM; i++) {
-        gb[i] = 0.;
-#pragma unroll
-        for (int j = 0; j < MAXM; j++) G[i][j] = 0.;
-    }
-    int head = headSaved;
-    bool active = real;
-    const T given = (real && P.tolFixed) ? P.tolFixed[gl] : T(0);
-    bool initFlag = !(given > T(0));
-    int status = ST_NOTRUN;
-    long iters = 0;
-    T tol = initFlag ? T(0) : given, lastErr = T(NAN), wLast = T(NAN);
-    const T c0 = T(0.99), c1 = T(0.99), c2 = T(0.99), beta = T(0.5), sigma = T(0.1), lambda = T(1);
-    const T inf = T(INFINITY);
-    T zeta = inf, w = inf, wSafe = inf;
-    long k0 = 0, k1 = 0, k2 = 0, k2bt = 0, k3 = 0, nT = 0, nEll = 0;
-    if (resume) {  // this lane's SuperMann state
-        const LaneSave &sv = P.save[gl];
-#pragma unroll
-        for (int i = 0; i < MAXM; i++) {
-            gb[i] = sv.gb[i];
-#pragma unroll
-            for (int j = 0; j < MAXM; j++) G[i][j] = sv.G[i][j];
-        }
-        tol = (T) sv.tol;
-        lastErr = (T) sv.lastErr;
-        wLast = (T) sv.wLast;
-        zeta = (T) sv.zeta;
-        w = (T) sv.w;
-        wSafe = (T) sv.wSafe;
-        iters = sv.iters;
-        k0 = sv.k0; k1 = sv.k1; k2 = sv.k2; k2bt = sv.k2bt; k3 = sv.k3; nT = sv.nT; nEll = sv.nEll;
-        status = sv.status;
-        active = sv.active != 0;
-        initFlag = sv.initFlag != 0;
-    }
-    // the iteration cap is only tested at error checks (src/cache.cuh:937-940): the last outer
-    // iteration is the first check index at or beyond maxIters
-    const long hardCap =
-        P.maxIters > 0 ? (P.maxIters + P.checkPeriod - 1) / P.checkPeriod * P.checkPeriod + 1 : 2000000;
-    long it;
-    bool chunkExit = false;
-    for (it = itStart; it < hardCap; it++) {
-        if (!anyLane(active)) break;
-        if (P.chunkIters > 0 && it >= itStart + P.chunkIters) {  // uniform: save and leave
-            chunkExit = true;
-            break;
-        }
-        if (it != 0 && P.method == METHOD_SPOCK) {  // acceptCandidate: z <- cand
-            T *t = Z;
-            Z = C;
-            C = t;
-        }
-        clk.mark(PH_OTHER);
-        applyT(Pp, Z, C, phaseAcc);
-        clk.skip();
-        if (active) nT++;
-        T sRR;
-        {  // r = z - T(z) and ||r||^2
-            const LV<T> rr = V(RR), z = V(Z), cc = V(C);
-            T acc = T(0);
-            pass<16>(n, c, [&](long e) { return V2<T>{z[e], cc[e]}; }, [&](long e, const V2<T> &v) {
-                const T d = v.a - v.b;
-                rr[e] = d;
-                acc += d * d;
-            });
-            sRR = (T) laneSum((double) acc, lds, c);
-        }
-        clk.mark(PH_RES);
-        // <r, M r> = ||r||^2 - 2 gamma r_p . L* r_d; the full [L*; L] only at the error checks
-        const bool checkIt = it % P.checkPeriod == 0;
-        T mx = T(0), crossRR;
-        if (checkIt) {
-            const DotMax<T> dm = reduceEll(Pp, RR, RR, lds);
-            mx = dm.mx;
-            crossRR = dm.cross;
-        } else {
-            crossRR = adjDots(Pp, RR, RR, lds, false).c1;
-        }
-        const T dotRR = sRR - T(2) * P.gamma * crossRR;
-        if (active) nEll++;
-        clk.mark(PH_ELL);
-        bool done = false;
-        if (active && checkIt) {  // computeError (src/cache.cuh:910-961)
-            if (initFlag) {
-                tol = fmax(P.tolAbs, P.tolRel * mx);
-                initFlag = false;
-            } else {
-                lastErr = mx;
-                int code = ST_NOTRUN;
-                if (mx <= tol) code = ST_CONVERGED;
-                if (P.maxIters && it >= P.maxIters) code = ST_OUTOFITER;
-                if (P.maxTicks > 0 && it % 1000 == 0 &&
-                    (double) (wall_clock64() - t0 + ticksBefore) >= P.maxTicks)
-                    code = ST_OUTOFTIME;
-                if (code != ST_NOTRUN) {
-                    status = code;
-                    iters = it;
-                    done = true;
-                }
-            }
-        }
-        if (anyLane(done)) {  // accepted candidate and its residual are the solve's output
-            const LV<T> zio = V(P.zio), rio = V(P.rio), cc = V(C), rr = V(RR);
-            pass<16>(
-                n, c, [&](long e) { return V2<T>{cc[e], rr[e]}; },
-                [&](long e, const V2<T> &v) {
-                    if (done) {
-                        zio[e] = v.a;
-                        rio[e] = v.b;
-                    }
-                });
-            __syncthreads();
-        }
-        if (done) active = false;
-        if (P.method == METHOD_CP) {
-            T *t = Z;
-            Z = C;
-            C = t;
-            continue;
-        }
-        {  // backup (src/cache.cuh:804-806)
-            T *t = BK;
-            BK = C;
-            C = t;
-        }
-        w = sqrt(dotRR);
-        if (active) wLast = w;
-        head = (head + 1) % m;
-        {  // Anderson column (src/cache.cuh:861-896) with the Gram entries of the new column
-            double acc[2 * MAXM];
-#pragma unroll
-            for (int q = 0; q < 2 * MAXM; q++) acc[q] = 0.;
-            const LV<T> Yh = V(P.hY + (long) head * n * B), Sh = V(P.hS + (long) head * n * B);
-            const LV<T> rr = V(RR), rp = V(RP), z = V(Z), zp = V(ZP);
-            LV<T> Ys[MAXM];
-#pragma unroll
-            for (int s2 = 0; s2 < MAXM; s2++) Ys[s2] = V(P.hY + (long) (s2 < m ? s2 : 0) * n * B);
-            pass<4>(
-                n, c,
-                [&](long e) {
-                    V4H<T, MAXM> v;
-                    v.a = rr[e];
-                    v.b = rp[e];
-                    v.c = z[e];
-                    v.d = zp[e];
-#pragma unroll
-                    for (int s2 = 0; s2 < MAXM; s2++) v.h[s2] = (s2 < m && s2 != head) ? Ys[s2][e] : T(0);
-                    return v;
-                },
-                [&](long e, const V4H<T, MAXM> &v) {
-                    const T dr = v.a - v.b;
-                    Yh[e] = dr;
-                    Sh[e] = (v.c - v.d) - dr;
-                    const double drd = (double) dr, rd = (double) v.a;
-#pragma unroll
-                    for (int s2 = 0; s2 < MAXM; s2++) {
-                        if (s2 < m) {
-                            const double ys = (s2 == head) ? drd : (double) v.h[s2];
-                            acc[s2] += drd * ys;
-                            acc[MAXM + s2] += ys * rd;
-                        }
-                    }
-                });
-#pragma unroll
-            for (int q = 0; q < 2 * MAXM; q++)
-                if ((q & (MAXM - 1)) < m) acc[q] = laneSum(acc[q], lds, c);
-#pragma unroll
-            for (int s2 = 0; s2 < MAXM; s2++) {
-                if (s2 < m) {
-#pragma unroll
-                    for (int hh = 0; hh < MAXM; hh++) {
-                        if (hh == head) {
-                            G[hh][s2] = acc[s2];
-                            G[s2][hh] = acc[s2];
-                        }
-                    }
-                    gb[s2] = acc[MAXM + s2];
-                }
-            }
-        }
-        clk.mark(PH_ACOL);
-        {  // saveToPrev as role moves
-            T *t = ZP;
-            ZP = Z;
-            Z = t;
-            t = RP;
-            RP = RR;
-            RR = t;
-        }
-        double gs[MAXM];
-#pragma unroll
-        for (int q = 0; q < MAXM; q++) gs[q] = 0.;
-        const bool useLS = (it >= m) && gramSolve(G, gb, m, gs);
-        clk.mark(PH_ALS);
-        const bool blind = active && (w <= c0 * zeta);
-        {  // d = -S g - r; K0 lanes write cand = z_prev + d (src/cache.cuh:897-898, 1055-1057)
-            T gg[MAXM];
-            LV<T> Ss[MAXM];
-#pragma unroll
-            for (int s2 = 0; s2 < MAXM; s2++) {
-                gg[s2] = (useLS && s2 < m) ? (T) gs[s2] : T(0);
-                Ss[s2] = V(P.hS + (long) (s2 < m ? s2 : 0) * n * B);
-            }
-            const LV<T> rp = V(RP), zp = V(ZP), dd = V(D), cc = V(C);
-            pass<4>(
-                n, c,
-                [&](long e) {
-                    VH<T, MAXM> v;
-                    v.a = rp[e];
-                    v.b = zp[e];
-#pragma unroll
-                    for (int s2 = 0; s2 < MAXM; s2++) v.h[s2] = (s2 < m) ? Ss[s2][e] : T(0);
-                    return v;
-                },
-                [&](long e, const VH<T, MAXM> &v) {
-                    T sg = T(0);
-#pragma unroll
-                    for (int s2 = 0; s2 < MAXM; s2++)
-                        if (s2 < m) sg += v.h[s2] * gg[s2];
-                    const T d = -sg - v.a;
-                    if (blind) cc[e] = v.b + d;
-                    else dd[e] = d;
-                });
-        }
-        __syncthreads();
-        clk.mark(PH_AAPPLY);
-        if (blind) {
-            zeta = w;
-            k0++;
-        }
-        bool ls = active && !blind;
-        T tau = T(1);
-        for (int in = 0; in < P.maxInner; in++) {  // line search (src/cache.cuh:1063-1099)
-            if (!anyLane(ls)) break;
-            {
-                const LV<T> d = V(D), zp = V(ZP), ds = V(DS), z = V(Z);
-                pass<16>(
-                    n, c, [&](long e) { return V2<T>{d[e], zp[e]}; },
-                    [&](long e, const V2<T> &v) {
-                        if (ls) {
-                            const T s = v.a * tau;
-                            ds[e] = s;
-                            z[e] = v.b + s;
-                        }
-                    });
-            }
-            __syncthreads();
-            clk.mark(PH_LINE);
-            applyT(Pp, Z, C2, phaseAcc);
-            clk.skip();
-            if (ls) nT++;
-            T sRRt, sRD;
-            {  // r~ = z - T(z), ||r~||^2 and r~ . ds
-                const LV<T> rr = V(RR), z = V(Z), cc = V(C2), ds = V(DS);
-                T a1 = T(0), a2 = T(0);
-                pass<16>(n, c, [&](long e) { return V4<T>{z[e], cc[e], ds[e], T(0)}; },
-                        [&](long e, const V4<T> &v) {
-                            const T d = v.a - v.b;
-                            rr[e] = d;
-                            a1 += d * d;
-                            a2 += d * v.c;
-                        });
-                sRRt = (T) laneSum((double) a1, lds, c);
-                sRD = (T) laneSum((double) a2, lds, c);
-            }
-            clk.mark(PH_RES);
-            // <r~, M r~> and <r~, M ds> from two adjoints (src/cache.cuh:1066,1080)
-            const Cross2<T> cr = adjDots(Pp, RR, DS, lds, true);
-            const T wTilde = sqrt(sRRt - T(2) * P.gamma * cr.c1);
-            const T dotRD = sRD - P.gamma * cr.c2;
-            if (ls) nEll++;
-            clk.mark(PH_ELL);
-            const bool takeK1 = ls && (w <= wSafe) && (wTilde <= c1 * w);
-            if (anyLane(takeK1)) {  // K1: cand <- z
-                const LV<T> cc = V(C), z = V(Z);
-                pass<16>(n, c, [&](long e) { return z[e]; }, [&](long e, T v) {
-                    if (takeK1) cc[e] = v;
-                });
-                __syncthreads();
-            }
-            if (takeK1) {
-                wSafe = wTilde + pow(c2, T(it));
-                k1++;
-            }
-            const bool still = ls && !takeK1;
-            const T rho = wTilde * wTilde - dotRD;
-            const bool takeK2 = still && (rho >= sigma * wTilde * w);
-            if (anyLane(takeK2)) {  // K2: cand <- z_prev - f r
-                const T f = lambda * rho / (wTilde * wTilde);
-                const LV<T> cc = V(C), rr = V(RR), zp = V(ZP);
-                pass<16>(
-                    n, c, [&](long e) { return V2<T>{rr[e], zp[e]}; },
-                    [&](long e, const V2<T> &v) {
-                        if (takeK2) cc[e] = v.b - v.a * f;
-                    });
-                __syncthreads();
-            }
-            if (takeK2) k2++;
-            const bool back = still && !takeK2;
-            if (back) {
-                tau *= beta;
-                k2bt++;
-            }
-            const bool takeK3 = back && (in >= P.maxInner - 1);
-            if (anyLane(takeK3)) {  // K3: restore the plain CP step
-                const LV<T> cc = V(C), bk = V(BK);
-                pass<16>(n, c, [&](long e) { return bk[e]; }, [&](long e, T v) {
-                    if (takeK3) cc[e] = v;
-                });
-                __syncthreads();
-            }
-            if (takeK3) k3++;
-            ls = back;
-            clk.mark(PH_LINE);
-        }
-    }
-    if (chunkExit) {  // keep everything a resume needs; results report the progress so far
-        if (threadIdx.x == 0) {
-            gs->finished = 0;
-            T *cur[6] = {Z, C, ZP, RR, RP, BK};
-            for (int i = 0; i < 6; i++)
-                for (int j = 0; j < 6; j++)
-                    if (cur[i] == P.work[j]) gs->roles[i] = j;
-            gs->head = head;
-            gs->it = it;
-            gs->ticks = ticksBefore + (long) (wall_clock64() - t0);
-        }
-        if (c.wave == 0) {
-            LaneSave &sv = P.save[gl];
-            for (int i = 0; i < MAXM; i++) {
-                sv.gb[i] = gb[i];
-                for (int j = 0; j < MAXM; j++) sv.G[i][j] = G[i][j];
-            }
-            sv.tol = (double) tol;
-            sv.lastErr = (double) lastErr;
-            sv.wLast = (double) wLast;
-            sv.zeta = (double) zeta;
-            sv.w = (double) w;
-            sv.wSafe = (double) wSafe;
-            sv.iters = active ? it : iters;
-            sv.k0 = k0; sv.k1 = k1; sv.k2 = k2; sv.k2bt = k2bt; sv.k3 = k3; sv.nT = nT; sv.nEll = nEll;
-            sv.status = status;
-            sv.active = active ? 1 : 0;
-            sv.initFlag = initFlag ? 1 : 0;
-        }
-        if (active) iters = it;
-    }
-    if (!chunkExit && threadIdx.x == 0) gs->finished = 1;
-    const bool capped = active && !chunkExit;
-    if (anyLane(capped)) {  // iteration cap of the launch: the current iterate is the output
-        const LV<T> zio = V(P.zio), rio = V(P.rio), z = V(Z), rr = V(RR);
-        pass<16>(
-            n, c, [&](long e) { return V2<T>{z[e], rr[e]}; },
-            [&](long e, const V2<T> &v) {
-                if (capped) {
-                    zio[e] = v.a;
-                    rio[e] = v.b;
-                }
-            });
-    }
-    if (capped) {
-        status = ST_OUTOFITER;
-        iters = it;
-    }
-    __syncthreads();
-    if (c.wave == 0 && real) {
-        Out &o = outs[gl];
-        o.status = status;
-        o.iters = iters;
-        o.err = (double) lastErr;
-        o.tol = (double) tol;
-        o.w = (double) wLast;
-        o.k0 = k0;
-        o.k1 = k1;
-        o.k2 = k2;
-        o.k2bt = k2bt;
-        o.k3 = k3;
-        o.nT = nT;
-        o.nEll = nEll;
-        for (int i = 0; i < NUM_PHASES; i++) o.ticks[i] = phaseAcc[i];
-    }
-}
 
 template <typename T>
 __global__ void __launch_bounds__(SPK_LANE_MAX_THREADS) SPK_LANE_KERNEL(k_lane_solve)(const LaneInst<T> *__restrict__ P, Out *outs) {
     __shared__ double lds[16 * 64];
     __shared__ long phaseAcc[NUM_PHASES];
-    laneSolve(P, outs, lds, phaseAcc);
+    lockstepSolve<T, lane::Policy>(P, outs, lds, phaseAcc);
 }
 
 // Single operators on every instance (parity tests): z = zio, cand = work[1].
@@ -1377,22 +843,118 @@ template __global__ void SPK_LANE_KERNEL(k_lane_debug)<double>(const LaneInst<do
 // [instance][element] rows (leading dimension ld) to the block-interleaved layout of a region of
 // n rows and back, with dtype conversion.
 template <typename S, typename D>
-__global__ void k_interleave(const S *__restrict__ in, long ld, D *__restrict__ out, long n, int count, long) {
+__global__ void k_interleave(const S *__restrict__ in, long ld, D *__restrict__ out, long n, int count, long w) {
     const long total = n * (long) count;
     for (long t = blockIdx.x * (long) blockDim.x + threadIdx.x; t < total; t += (long) gridDim.x * blockDim.x) {
         const long b = t % count, e = t / count;
-        out[laneIndex(n, e, b)] = (D) in[b * ld + e];
+        out[laneIndex(n, e, b, w)] = (D) in[b * ld + e];
     }
 }
 
 template <typename S, typename D>
-__global__ void k_deinterleave(const S *__restrict__ in, long, D *__restrict__ out, long ld, long n, int count) {
+__global__ void k_deinterleave(const S *__restrict__ in, long w, D *__restrict__ out, long ld, long n, int count) {
     const long total = n * (long) count;
     for (long t = blockIdx.x * (long) blockDim.x + threadIdx.x; t < total; t += (long) gridDim.x * blockDim.x) {
         const long e = t % n, b = t / n;
-        out[b * ld + e] = (D) in[laneIndex(n, e, b)];
+        out[b * ld + e] = (D) in[laneIndex(n, e, b, w)];
     }
 }
+
+// A region of n rows from blocks of win instances to blocks of wout instances (all padded slots).
+template <typename T>
+__global__ void k_relayout(const T *__restrict__ in, long win, T *__restrict__ out, long wout, long n, long B) {
+    const long total = n * B;
+    for (long t = blockIdx.x * (long) blockDim.x + threadIdx.x; t < total; t += (long) gridDim.x * blockDim.x) {
+        const long b = t % B, e = t / B;
+        out[laneIndex(n, e, b, wout)] = in[laneIndex(n, e, b, win)];
+    }
+}
+
+template __global__ void k_relayout<float>(const float *, long, float *, long, long, long);
+template __global__ void k_relayout<double>(const double *, long, double *, long, long, long);
+
+// Starts new solves in lanes of a resumable (chunked) solve of a shared batch with blocks of w
+// instances: lane lanes[i] gets the scaled initial state x0s[i] (nx values) and the starting
+// iterate z[i] (n values, leading dimension ldz; z == null: the zero iterate of a cold start), with
+// the previous iterate and residual that initialisePrev sets (src/cache.cuh:611-632; a warm start
+// takes the iterate and a zero residual, a cold start ones), fresh SuperMann state, and an
+// iteration count that starts at its workgroup's next iteration.  The starting iterate goes to the
+// buffer the next iteration reads as z: the candidate role, which that iteration accepts, or the
+// iterate role itself if the workgroup has not run yet.
+template <typename T, typename Z>
+__global__ void k_lane_refill(const LaneInst<T> *__restrict__ Pp, const int *__restrict__ lanes, int count,
+                              const Z *__restrict__ z, long ldz, const T *__restrict__ x0s, long w) {
+    const LaneInst<T> &P = *Pp;
+    const long n = P.sIter;
+    const long stride = (long) gridDim.x * blockDim.x;
+    const long tid = blockIdx.x * (long) blockDim.x + threadIdx.x;
+    for (long t = tid; t < n * count; t += stride) {
+        const long li = t / n, e = t % n;
+        const long gl = lanes[li];
+        const LaneGroupSave &gs = P.gsave[gl / w];
+        const T v = z ? (T) z[li * ldz + e] : T(0);
+        const size_t at = laneIndex(n, e, gl, w);
+        P.work[gs.it == 0 ? gs.roles[0] : gs.roles[1]][at] = v;
+        P.work[gs.roles[2]][at] = z ? v : T(1);
+        P.work[gs.roles[4]][at] = z ? T(0) : T(1);
+        P.zio[at] = v;
+        P.rio[at] = T(0);
+    }
+    T *x0 = const_cast<T *>(P.x0s);
+    for (long t = tid; t < (long) P.nx * count; t += stride) {
+        const long li = t / P.nx, e = t % P.nx;
+        x0[laneIndex(P.nx, e, lanes[li], w)] = x0s[li * P.nx + e];
+    }
+    for (long li = tid; li < count; li += stride) {
+        const long gl = lanes[li];
+        LaneSave &sv = P.save[gl];
+        for (int i = 0; i < 4; i++) {
+            sv.gb[i] = 0.;
+            for (int j = 0; j < 4; j++) sv.G[i][j] = 0.;
+        }
+        const double given = P.tolFixed ? (double) P.tolFixed[gl] : 0.;
+        sv.tol = given > 0. ? given : 0.;
+        sv.initFlag = given > 0. ? 0 : 1;
+        sv.lastErr = sv.wLast = NAN;
+        sv.zeta = sv.w = sv.wSafe = INFINITY;
+        sv.iters = sv.k0 = sv.k1 = sv.k2 = sv.k2bt = sv.k3 = sv.nT = sv.nEll = 0;
+        sv.it0 = P.gsave[gl / w].it;
+        sv.status = ST_NOTRUN;
+        sv.active = 1;
+    }
+}
+
+// After the refill kernel: the refilled lanes' workgroups run on their next resume.
+template <typename T>
+__global__ void k_lane_refill_groups(const LaneInst<T> *__restrict__ Pp, const int *__restrict__ lanes, int count, long w) {
+    const long stride = (long) gridDim.x * blockDim.x;
+    for (long li = blockIdx.x * (long) blockDim.x + threadIdx.x; li < count; li += stride)
+        Pp->gsave[lanes[li] / w].finished = 0;
+}
+
+template __global__ void k_lane_refill<float, float>(const LaneInst<float> *, const int *, int, const float *, long,
+                                                      const float *, long);
+template __global__ void k_lane_refill<double, float>(const LaneInst<double> *, const int *, int, const float *, long,
+                                                       const double *, long);
+template __global__ void k_lane_refill<double, double>(const LaneInst<double> *, const int *, int, const double *,
+                                                        long, const double *, long);
+template __global__ void k_lane_refill<float, double>(const LaneInst<float> *, const int *, int, const double *, long,
+                                                       const float *, long);
+// The accepted iterates of the listed lanes (blocks of w instances) as rows of leading dimension ld.
+template <typename S, typename D>
+__global__ void k_gather_lanes(const S *__restrict__ in, long w, D *__restrict__ out, long ld, long n,
+                               const int *__restrict__ lanes, int count) {
+    const long total = n * (long) count;
+    for (long t = blockIdx.x * (long) blockDim.x + threadIdx.x; t < total; t += (long) gridDim.x * blockDim.x) {
+        const long li = t / n, e = t % n;
+        out[li * ld + e] = (D) in[laneIndex(n, e, lanes[li], w)];
+    }
+}
+
+template __global__ void k_gather_lanes<float, float>(const float *, long, float *, long, long, const int *, int);
+template __global__ void k_gather_lanes<double, double>(const double *, long, double *, long, long, const int *, int);
+template __global__ void k_lane_refill_groups<float>(const LaneInst<float> *, const int *, int, long);
+template __global__ void k_lane_refill_groups<double>(const LaneInst<double> *, const int *, int, long);
 
 template __global__ void k_interleave<float, float>(const float *, long, float *, long, int, long);
 template __global__ void k_interleave<float, double>(const float *, long, double *, long, int, long);

@@ -659,4 +659,16 @@ int spk_mlp_predict(spk_mlp *m, const double *x0, int rows, spk_batch *dst) {
     }
 }
 
+int spk_mlp_predict_lanes(spk_mlp *m, const double *x0, int rows, spk_batch *dst, const int *lanes) {
+    try {
+        uploadInputs(m, x0, rows);
+        forward(m, rows, true);
+        check(hipStreamSynchronize(m->ctx->stream), "sync");
+        if (spk_batch_refill_rows(dst, rows, lanes, x0, m->Y.as<float>(), m->out())) throw std::runtime_error(m->ctx->err);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(m->ctx, e);
+    }
+}
+
 }  // extern "C"

@@ -30,9 +30,11 @@ EXPORTS = [
     "spk_batch_set_iterates", "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates",
     "spk_batch_solve", "spk_batch_resume", "spk_batch_results", "spk_batch_debug_op",
     "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
+    "spk_batch_refill", "spk_batch_refill_from_batch", "spk_batch_refill_rows",
     "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
     "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
     "spk_mlp_loss_backward", "spk_mlp_adam", "spk_mlp_dataset", "spk_mlp_train", "spk_mlp_predict",
+    "spk_mlp_predict_lanes",
 ]
 
 
@@ -116,6 +118,10 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_debug_op.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     lib.spk_batch_set_iterates_device.argtypes = [vp, vp, C.c_long]
     lib.spk_batch_get_iterates_device.argtypes = [vp, vp, C.c_long]
+    _ip = C.POINTER(C.c_int)
+    lib.spk_batch_refill.argtypes = [vp, C.c_int, _ip, _dp]
+    lib.spk_batch_refill_from_batch.argtypes = [vp, C.c_int, _ip, _dp, vp, _ip]
+    lib.spk_batch_refill_rows.argtypes = [vp, C.c_int, _ip, _dp, vp, C.c_long]
     _fp = C.POINTER(C.c_float)
     lib.spk_mlp_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)]
     lib.spk_mlp_destroy.argtypes = [vp]
@@ -131,6 +137,7 @@ def load_library(path=LIB_PATH):
     lib.spk_mlp_dataset.argtypes = [vp, vp, _dp]
     lib.spk_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _dp]
     lib.spk_mlp_predict.argtypes = [vp, _dp, C.c_int, vp]
+    lib.spk_mlp_predict_lanes.argtypes = [vp, _dp, C.c_int, vp, _ip]
     _LIB = lib
     return lib
 
@@ -202,10 +209,11 @@ class Problem:
 class Batch:
     """A set of solve instances (``spk_batch``); one instance = one Cache of the reference.
 
-    ``shared=True`` (all instances of one problem) selects the instance-interleaved layout
-    solved by the lane engine (``spk_batch_create_shared``)."""
+    ``shared=True`` (all instances of one problem) selects the instance-interleaved layout solved
+    in lockstep (``spk_batch_create_shared``): by the tile engine unless ``engine`` names the lane
+    engine's schedule (0 or 8, see ``set_schedule``)."""
 
-    def __init__(self, ctx, problems, dtype=F64, shared=False):
+    def __init__(self, ctx, problems, dtype=F64, shared=False, engine=None):
         self.ctx = ctx
         self.problems = list(problems)
         self.dtype = dtype
@@ -223,6 +231,8 @@ class Batch:
         self.h = h
         self.sizes = [p.n_iter for p in self.problems]
         self.total = int(sum(self.sizes))
+        if self.shared and engine is not None:
+            self.set_schedule(engine)
 
     def __del__(self):
         try:
@@ -236,19 +246,22 @@ class Batch:
         self.ctx.check(self.ctx.lib.spk_batch_reset(self.h), "reset")
 
     def set_schedule(self, mv, block=0):
-        """Lane-kernel schedule of a shared batch: mv 0 (scalar factor loads) or 8 (LDS-staged)."""
+        """Engine and schedule of a shared batch: mv 16 the tile engine (MFMA, 16 instances per
+        workgroup, the default), 0 or 8 the lane engine (64 per workgroup; scalar or LDS-staged factor
+        loads).  block: workgroup size (0: the engine's default)."""
         self.ctx.check(self.ctx.lib.spk_batch_set_schedule(self.h, int(mv), int(block)), "set_schedule")
 
+    AUTOTUNE_CANDIDATES = [(0, 1024), (8, 1024), (0, 512), (8, 512), (16, 256), (16, 512)]
+
     def autotune(self, opts=None, iters=50):
-        """Time the lane-kernel schedules on this batch and keep the fastest; returns
+        """Time the engines and schedules on this batch and keep the fastest; returns
         (mv, block, {(mv, block): seconds})."""
         mv, blk = C.c_int(), C.c_int()
-        sec = np.zeros(4)
+        sec = np.zeros(len(self.AUTOTUNE_CANDIDATES))
         o = opts if opts is not None else options()
         self.ctx.check(self.ctx.lib.spk_batch_autotune(self.h, C.byref(o), int(iters), C.byref(mv), C.byref(blk),
                                                         _d(sec)), "autotune")
-        cands = [(0, 1024), (8, 1024), (0, 512), (8, 512)]
-        return mv.value, blk.value, dict(zip(cands, sec.tolist()))
+        return mv.value, blk.value, dict(zip(self.AUTOTUNE_CANDIDATES, sec.tolist()))
 
     def autotune_workgroups(self, opts=None, iters=50):
         """One-instance batch: time 1/8/32/64/128 cooperating workgroups and keep the fastest;
@@ -323,6 +336,27 @@ class Batch:
     def resume_graph(self, launches):
         """Continue a chunked solve with `launches` resume launches captured as one HIP graph."""
         self.ctx.check(self.ctx.lib.spk_batch_resume_graph(self.h, int(launches)), "resume_graph")
+
+    def _lanes_x0(self, lanes, x0):
+        lanes = np.ascontiguousarray(np.asarray(lanes, dtype=np.int32).ravel())
+        x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64).reshape(lanes.size, -1))
+        return lanes, x0
+
+    def refill(self, lanes, x0):
+        """Continuous batching: cold-start new solves with initial states x0 (rows) in slots
+        ``lanes`` of a chunked solve; they run from the next resume (``spk_batch_refill``)."""
+        lanes, x0 = self._lanes_x0(lanes, x0)
+        self.ctx.check(self.ctx.lib.spk_batch_refill(self.h, lanes.size, lanes.ctypes.data_as(C.POINTER(C.c_int)),
+                                                     _d(x0)), "refill")
+
+    def refill_from(self, lanes, x0, src, src_lanes):
+        """Warm-start new solves in slots ``lanes`` from the accepted iterates of slots ``src_lanes``
+        of the shared batch ``src`` (``spk_batch_refill_from_batch``)."""
+        lanes, x0 = self._lanes_x0(lanes, x0)
+        src_lanes = np.ascontiguousarray(np.asarray(src_lanes, dtype=np.int32).ravel())
+        ip = C.POINTER(C.c_int)
+        self.ctx.check(self.ctx.lib.spk_batch_refill_from_batch(self.h, lanes.size, lanes.ctypes.data_as(ip), _d(x0),
+                                                                src.h, src_lanes.ctypes.data_as(ip)), "refill_from")
 
     def solve_chunked(self, opts, warm=False, on_chunk=None):
         """Run a chunked solve to its end; on_chunk(results, kernel_seconds) after every chunk.
@@ -430,6 +464,15 @@ class WarmStartMLP:
     def predict(self, x0, batch):
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         self.ctx.check(self.ctx.lib.spk_mlp_predict(self.h, _d(x0), x0.shape[0], batch.h), "mlp predict")
+
+    def predict_lanes(self, x0, batch, lanes):
+        """Continuous batching: new solves in slots ``lanes`` of ``batch`` warm-started from the
+        prediction for initial states x0 (``spk_mlp_predict_lanes``)."""
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        lanes = np.ascontiguousarray(np.asarray(lanes, dtype=np.int32).ravel())
+        assert lanes.size == x0.shape[0]
+        self.ctx.check(self.ctx.lib.spk_mlp_predict_lanes(self.h, _d(x0), x0.shape[0], batch.h,
+                                                          lanes.ctypes.data_as(C.POINTER(C.c_int))), "mlp predict_lanes")
 
     def save(self, path):
         a, b, c, d = self.normalization()

@@ -91,13 +91,15 @@ def test_fp32_per_instance_operators_match_fp64_oracle(ctx, problem_dirs, name):
     check_ops(o, b, zs, ys, x0s, count, name)
 
 
+@pytest.mark.parametrize("engine", [16, 0])
 @pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine"])
-def test_fp32_shared_operators_match_fp64_oracle(ctx, problem_dirs, name):
+def test_fp32_shared_operators_match_fp64_oracle(ctx, problem_dirs, name, engine):
+    """engine 16: the tile engine (matrix-core products); 0: the lane engine."""
     d = problem_dirs[name]
     o = Oracle(d)
     p = rt.Problem(ctx, d)
     count = 3
-    b = rt.Batch(ctx, [p] * count, rt.F32, shared=True)
+    b = rt.Batch(ctx, [p] * count, rt.F32, shared=True, engine=engine)
     rng = np.random.default_rng(22)
     zs = [f32(inert_zero(o, 3.0 * rng.standard_normal(o.n_iter))) for _ in range(count)]
     ys = [f32(inert_zero(o, rng.standard_normal(o.n_iter))) for _ in range(count)]

@@ -1,5 +1,6 @@
-"""Shared-problem batches (the lane engine: 64 instances per wave, interleaved layout) vs the
-FP64 CPU oracle, through the C ABI.
+"""Shared-problem batches vs the FP64 CPU oracle, through the C ABI: the tile engine (16 instances
+per workgroup, per-node products on the matrix cores, schedule 16, the default) and the lane
+engine (64 instances per wave, schedule 0), both in an interleaved layout.
 
 The lane engine solves its Anderson least squares from the Gram matrix and forms SuperMann's
 M-inner products from adjoints only (<x, M y> = x'y - gamma (x_p' L* y_d + (L* x_d)' y_p)), so
@@ -48,17 +49,21 @@ def inert_zero(o, z):
     return z
 
 
-def shared(ctx, d, count, dtype=rt.F64):
+def shared(ctx, d, count, dtype=rt.F64, engine=None):
     p = rt.Problem(ctx, d)
-    return p, rt.Batch(ctx, [p] * count, dtype, shared=True)
+    return p, rt.Batch(ctx, [p] * count, dtype, shared=True, engine=engine)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "affine"])
-def test_shared_operators_match_oracle(ctx, problem_dirs, name):
+ENGINES = [16, 0]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "affine", "nocon"])
+def test_shared_operators_match_oracle(ctx, problem_dirs, name, engine):
     d = problem_dirs[name]
     o = Oracle(d)
     count = 3
-    _, b = shared(ctx, d, count)
+    _, b = shared(ctx, d, count, engine=engine)
     rng = np.random.default_rng(11)
     zs = [inert_zero(o, 3.0 * rng.standard_normal(o.n_iter)) for _ in range(count)]
     ys = [inert_zero(o, rng.standard_normal(o.n_iter)) for _ in range(count)]
@@ -97,15 +102,16 @@ def test_shared_operators_match_oracle(ctx, problem_dirs, name):
         assert abs(sc[2 * k + 1] - np.max(np.abs(y / o.gamma - ell))) <= 1e-12 * np.max(np.abs(y / o.gamma))
 
 
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("name,iters,method", [("fixture", 50, "spock"), ("small", 50, "spock"),
                                                ("mid", 50, "spock"), ("small", 250, "cp"), ("poly", 50, "spock"),
                                                ("nocon", 50, "spock"), ("affine", 50, "spock")])
-def test_shared_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method):
+def test_shared_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method, engine):
     d = problem_dirs[name]
     o = Oracle(d)
     count = 3
     x0s = x0s_for(name, o.nx, count)
-    _, b = shared(ctx, d, count)
+    _, b = shared(ctx, d, count, engine=engine)
     b.set_x0(x0s)
     b.solve(rt.options(tol_abs=1e-30, max_iters=iters, method=rt.SPOCK if method == "spock" else rt.CP))
     res, _ = b.results()
@@ -247,8 +253,7 @@ def test_lds_schedule_is_bitwise_identical(ctx, problem_dirs, name, dtype):
     x0s = x0s_for(name, o.nx, count)
     out = []
     for mv in (0, 8):
-        p, b = shared(ctx, d, count, dtype)
-        b.set_schedule(mv)
+        p, b = shared(ctx, d, count, dtype, engine=mv)
         b.set_x0(x0s)
         b.solve(rt.options(tol_abs=1e-3, tol_rel=1e-3, max_iters=100))
         res, _ = b.results()
