@@ -180,13 +180,23 @@ int spk_batch_resume(spk_batch *b);
  * (a warm start's previous iterate is the starting iterate and its previous residual zero, as
  * initialisePrev with a previous solution, src/cache.cuh:611-632). */
 int spk_batch_refill(spk_batch *b, int count, const int *lanes, const double *x0);
+/* an empty pool: a chunked solve (opts->chunk_iters > 0) in which every slot is idle (its result
+ * stays SPK_NOT_RUN) until a refill starts a solve in it; spk_batch_resume then runs the slots that
+ * have one.  opts (tolerances, caps, ...) apply to every solve of the pool. */
+int spk_batch_pool_open(spk_batch *b, const spk_options *opts);
+/* outer iterations per launch of the following resumes of a chunked solve */
+int spk_batch_set_chunk(spk_batch *b, long chunk_iters);
 int spk_batch_refill_from_batch(spk_batch *b, int count, const int *lanes, const double *x0, spk_batch *src,
                                 const int *src_lanes);
 int spk_batch_refill_rows(spk_batch *b, int count, const int *lanes, const double *x0, const float *z, long ld);
-/* shared batches: the engine and schedule.  mv = 16 (default): the tile engine, 16 instances per
- * workgroup in blocks of 16 (element e of instance k at ((k / 16) * n + e) * 16 + k % 16) with the
- * per-node products on the matrix cores (v_mfma 16x16x4, the 16 instances as the B operand), block
- * 64..512 (0: 256).  mv = 0 or 8: the lane engine, 64 instances per workgroup in blocks of 64 with
+/* the accepted iterates of `count` slots `lanes` of a shared batch as FP32 device rows (leading
+ * dimension ld >= size_iterate) */
+int spk_batch_get_lanes_device(spk_batch *b, int count, const int *lanes, float *dst, long ld);
+/* shared batches: the engine and schedule.  mv = 17 (default) or 16: the tile engine, 16 instances
+ * per workgroup in blocks of 16 (element e of instance k at ((k / 16) * n + e) * 16 + k % 16) with the
+ * per-node products on the matrix cores (v_mfma 16x16x4, the 16 instances as the B operand), built
+ * for workgroups of up to 1024 threads (17; block 0: 1024) or 512 (16; block 0: 256).
+ * mv = 0 or 8: the lane engine, 64 instances per workgroup in blocks of 64 with
  * the factor entries read by scalar loads (0) or staged through LDS in 8-column chunks (8), which
  * give bitwise identical iterates, block 64..1024 (0: 1024).  Switching between the engines re-lays
  * out the initial states and accepted iterates and clears the workspace.  No counterpart in the
@@ -194,11 +204,11 @@ int spk_batch_refill_rows(spk_batch *b, int count, const int *lanes, const doubl
 int spk_batch_set_schedule(spk_batch *b, int mv, int block);
 /* shared batches: time `iters` cold outer iterations (from the batch's initial states) under each of
  * the SPK_AUTOTUNE_CANDIDATES schedules (mv, block) = (0, 1024), (8, 1024), (0, 512), (8, 512),
- * (16, 256), (16, 512), keep the fastest for the batch's later solves and return it (mv, block:
+ * (16, 256), (16, 512), (17, 1024), keep the fastest for the batch's later solves and return it (mv, block:
  * nullable; seconds: nullable, the kernel times in that order).  The iterates are reset (initial
  * states and given tolerances are kept).  opts supplies the solver options of the timed runs
  * (nullable: defaults). */
-enum { SPK_AUTOTUNE_CANDIDATES = 6 };
+enum { SPK_AUTOTUNE_CANDIDATES = 7 };
 int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv, int *block, double *seconds);
 /* shared batches: a chunked solve (opts->chunk_iters > 0) captured as one HIP graph of `launches`
  * kernel launches (the first and launches - 1 resumes) and replayed with no host round trip between
@@ -252,6 +262,9 @@ int spk_mlp_adam(spk_mlp *m, float lr, float beta1, float beta2, float eps);
 /* device-resident training set from a solved batch (targets = its accepted iterates, standardized)
  * and the x0 of its instances (host, count x in); sets the normalization */
 int spk_mlp_dataset(spk_mlp *m, spk_batch *solved, const double *x0);
+/* the same from `count` slots `lanes` of a solved shared batch only (e.g. its converged solves), x0
+ * holding their initial states (host, count x in) */
+int spk_mlp_dataset_lanes(spk_mlp *m, spk_batch *solved, int count, const int *lanes, const double *x0);
 /* epochs of shuffled minibatch Adam on the training set; epoch_loss (nullable) gets the mean loss */
 int spk_mlp_train(spk_mlp *m, int epochs, int minibatch, float lr, unsigned long long seed, double *epoch_loss);
 /* warm-start iterates of `rows` initial states (host, rows x in) written into dst (rows = its count) */

@@ -38,6 +38,10 @@ __global__ void k_tile_solve(const LaneInst<T> *, Out *);
 template <typename T>
 __global__ void k_tile_debug(const LaneInst<T> *, Out *, int);
 template <typename T>
+__global__ void k_tile_solve_x1024(const LaneInst<T> *, Out *);
+template <typename T>
+__global__ void k_tile_debug_x1024(const LaneInst<T> *, Out *, int);
+template <typename T>
 __global__ void k_relayout(const T *, long, T *, long, long, long);
 template <typename T, typename Z>
 __global__ void k_lane_refill(const LaneInst<T> *, const int *, int, const Z *, long, const T *, long);
@@ -115,7 +119,7 @@ struct spk_batch {
     bool resumable = false;
     size_t lRegion[16] = {};  // element offsets: work[0..8], zio, rio, hS, hY, qd, dd, x0
     static constexpr int LANE_MAXM = 4;
-    int laneMv = 16;     // schedule: 16 the tile engine (MFMA); 0, 8 the lane engine (scalar loads, LDS-staged)
+    int laneMv = 17;     // schedule: 16, 17 the tile engine (MFMA; 512- or 1024-thread build); 0, 8 the lane engine
     DevBuf dInstResume;  // the lane instance of the resume launches of a captured chunked solve
     DevBuf rLanes, rX0, rZ;  // staging of a refill: lane indices, scaled initial states, starting iterates
     hipGraph_t graph = nullptr;
@@ -222,13 +226,15 @@ double tolScaleOf(const HostProblem &h) {
     return h.precond ? *std::min_element(h.scaling.begin(), h.scaling.end()) : 1.;
 }
 
+// The instance descriptors of a per-instance solve in b->hInst: Anderson rings of instance k at
+// anders + andOff[k], grid-mode partials at gpart.
 template <typename T>
-void buildInsts(spk_batch *b, const spk_options &o, int warm) {
-    if (b->count == 1 && !b->gpart.p) b->gpart.alloc((size_t) 2 * spk::GRID_MAXK * spk::GRID_MAXWG * sizeof(T));
+void fillInsts(spk_batch *b, const spk_options &o, int warm, T *anders, const std::vector<size_t> &andOff,
+               T *gpart) {
     b->hInst.assign(sizeof(spk::Inst<T>) * b->count, 0);
     auto *arr = reinterpret_cast<spk::Inst<T> *>(b->hInst.data());
     T *core = static_cast<T *>(b->core.p);
-    T *and_ = static_cast<T *>(b->anders.p);
+    T *and_ = anders;
     for (int k = 0; k < b->count; k++) {
         spk_problem *p = b->probs[k];
         const HostProblem &h = p->hp;
@@ -257,7 +263,7 @@ void buildInsts(spk_batch *b, const spk_options &o, int warm) {
         I.qd = c + 8 * n;
         I.dd = I.qd + (size_t) h.nx * h.N;
         I.x0s = static_cast<const T *>(b->x0buf.p) + b->x0Off[k];
-        T *a = and_ + b->andOff[k];
+        T *a = and_ + andOff[k];
         const int m = o.anderson_buffer;
         I.hS = a; I.hY = a + (size_t) m * n; I.W = a + (size_t) 2 * m * n;
         double tolScale = 1.;
@@ -274,8 +280,14 @@ void buildInsts(spk_batch *b, const spk_options &o, int warm) {
         I.lsMethod = o.ls_method;
         I.dynInverse = o.dyn_inverse;
         I.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
-        I.gpart = static_cast<T *>(b->gpart.p);
+        I.gpart = gpart;
     }
+}
+
+template <typename T>
+void buildInsts(spk_batch *b, const spk_options &o, int warm) {
+    if (b->count == 1 && !b->gpart.p) b->gpart.alloc((size_t) 2 * spk::GRID_MAXK * spk::GRID_MAXWG * sizeof(T));
+    fillInsts<T>(b, o, warm, static_cast<T *>(b->anders.p), b->andOff, static_cast<T *>(b->gpart.p));
     b->dInst.alloc(b->hInst.size());
     check(hipMemcpyAsync(b->dInst.p, b->hInst.data(), b->hInst.size(), hipMemcpyHostToDevice, b->ctx->stream),
           "upload instances");
@@ -385,12 +397,13 @@ void launchMaybeGrid(spk_batch *b, K kernel, int bs, size_t shm, A... args) {
     }
 }
 
+// insDev: the device copy of the descriptors (default: b->dInst)
 template <typename T>
-void launchKernel(spk_batch *b, int bs, bool debug, int op) {
+void launchKernel(spk_batch *b, int bs, bool debug, int op, const void *insDev = nullptr) {
     const int cls = chunkClass(b);
     const int pool = ldsPoolInts(b);
     const size_t shm = (size_t) pool * sizeof(int);
-    auto *ins = static_cast<const spk::Inst<T> *>(b->dInst.p);
+    auto *ins = static_cast<const spk::Inst<T> *>(insDev ? insDev : b->dInst.p);
     auto *outs = static_cast<spk::Out *>(b->dOut.p);
     const int count = b->count;
     const double tick = b->ctx->tickRateHz;
@@ -403,11 +416,15 @@ void launchKernel(spk_batch *b, int bs, bool debug, int op) {
     }
 }
 
+int solveBlock(const spk_options &o) {
+    const int bs = o.block_size > 0 ? o.block_size : 256;
+    return std::max(64, std::min(512, bs / 64 * 64));
+}
+
 template <typename T>
 void launchSolve(spk_batch *b, const spk_options &o, int warm) {
     buildInsts<T>(b, o, warm);
-    int bs = o.block_size > 0 ? o.block_size : 256;
-    bs = std::max(64, std::min(512, bs / 64 * 64));
+    const int bs = solveBlock(o);
     b->dOut.alloc(sizeof(spk::Out) * b->count);
     check(hipEventRecord(b->ev0, b->ctx->stream), "event");
     launchKernel<T>(b, bs, false, 0);
@@ -415,7 +432,6 @@ void launchSolve(spk_batch *b, const spk_options &o, int warm) {
     check(hipEventRecord(b->ev1, b->ctx->stream), "event");
     b->pending = true;
 }
-
 
 // ---------------------------------------------------------------- shared-problem (lane) batches
 // Regions of the interleaved state, each (rows x Bpad): work[0..8] and zio, rio (sIter rows),
@@ -493,12 +509,17 @@ void buildLaneInst(spk_batch *b, const spk_options &o, int warm, int resume = 0)
           "upload lane instance");
 }
 
+// Schedules of shared batches: the lane engine with scalar (0) or LDS-staged (8) factor loads,
+// the tile engine built for workgroups of up to 512 (16) or 1024 threads (17).
+bool tileSchedule(int mv) { return mv == 16 || mv == 17; }
+int scheduleMaxThreads(int mv) { return mv == 16 ? 512 : 1024; }
+
 // Workgroup size of the shared-batch kernels: the caller's, else the autotuner's, else the
-// engine's default (tile engine 256, at most 512; lane engine 1024).
+// schedule's default (256 for the 512-thread tile build, else its launch bound).
 int laneBlock(const spk_batch *b, const spk_options &o) {
-    const bool tile = b->laneW == 16;
-    const int bs = o.block_size > 0 ? o.block_size : (b->laneBlockSz > 0 ? b->laneBlockSz : (tile ? 256 : 1024));
-    return std::max(64, std::min(tile ? 512 : 1024, bs / 64 * 64));
+    const int most = scheduleMaxThreads(b->laneMv);
+    const int bs = o.block_size > 0 ? o.block_size : (b->laneBlockSz > 0 ? b->laneBlockSz : (b->laneMv == 16 ? 256 : most));
+    return std::max(64, std::min(most, bs / 64 * 64));
 }
 
 template <typename T>
@@ -506,7 +527,8 @@ void launchLaneKernel(spk_batch *b, const spk_options &o, const void *inst = nul
     const dim3 grid((unsigned) (b->Bpad / b->laneW)), block(laneBlock(b, o));
     auto *P = static_cast<const spk::LaneInst<T> *>(inst ? inst : b->dInst.p);
     auto *outs = static_cast<spk::Out *>(b->dOut.p);
-    if (b->laneW == 16) hipLaunchKernelGGL(spk::k_tile_solve<T>, grid, block, 0, b->ctx->stream, P, outs);
+    if (b->laneMv == 17) hipLaunchKernelGGL(spk::k_tile_solve_x1024<T>, grid, block, 0, b->ctx->stream, P, outs);
+    else if (b->laneMv == 16) hipLaunchKernelGGL(spk::k_tile_solve<T>, grid, block, 0, b->ctx->stream, P, outs);
     else if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_solve_x8<T>, grid, block, 0, b->ctx->stream, P, outs);
     else hipLaunchKernelGGL(spk::k_lane_solve<T>, grid, block, 0, b->ctx->stream, P, outs);
 }
@@ -1327,6 +1349,82 @@ int spk_batch_refill_rows(spk_batch *b, int count, const int *lanes, const doubl
     }
 }
 
+int spk_batch_get_lanes_device(spk_batch *b, int count, const int *lanes, float *dst, long ld) {
+    try {
+        if (!dst || ld < b->probs[0]->hp.sIter) throw std::runtime_error("rows must hold the iterate size");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        stageLanes(b, count, lanes);
+        const long n = b->probs[0]->hp.sIter;
+        if (b->dtype == SPK_F64)
+            hipLaunchKernelGGL((spk::k_gather_lanes<double, float>), dim3(1024), dim3(256), 0, b->ctx->stream,
+                               lanePtr<double>(b, LR_ZIO), (long) b->laneW, dst, ld, n, b->rLanes.as<int>(), count);
+        else
+            hipLaunchKernelGGL((spk::k_gather_lanes<float, float>), dim3(1024), dim3(256), 0, b->ctx->stream,
+                               lanePtr<float>(b, LR_ZIO), (long) b->laneW, dst, ld, n, b->rLanes.as<int>(), count);
+        check(hipGetLastError(), "launch k_gather_lanes");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_pool_open(spk_batch *b, const spk_options *opts) {
+    try {
+        if (!b->lanes) throw std::runtime_error("a pool is a shared batch");
+        if (!opts || opts->chunk_iters <= 0) throw std::runtime_error("a pool runs in chunks (chunk_iters > 0)");
+        if (opts->anderson_buffer < 1 || opts->anderson_buffer > spk_batch::LANE_MAXM)
+            throw std::runtime_error("shared batches support anderson_buffer 1..4");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        const int groups = (int) (b->Bpad / 16);
+        std::vector<spk::LaneSave> ls(b->Bpad);
+        for (auto &v : ls) {
+            std::memset(static_cast<void *>(&v), 0, sizeof(v));
+            v.status = spk::ST_NOTRUN;
+            v.active = 0;
+            v.initFlag = 1;
+        }
+        std::vector<spk::LaneGroupSave> gs(groups);
+        for (auto &g : gs) {
+            for (int i = 0; i < 6; i++) g.roles[i] = i;
+            g.head = opts->anderson_buffer - 1;
+            g.finished = 1;
+            g.it = 0;
+            g.ticks = 0;
+        }
+        check(hipMemcpy(b->lsave.p, ls.data(), ls.size() * sizeof(spk::LaneSave), hipMemcpyHostToDevice), "pool state");
+        check(hipMemcpy(b->lgsave.p, gs.data(), gs.size() * sizeof(spk::LaneGroupSave), hipMemcpyHostToDevice),
+              "pool state");
+        if (b->dOut.bytes != sizeof(spk::Out) * b->count) b->dOut.alloc(sizeof(spk::Out) * b->count);
+        std::vector<spk::Out> outs(b->count);
+        for (auto &o : outs) {
+            std::memset(static_cast<void *>(&o), 0, sizeof(o));
+            o.status = spk::ST_NOTRUN;
+        }
+        check(hipMemcpy(b->dOut.p, outs.data(), outs.size() * sizeof(spk::Out), hipMemcpyHostToDevice), "pool results");
+        b->lastOpts = *opts;
+        b->lastWarm = 0;
+        b->resumable = true;
+        b->pending = false;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_set_chunk(spk_batch *b, long chunk_iters) {
+    try {
+        if (!b->lanes || !b->resumable) throw std::runtime_error("no chunked solve of a shared batch");
+        if (chunk_iters <= 0) throw std::runtime_error("chunk_iters must be positive");
+        b->lastOpts.chunk_iters = chunk_iters;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
 int spk_batch_resume(spk_batch *b) {
     try {
         if (!b->lanes || !b->resumable) throw std::runtime_error("no chunked solve of a shared batch to resume");
@@ -1372,12 +1470,12 @@ int spk_batch_resume_graph(spk_batch *b, int launches) {
 int spk_batch_set_schedule(spk_batch *b, int mv, int block) {
     try {
         if (!b->lanes) throw std::runtime_error("schedules apply to shared batches");
-        if (mv != 0 && mv != 8 && mv != 16) throw std::runtime_error("schedule must be 0, 8 or 16");
-        const int most = mv == 16 ? 512 : 1024;
+        if (mv != 0 && mv != 8 && !tileSchedule(mv)) throw std::runtime_error("schedule must be 0, 8, 16 or 17");
+        const int most = scheduleMaxThreads(mv);
         if (block != 0 && (block < 64 || block > most || block % 64))
             throw std::runtime_error("block must be 0 or a multiple of 64 in 64.." + std::to_string(most));
         check(hipStreamSynchronize(b->ctx->stream), "sync");
-        setLaneWidth(b, mv == 16 ? 16 : 64);
+        setLaneWidth(b, tileSchedule(mv) ? 16 : 64);
         b->laneMv = mv;
         b->laneBlockSz = block;
         b->resumable = false;  // a chunked solve continues with the schedule it started with
@@ -1400,11 +1498,12 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
         o.block_size = 0;
         o.max_time_secs = 0.;
         TuneGuard guard(b);
-        const int cand[SPK_AUTOTUNE_CANDIDATES][2] = {{0, 1024}, {8, 1024}, {0, 512}, {8, 512}, {16, 256}, {16, 512}};
+        const int cand[SPK_AUTOTUNE_CANDIDATES][2] = {{0, 1024}, {8, 1024}, {0, 512}, {8, 512},
+                                                       {16, 256}, {16, 512}, {17, 1024}};
         double best = 1e300;
         int bi = 0;
         for (int i = 0; i < SPK_AUTOTUNE_CANDIDATES; i++) {
-            setLaneWidth(b, cand[i][0] == 16 ? 16 : 64);
+            setLaneWidth(b, tileSchedule(cand[i][0]) ? 16 : 64);
             b->laneMv = cand[i][0];
             b->laneBlockSz = cand[i][1];
             if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
@@ -1418,7 +1517,7 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
                 bi = i;
             }
         }
-        setLaneWidth(b, cand[bi][0] == 16 ? 16 : 64);
+        setLaneWidth(b, tileSchedule(cand[bi][0]) ? 16 : 64);
         b->laneMv = cand[bi][0];
         b->laneBlockSz = cand[bi][1];
         b->tuned = true;
@@ -1531,13 +1630,15 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
             if (f64) {
                 buildLaneInst<double>(b, o, 0);
                 auto *P = static_cast<const spk::LaneInst<double> *>(b->dInst.p);
-                if (tile) hipLaunchKernelGGL(spk::k_tile_debug<double>, grid, block, 0, b->ctx->stream, P, outs, op);
+                if (b->laneMv == 17) hipLaunchKernelGGL(spk::k_tile_debug_x1024<double>, grid, block, 0, b->ctx->stream, P, outs, op);
+                else if (tile) hipLaunchKernelGGL(spk::k_tile_debug<double>, grid, block, 0, b->ctx->stream, P, outs, op);
                 else if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_debug_x8<double>, grid, block, 0, b->ctx->stream, P, outs, op);
                 else hipLaunchKernelGGL(spk::k_lane_debug<double>, grid, block, 0, b->ctx->stream, P, outs, op);
             } else {
                 buildLaneInst<float>(b, o, 0);
                 auto *P = static_cast<const spk::LaneInst<float> *>(b->dInst.p);
-                if (tile) hipLaunchKernelGGL(spk::k_tile_debug<float>, grid, block, 0, b->ctx->stream, P, outs, op);
+                if (b->laneMv == 17) hipLaunchKernelGGL(spk::k_tile_debug_x1024<float>, grid, block, 0, b->ctx->stream, P, outs, op);
+                else if (tile) hipLaunchKernelGGL(spk::k_tile_debug<float>, grid, block, 0, b->ctx->stream, P, outs, op);
                 else if (b->laneMv == 8) hipLaunchKernelGGL(spk::k_lane_debug_x8<float>, grid, block, 0, b->ctx->stream, P, outs, op);
                 else hipLaunchKernelGGL(spk::k_lane_debug<float>, grid, block, 0, b->ctx->stream, P, outs, op);
             }

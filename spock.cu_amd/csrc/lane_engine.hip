@@ -953,6 +953,7 @@ __global__ void k_gather_lanes(const S *__restrict__ in, long w, D *__restrict__
 
 template __global__ void k_gather_lanes<float, float>(const float *, long, float *, long, long, const int *, int);
 template __global__ void k_gather_lanes<double, double>(const double *, long, double *, long, long, const int *, int);
+template __global__ void k_gather_lanes<double, float>(const double *, long, float *, long, long, const int *, int);
 template __global__ void k_lane_refill_groups<float>(const LaneInst<float> *, const int *, int, long);
 template __global__ void k_lane_refill_groups<double>(const LaneInst<double> *, const int *, int, long);
 

@@ -285,7 +285,9 @@ __device__ void lockstepSolve(const LaneInst<T> *Pp, Out *outs, double *lds, lon
         clk.skip();
         if (active) nT++;
         T sRR;
-        {  // r = z - T(z) and ||r||^2
+        const bool spock = P.method == METHOD_SPOCK;
+        if (spock) head = (head + 1) % m;
+        if (!spock) {  // r = z - T(z) and ||r||^2
             const VW rr = V(RR), z = V(Z), cc = V(C);
             T acc = T(0);
             E::template pass<16>(n, c, [&](long e) { return V2<T>{z[e], cc[e]}; }, [&](long e, const V2<T> &v) {
@@ -294,6 +296,65 @@ __device__ void lockstepSolve(const LaneInst<T> *Pp, Out *outs, double *lds, lon
                 acc += d * d;
             });
             sRR = (T) E::sum((double) acc, lds, c);
+        } else {
+            // r = z - T(z), ||r||^2 and, in the same pass, the new Anderson column
+            // (src/cache.cuh:861-896) with its Gram entries: Y_head = r - r_prev,
+            // S_head = (z - z_prev) - Y_head
+            double acc[2 * MAXM];
+#pragma unroll
+            for (int q = 0; q < 2 * MAXM; q++) acc[q] = 0.;
+            T accR = T(0);
+            const VW Yh = V(P.hY + (long) head * n * B), Sh = V(P.hS + (long) head * n * B);
+            const VW rr = V(RR), cc = V(C), rp = V(RP), z = V(Z), zp = V(ZP);
+            VW Ys[MAXM];
+#pragma unroll
+            for (int s2 = 0; s2 < MAXM; s2++) Ys[s2] = V(P.hY + (long) (s2 < m ? s2 : 0) * n * B);
+            E::template pass<4>(
+                n, c,
+                [&](long e) {
+                    V4H<T, MAXM> v;
+                    v.a = z[e];
+                    v.b = cc[e];
+                    v.c = rp[e];
+                    v.d = zp[e];
+#pragma unroll
+                    for (int s2 = 0; s2 < MAXM; s2++) v.h[s2] = (s2 < m && s2 != head) ? Ys[s2][e] : T(0);
+                    return v;
+                },
+                [&](long e, const V4H<T, MAXM> &v) {
+                    const T r = v.a - v.b;
+                    rr[e] = r;
+                    accR += r * r;
+                    const T dr = r - v.c;
+                    Yh[e] = dr;
+                    Sh[e] = (v.a - v.d) - dr;
+                    const double drd = (double) dr, rd = (double) r;
+#pragma unroll
+                    for (int s2 = 0; s2 < MAXM; s2++) {
+                        if (s2 < m) {
+                            const double ys = (s2 == head) ? drd : (double) v.h[s2];
+                            acc[s2] += drd * ys;
+                            acc[MAXM + s2] += ys * rd;
+                        }
+                    }
+                });
+            sRR = (T) E::sum((double) accR, lds, c);
+#pragma unroll
+            for (int q = 0; q < 2 * MAXM; q++)
+                if ((q & (MAXM - 1)) < m) acc[q] = E::sum(acc[q], lds, c);
+#pragma unroll
+            for (int s2 = 0; s2 < MAXM; s2++) {
+                if (s2 < m) {
+#pragma unroll
+                    for (int hh = 0; hh < MAXM; hh++) {
+                        if (hh == head) {
+                            G[hh][s2] = acc[s2];
+                            G[s2][hh] = acc[s2];
+                        }
+                    }
+                    gb[s2] = acc[MAXM + s2];
+                }
+            }
         }
         clk.mark(PH_RES);
         // <r, M r> = ||r||^2 - 2 gamma r_p . L* r_d; the full [L*; L] only at the error checks
@@ -355,59 +416,6 @@ __device__ void lockstepSolve(const LaneInst<T> *Pp, Out *outs, double *lds, lon
         }
         w = sqrt(dotRR);
         if (active) wLast = w;
-        head = (head + 1) % m;
-        {  // Anderson column (src/cache.cuh:861-896) with the Gram entries of the new column
-            double acc[2 * MAXM];
-#pragma unroll
-            for (int q = 0; q < 2 * MAXM; q++) acc[q] = 0.;
-            const VW Yh = V(P.hY + (long) head * n * B), Sh = V(P.hS + (long) head * n * B);
-            const VW rr = V(RR), rp = V(RP), z = V(Z), zp = V(ZP);
-            VW Ys[MAXM];
-#pragma unroll
-            for (int s2 = 0; s2 < MAXM; s2++) Ys[s2] = V(P.hY + (long) (s2 < m ? s2 : 0) * n * B);
-            E::template pass<4>(
-                n, c,
-                [&](long e) {
-                    V4H<T, MAXM> v;
-                    v.a = rr[e];
-                    v.b = rp[e];
-                    v.c = z[e];
-                    v.d = zp[e];
-#pragma unroll
-                    for (int s2 = 0; s2 < MAXM; s2++) v.h[s2] = (s2 < m && s2 != head) ? Ys[s2][e] : T(0);
-                    return v;
-                },
-                [&](long e, const V4H<T, MAXM> &v) {
-                    const T dr = v.a - v.b;
-                    Yh[e] = dr;
-                    Sh[e] = (v.c - v.d) - dr;
-                    const double drd = (double) dr, rd = (double) v.a;
-#pragma unroll
-                    for (int s2 = 0; s2 < MAXM; s2++) {
-                        if (s2 < m) {
-                            const double ys = (s2 == head) ? drd : (double) v.h[s2];
-                            acc[s2] += drd * ys;
-                            acc[MAXM + s2] += ys * rd;
-                        }
-                    }
-                });
-#pragma unroll
-            for (int q = 0; q < 2 * MAXM; q++)
-                if ((q & (MAXM - 1)) < m) acc[q] = E::sum(acc[q], lds, c);
-#pragma unroll
-            for (int s2 = 0; s2 < MAXM; s2++) {
-                if (s2 < m) {
-#pragma unroll
-                    for (int hh = 0; hh < MAXM; hh++) {
-                        if (hh == head) {
-                            G[hh][s2] = acc[s2];
-                            G[s2][hh] = acc[s2];
-                        }
-                    }
-                    gb[s2] = acc[MAXM + s2];
-                }
-            }
-        }
         clk.mark(PH_ACOL);
         {  // saveToPrev as role moves
             T *t = ZP;

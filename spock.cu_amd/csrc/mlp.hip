@@ -573,14 +573,40 @@ int spk_mlp_adam(spk_mlp *m, float lr, float b1, float b2, float eps) {
     }
 }
 
+// The training set: targets already in trT (N rows), inputs x0 (host, N rows).
+static void finishDataset(spk_mlp *m, int N, const double *x0);
+
 int spk_mlp_dataset(spk_mlp *m, spk_batch *solved, const double *x0) {
     try {
         const int N = spk_batch_count(solved);
+        const long out = m->out();
+        m->trT.alloc((size_t) N * out * 4);
+        if (spk_batch_get_iterates_device(solved, m->trT.as<float>(), out)) throw std::runtime_error(m->ctx->err);
+        finishDataset(m, N, x0);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(m->ctx, e);
+    }
+}
+
+int spk_mlp_dataset_lanes(spk_mlp *m, spk_batch *solved, int count, const int *lanes, const double *x0) {
+    try {
+        if (count < 1) throw std::runtime_error("an empty training set");
+        const long out = m->out();
+        m->trT.alloc((size_t) count * out * 4);
+        if (spk_batch_get_lanes_device(solved, count, lanes, m->trT.as<float>(), out)) throw std::runtime_error(m->ctx->err);
+        finishDataset(m, count, x0);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(m->ctx, e);
+    }
+}
+
+static void finishDataset(spk_mlp *m, int N, const double *x0) {
+    {
         const int in = m->in();
         const long out = m->out();
         m->nTrain = N;
-        m->trT.alloc((size_t) N * out * 4);
-        if (spk_batch_get_iterates_device(solved, m->trT.as<float>(), out)) throw std::runtime_error(m->ctx->err);
         hipStream_t st = m->ctx->stream;
         hipLaunchKernelGGL(M::k_col_stats, dim3((unsigned) ((out + 255) / 256)), dim3(256), 0, st, m->trT.as<float>(),
                            (long) N, out, m->outMean.as<float>(), m->outStd.as<float>());
@@ -606,9 +632,6 @@ int spk_mlp_dataset(spk_mlp *m, spk_batch *solved, const double *x0) {
         m->trX.alloc(xs.size() * 4);
         check(hipMemcpy(m->trX.p, xs.data(), xs.size() * 4, hipMemcpyHostToDevice), "inputs");
         check(hipStreamSynchronize(st), "sync");
-        return 0;
-    } catch (const std::exception &e) {
-        return fail(m->ctx, e);
     }
 }
 

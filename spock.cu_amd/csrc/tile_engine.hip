@@ -26,8 +26,18 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+// Launch bound of this build: 512 threads (two waves per SIMD, up to 256 registers each) or 1024
+// (four waves per SIMD, 128 registers each: more waves to hide memory latency, more spills).  The
+// library is built once per bound; the 1024 build's kernels carry the suffix _x1024.
 #ifndef SPK_TILE_MAX_THREADS
 #define SPK_TILE_MAX_THREADS 512
+#endif
+#if SPK_TILE_MAX_THREADS == 512
+#define SPK_TILE_KERNEL(name) name
+#else
+#define SPK_TILE_KERNEL_CAT2(a, b) a##_x##b
+#define SPK_TILE_KERNEL_CAT(a, b) SPK_TILE_KERNEL_CAT2(a, b)
+#define SPK_TILE_KERNEL(name) SPK_TILE_KERNEL_CAT(name, SPK_TILE_MAX_THREADS)
 #endif
 #ifndef SPK_TILE_KB
 #define SPK_TILE_KB 4  // k-steps of 4 columns whose operands are loaded together
@@ -825,7 +835,7 @@ struct Policy {
 }  // namespace tile
 
 template <typename T>
-__global__ void __launch_bounds__(SPK_TILE_MAX_THREADS) k_tile_solve(const LaneInst<T> *__restrict__ P, Out *outs) {
+__global__ void __launch_bounds__(SPK_TILE_MAX_THREADS) SPK_TILE_KERNEL(k_tile_solve)(const LaneInst<T> *__restrict__ P, Out *outs) {
     __shared__ double lds[tile::MAXW * tile::W];
     __shared__ long phaseAcc[NUM_PHASES];
     lockstepSolve<T, tile::Policy>(P, outs, lds, phaseAcc);
@@ -833,7 +843,7 @@ __global__ void __launch_bounds__(SPK_TILE_MAX_THREADS) k_tile_solve(const LaneI
 
 // Single operators on every instance (parity tests): z = zio, cand = work[1].
 template <typename T>
-__global__ void __launch_bounds__(SPK_TILE_MAX_THREADS) k_tile_debug(const LaneInst<T> *__restrict__ Pp, Out *outs,
+__global__ void __launch_bounds__(SPK_TILE_MAX_THREADS) SPK_TILE_KERNEL(k_tile_debug)(const LaneInst<T> *__restrict__ Pp, Out *outs,
                                                                      int op) {
     using namespace tile;
     __shared__ double lds[MAXW * W];
@@ -871,9 +881,9 @@ __global__ void __launch_bounds__(SPK_TILE_MAX_THREADS) k_tile_debug(const LaneI
     }
 }
 
-template __global__ void k_tile_solve<float>(const LaneInst<float> *, Out *);
-template __global__ void k_tile_solve<double>(const LaneInst<double> *, Out *);
-template __global__ void k_tile_debug<float>(const LaneInst<float> *, Out *, int);
-template __global__ void k_tile_debug<double>(const LaneInst<double> *, Out *, int);
+template __global__ void SPK_TILE_KERNEL(k_tile_solve)<float>(const LaneInst<float> *, Out *);
+template __global__ void SPK_TILE_KERNEL(k_tile_solve)<double>(const LaneInst<double> *, Out *);
+template __global__ void SPK_TILE_KERNEL(k_tile_debug)<float>(const LaneInst<float> *, Out *, int);
+template __global__ void SPK_TILE_KERNEL(k_tile_debug)<double>(const LaneInst<double> *, Out *, int);
 
 }  // namespace spk

@@ -30,11 +30,12 @@ EXPORTS = [
     "spk_batch_set_iterates", "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates",
     "spk_batch_solve", "spk_batch_resume", "spk_batch_results", "spk_batch_debug_op",
     "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
-    "spk_batch_refill", "spk_batch_refill_from_batch", "spk_batch_refill_rows",
+    "spk_batch_refill", "spk_batch_refill_from_batch", "spk_batch_refill_rows", "spk_batch_get_lanes_device",
+    "spk_batch_pool_open", "spk_batch_set_chunk",
     "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
     "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
     "spk_mlp_loss_backward", "spk_mlp_adam", "spk_mlp_dataset", "spk_mlp_train", "spk_mlp_predict",
-    "spk_mlp_predict_lanes",
+    "spk_mlp_predict_lanes", "spk_mlp_dataset_lanes",
 ]
 
 
@@ -122,6 +123,9 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_refill.argtypes = [vp, C.c_int, _ip, _dp]
     lib.spk_batch_refill_from_batch.argtypes = [vp, C.c_int, _ip, _dp, vp, _ip]
     lib.spk_batch_refill_rows.argtypes = [vp, C.c_int, _ip, _dp, vp, C.c_long]
+    lib.spk_batch_get_lanes_device.argtypes = [vp, C.c_int, _ip, vp, C.c_long]
+    lib.spk_batch_pool_open.argtypes = [vp, C.POINTER(Options)]
+    lib.spk_batch_set_chunk.argtypes = [vp, C.c_long]
     _fp = C.POINTER(C.c_float)
     lib.spk_mlp_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)]
     lib.spk_mlp_destroy.argtypes = [vp]
@@ -138,6 +142,7 @@ def load_library(path=LIB_PATH):
     lib.spk_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _dp]
     lib.spk_mlp_predict.argtypes = [vp, _dp, C.c_int, vp]
     lib.spk_mlp_predict_lanes.argtypes = [vp, _dp, C.c_int, vp, _ip]
+    lib.spk_mlp_dataset_lanes.argtypes = [vp, vp, C.c_int, _ip, _dp]
     _LIB = lib
     return lib
 
@@ -246,12 +251,13 @@ class Batch:
         self.ctx.check(self.ctx.lib.spk_batch_reset(self.h), "reset")
 
     def set_schedule(self, mv, block=0):
-        """Engine and schedule of a shared batch: mv 16 the tile engine (MFMA, 16 instances per
-        workgroup, the default), 0 or 8 the lane engine (64 per workgroup; scalar or LDS-staged factor
-        loads).  block: workgroup size (0: the engine's default)."""
+        """Engine and schedule of a shared batch: mv 17 (the default) or 16 the tile engine (MFMA, 16
+        instances per workgroup; built for up to 1024 or 512 threads), 0 or 8 the lane engine (64 per
+        workgroup; scalar or LDS-staged factor loads).  block: workgroup size (0: the schedule's
+        default)."""
         self.ctx.check(self.ctx.lib.spk_batch_set_schedule(self.h, int(mv), int(block)), "set_schedule")
 
-    AUTOTUNE_CANDIDATES = [(0, 1024), (8, 1024), (0, 512), (8, 512), (16, 256), (16, 512)]
+    AUTOTUNE_CANDIDATES = [(0, 1024), (8, 1024), (0, 512), (8, 512), (16, 256), (16, 512), (17, 1024)]
 
     def autotune(self, opts=None, iters=50):
         """Time the engines and schedules on this batch and keep the fastest; returns
@@ -358,6 +364,14 @@ class Batch:
         self.ctx.check(self.ctx.lib.spk_batch_refill_from_batch(self.h, lanes.size, lanes.ctypes.data_as(ip), _d(x0),
                                                                 src.h, src_lanes.ctypes.data_as(ip)), "refill_from")
 
+    def pool_open(self, opts):
+        """An empty pool: every slot idle until a refill (``spk_batch_pool_open``)."""
+        self.ctx.check(self.ctx.lib.spk_batch_pool_open(self.h, C.byref(opts)), "pool_open")
+
+    def set_chunk(self, chunk_iters):
+        """Outer iterations per launch of the following resumes (``spk_batch_set_chunk``)."""
+        self.ctx.check(self.ctx.lib.spk_batch_set_chunk(self.h, int(chunk_iters)), "set_chunk")
+
     def solve_chunked(self, opts, warm=False, on_chunk=None):
         """Run a chunked solve to its end; on_chunk(results, kernel_seconds) after every chunk.
         Returns the final results and the summed kernel seconds."""
@@ -455,6 +469,16 @@ class WarmStartMLP:
     def dataset(self, solved_batch, x0):
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         self.ctx.check(self.ctx.lib.spk_mlp_dataset(self.h, solved_batch.h, _d(x0)), "mlp dataset")
+
+    def dataset_lanes(self, solved_batch, lanes, x0):
+        """Training set from slots ``lanes`` of a solved shared batch (x0: their initial states)."""
+        lanes = np.ascontiguousarray(np.asarray(lanes, dtype=np.int32).ravel())
+        if lanes.size == 0:
+            raise ValueError("mlp dataset_lanes: no slots given (no solved instance to learn from)")
+        x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64).reshape(lanes.size, -1))
+        self.ctx.check(self.ctx.lib.spk_mlp_dataset_lanes(self.h, solved_batch.h, lanes.size,
+                                                          lanes.ctypes.data_as(C.POINTER(C.c_int)), _d(x0)),
+                       "mlp dataset_lanes")
 
     def train(self, epochs, minibatch=256, lr=1e-3, seed=0):
         losses = np.zeros(epochs)

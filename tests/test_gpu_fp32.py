@@ -91,10 +91,11 @@ def test_fp32_per_instance_operators_match_fp64_oracle(ctx, problem_dirs, name):
     check_ops(o, b, zs, ys, x0s, count, name)
 
 
-@pytest.mark.parametrize("engine", [16, 0])
+@pytest.mark.parametrize("engine", [17, 16, 0])
 @pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine"])
 def test_fp32_shared_operators_match_fp64_oracle(ctx, problem_dirs, name, engine):
-    """engine 16: the tile engine (matrix-core products); 0: the lane engine."""
+    """engine 17, 16: the tile engine (matrix-core products; 1024- and 512-thread builds); 0: the
+    lane engine."""
     d = problem_dirs[name]
     o = Oracle(d)
     p = rt.Problem(ctx, d)

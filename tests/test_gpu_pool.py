@@ -33,20 +33,21 @@ def fresh(ctx, p, x0s, iters, engine, warm=None):
     return res, b.iterates().reshape(len(x0s), -1)
 
 
-def run_out(b):
-    """Resume a chunked solve until no slot is running; the final results."""
-    while True:
+def run_out(b, slots=None):
+    """Resume a chunked solve until none of `slots` (default: all) is running; the final results."""
+    for _ in range(1000):
         res, _ = b.results()
-        if all(r["status"] != -1 for r in res):
+        if all(res[k]["status"] != -1 for k in (slots if slots is not None else range(len(res)))):
             return res
         b.resume()
+    raise AssertionError("the solve did not end")
 
 
 def counts(r):
     return (r["status"], r["iters"], r["k0"], r["k1"], r["k2"], r["k2bt"], r["k3"])
 
 
-@pytest.mark.parametrize("engine", [16, 0])
+@pytest.mark.parametrize("engine", [17, 16, 0])
 def test_refilled_slots_match_fresh_solves(ctx, problem_dirs, engine):
     """20 slots solve 50 iterations (two chunks of 25); 12 of them are refilled with new initial
     states (cold) and, from the accepted iterates of another batch, 5 more with warm starts; after
@@ -111,3 +112,36 @@ def test_refill_rejects_bad_requests(ctx, problem_dirs):
         b.refill([4], x0[:1])
     with pytest.raises(rt.SpockError, match="twice"):
         b.refill([1, 1], x0[:2])
+
+
+@pytest.mark.parametrize("engine", [17, 0])
+def test_empty_pool_runs_only_filled_slots(ctx, problem_dirs, engine):
+    """spk_batch_pool_open: every slot idle (SPK_NOT_RUN) until a refill; 9 slots spread over the
+    workgroups get cold solves of 50 iterations and match fresh solves; the idle slots stay idle."""
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    p = rt.Problem(ctx, d)
+    count, iters = 40, 50
+    b = rt.Batch(ctx, [p] * count, rt.F64, shared=True, engine=engine)
+    b.pool_open(rt.options(tol_abs=1e-30, max_iters=iters, chunk_iters=20))
+    slots = [0, 3, 15, 16, 17, 31, 32, 38, 39]
+    x0 = configs.initial_states(23, o.nx, len(slots), "uniform")
+    b.refill(slots, x0)
+    b.resume()
+    res = run_out(b, slots)
+    ref, zr = fresh(ctx, p, x0, iters, engine)
+    z = b.iterates().reshape(count, -1)
+    for i, k in enumerate(slots):
+        assert counts(res[k]) == counts(ref[i]), k
+        assert rel_err(z[k], zr[i]) < 1e-10
+    for k in set(range(count)) - set(slots):
+        assert res[k]["status"] == -1 and res[k]["iters"] == 0
+    # a second round in other slots, with a shorter chunk
+    b.set_chunk(10)
+    b.refill([1, 2], x0[:2])
+    b.resume()
+    res = run_out(b, [1, 2])
+    z = b.iterates().reshape(count, -1)
+    for i, k in enumerate([1, 2]):
+        assert counts(res[k]) == counts(ref[i])
+        assert rel_err(z[k], zr[i]) < 1e-10

@@ -1,6 +1,7 @@
 """Shared-problem batches vs the FP64 CPU oracle, through the C ABI: the tile engine (16 instances
 per workgroup, per-node products on the matrix cores, schedule 16, the default) and the lane
-engine (64 instances per wave, schedule 0), both in an interleaved layout.
+engine (64 instances per wave, schedule 0), both in an interleaved layout; the tile engine in its
+1024-thread build (schedule 17, the default) and its 512-thread build (16).
 
 The lane engine solves its Anderson least squares from the Gram matrix and forms SuperMann's
 M-inner products from adjoints only (<x, M y> = x'y - gamma (x_p' L* y_d + (L* x_d)' y_p)), so
@@ -54,7 +55,7 @@ def shared(ctx, d, count, dtype=rt.F64, engine=None):
     return p, rt.Batch(ctx, [p] * count, dtype, shared=True, engine=engine)
 
 
-ENGINES = [16, 0]
+ENGINES = [17, 16, 0]
 
 
 @pytest.mark.parametrize("engine", ENGINES)
