@@ -161,6 +161,19 @@ void *spk_batch_iterate_ptr(spk_batch *b, int k);
 int spk_batch_copy_iterates(spk_batch *dst, const spk_batch *src);
 /* enqueue the solve of every instance; warm != 0 keeps the current iterates as previous solution */
 int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm);
+/* Allocation-free variant of spk_batch_solve (no counterpart in the reference).  For callers that
+ * cannot allocate device memory: code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc.  It calls no hipMalloc, hipFree or other allocating routine.  Per-instance
+ * batches (spk_batch_create) only.  The solve's instance descriptors, Anderson rings and grid-mode
+ * partials live in one fixed scratch array per device of SPK_NOALLOC_SCRATCH_BYTES, held until the
+ * process ends; a batch needing more is rejected with SPK_ERR and a message naming both sizes
+ * (the scratch holds sizeof descriptor x count + (3m+1) x size_iterate words per instance for
+ * anderson_buffer m, + 256 KiB of partials for a single instance).  The iterates, results and
+ * timing are the batch's as after spk_batch_solve, and equal to them.  Every call on a device shares
+ * that device's scratch: calls that overlap on one GPU (batches of different contexts, or threads)
+ * race and must be serialised by the caller; calls on one context are ordered by its stream. */
+#define SPK_NOALLOC_SCRATCH_BYTES (64u << 20)
+int spk_batch_solve_noalloc(spk_batch *b, const spk_options *opts, int warm);
 /* continue a chunked solve of a shared batch for another chunk_iters outer iterations; the
  * continued iterates, branch decisions and counters are those of one uninterrupted solve.  Results
  * after a chunk report SPK_NOT_RUN (and the iterations so far) for instances still running. */

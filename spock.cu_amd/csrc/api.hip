@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -15,6 +16,17 @@
 #include "engine.hpp"
 #include "loader.hpp"
 #include "common.hpp"
+
+// The scratch of spk_batch_solve_noalloc: the instance descriptors, Anderson rings and grid-mode
+// partials of one per-instance solve, laid out from offset 0 (256-byte aligned regions).  Each
+// device holds its own copy until the process ends.  Host code never names the array: it uses the
+// address hipGetSymbolAddress returned for the device (scratchOf), taken once per device by
+// spk_ctx_create with that device current.  No region needs to start at zero: the descriptors are
+// copied in before the launch, an Anderson column enters the least-squares system only once the
+// solve has written it (the first m iterations take the plain step), and every grid partial is
+// written before the grid barrier that precedes its read.  spk_batch_solve reuses its own rings
+// uncleared for the same reason.
+static __device__ __align__(256) unsigned char scratch[SPK_NOALLOC_SCRATCH_BYTES];
 
 namespace spk {
 template <typename T, int R, int V>
@@ -433,6 +445,69 @@ void launchSolve(spk_batch *b, const spk_options &o, int warm) {
     b->pending = true;
 }
 
+// ---------------------------------------------------------------- allocation-free solves
+// Host side of the scratch array (declared at the top of this file).
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratchMu;
+void *scratchAddr[SCRATCH_DEVICES] = {};
+
+void initScratch(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return;  // spk_batch_solve_noalloc reports it
+    std::lock_guard<std::mutex> lock(scratchMu);
+    if (scratchAddr[device]) return;
+    void *ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    scratchAddr[device] = ptr;
+}
+
+char *scratchOf(int device) {
+    void *ptr = nullptr;
+    if (device >= 0 && device < SCRATCH_DEVICES) {
+        std::lock_guard<std::mutex> lock(scratchMu);
+        ptr = scratchAddr[device];
+    }
+    if (!ptr) throw std::runtime_error("no allocation-free scratch on device " + std::to_string(device));
+    return static_cast<char *>(ptr);
+}
+
+// Bytes of scratch a per-instance solve of b with Anderson buffer m needs, and the element
+// offsets of its instances' rings.
+template <typename T>
+size_t scratchLayout(const spk_batch *b, int m, std::vector<size_t> &andOff, size_t &andAt, size_t &gpartAt) {
+    andOff.resize(b->count);
+    size_t el = 0;
+    for (int k = 0; k < b->count; k++) {
+        andOff[k] = el;
+        el += andCount(b->probs[k]->hp, m);
+    }
+    andAt = alignUp(sizeof(spk::Inst<T>) * b->count, 256);
+    gpartAt = alignUp(andAt + el * sizeof(T), 256);
+    return gpartAt + (b->count == 1 ? (size_t) 2 * spk::GRID_MAXK * spk::GRID_MAXWG * sizeof(T) : 0);
+}
+
+template <typename T>
+void launchSolveNoAlloc(spk_batch *b, const spk_options &o, int warm) {
+    std::vector<size_t> andOff;
+    size_t andAt = 0, gpartAt = 0;
+    const size_t need = scratchLayout<T>(b, o.anderson_buffer, andOff, andAt, gpartAt);
+    if (need > SPK_NOALLOC_SCRATCH_BYTES)
+        throw std::runtime_error("the allocation-free solve of this batch needs " + std::to_string(need) +
+                                 " bytes of scratch; the limit is " + std::to_string(SPK_NOALLOC_SCRATCH_BYTES) +
+                                 " (SPK_NOALLOC_SCRATCH_BYTES)");
+    if (b->dOut.bytes < sizeof(spk::Out) * b->count) throw std::runtime_error("the batch has no result records");
+    char *base = scratchOf(b->ctx->device);
+    fillInsts<T>(b, o, warm, reinterpret_cast<T *>(base + andAt), andOff,
+                 b->count == 1 ? reinterpret_cast<T *>(base + gpartAt) : nullptr);
+    check(hipMemcpyAsync(base, b->hInst.data(), b->hInst.size(), hipMemcpyHostToDevice, b->ctx->stream),
+          "upload instances");
+    check(hipEventRecord(b->ev0, b->ctx->stream), "event");
+    launchKernel<T>(b, solveBlock(o), false, 0, base);
+    check(hipGetLastError(), "launch k_solve");
+    check(hipEventRecord(b->ev1, b->ctx->stream), "event");
+    b->pending = true;
+}
+
+
 // ---------------------------------------------------------------- shared-problem (lane) batches
 // Regions of the interleaved state, each (rows x Bpad): work[0..8] and zio, rio (sIter rows),
 // the Anderson rings hS, hY (LANE_MAXM slots of sIter rows), qd (nx*N), dd (nu*nl), x0 (nx).
@@ -781,6 +856,7 @@ int spk_ctx_create(int device, spk_ctx **out) {
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
             c->tickRateHz = khz * 1000.0;
+        initScratch(device);
         *out = c;
         return 0;
     } catch (const std::exception &e) {
@@ -890,6 +966,7 @@ int spk_batch_create(spk_ctx *ctx, int count, spk_problem *const *problems, int 
         check(hipEventCreate(&b->ev0), "event");
         check(hipEventCreate(&b->ev1), "event");
         ensureAnderson(b.get(), 3);
+        b->dOut.alloc(sizeof(spk::Out) * count);  // result records (spk_batch_solve_noalloc writes here)
         check(hipStreamSynchronize(ctx->stream), "sync");
         *out = b.release();
         return 0;
@@ -1287,6 +1364,26 @@ int spk_batch_solve(spk_batch *b, const spk_options *opts, int warm) {
         ensureAnderson(b, o.anderson_buffer);
         if (b->dtype == SPK_F64) launchSolve<double>(b, o, warm);
         else launchSolve<float>(b, o, warm);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_solve_noalloc(spk_batch *b, const spk_options *opts, int warm) {
+    try {
+        spk_options o;
+        if (opts) o = *opts;
+        else spk_default_options(&o);
+        if (b->lanes) throw std::runtime_error("spk_batch_solve_noalloc takes per-instance batches (spk_batch_create)");
+        if (o.anderson_buffer < 1 || o.anderson_buffer > 8) throw std::runtime_error("anderson_buffer must be 1..8");
+        if (o.max_inner < 0) throw std::runtime_error("max_inner must be >= 0");
+        if (o.ls_method != SPK_LS_HOUSEHOLDER && o.ls_method != SPK_LS_GRAM)
+            throw std::runtime_error("ls_method must be SPK_LS_HOUSEHOLDER or SPK_LS_GRAM");
+        if (o.chunk_iters > 0) throw std::runtime_error("chunked solves need a shared batch");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        if (b->dtype == SPK_F64) launchSolveNoAlloc<double>(b, o, warm);
+        else launchSolveNoAlloc<float>(b, o, warm);
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);

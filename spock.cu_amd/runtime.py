@@ -28,7 +28,7 @@ EXPORTS = [
     "spk_batch_autotune_workgroups", "spk_batch_solve_graph", "spk_batch_resume_graph", "spk_batch_reset",
     "spk_batch_set_x0", "spk_batch_set_tolerances", "spk_batch_set_iterate", "spk_batch_get_iterate",
     "spk_batch_set_iterates", "spk_batch_get_iterates", "spk_batch_iterate_ptr", "spk_batch_copy_iterates",
-    "spk_batch_solve", "spk_batch_resume", "spk_batch_results", "spk_batch_debug_op",
+    "spk_batch_solve", "spk_batch_solve_noalloc", "spk_batch_resume", "spk_batch_results", "spk_batch_debug_op",
     "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
     "spk_batch_refill", "spk_batch_refill_from_batch", "spk_batch_refill_rows", "spk_batch_get_lanes_device",
     "spk_batch_pool_open", "spk_batch_set_chunk",
@@ -114,6 +114,7 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_iterate_ptr.restype = vp
     lib.spk_batch_copy_iterates.argtypes = [vp, vp]
     lib.spk_batch_solve.argtypes = [vp, C.POINTER(Options), C.c_int]
+    lib.spk_batch_solve_noalloc.argtypes = [vp, C.POINTER(Options), C.c_int]
     lib.spk_batch_resume.argtypes = [vp]
     lib.spk_batch_results.argtypes = [vp, C.POINTER(Result), _dp]
     lib.spk_batch_debug_op.argtypes = [vp, C.c_int, _dp, _dp, _dp]
@@ -326,6 +327,12 @@ class Batch:
     def solve(self, opts=None, warm=False):
         o = opts if opts is not None else options()
         self.ctx.check(self.ctx.lib.spk_batch_solve(self.h, C.byref(o), 1 if warm else 0), "solve")
+
+    def solve_noalloc(self, opts=None, warm=False):
+        """spk_batch_solve without device allocations (per-instance batches; the solve's working
+        memory is the device's fixed scratch of SPK_NOALLOC_SCRATCH_BYTES, 64 MiB)."""
+        o = opts if opts is not None else options()
+        self.ctx.check(self.ctx.lib.spk_batch_solve_noalloc(self.h, C.byref(o), 1 if warm else 0), "solve_noalloc")
 
     def resume(self):
         """Continue a chunked solve (shared batches, options.chunk_iters > 0)."""
