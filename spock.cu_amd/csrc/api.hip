@@ -1108,11 +1108,28 @@ int spk_batch_set_tolerances(spk_batch *b, const double *tol) {
     }
 }
 
+// Iterates cross the host boundary in the reference's order of the dual blocks; for
+// polyhedronWithIdentity the engines keep the constraint blocks per node (spk::reorderIdentityBlocks).
+// `rows` consecutive iterates of problem h in the engines' order: z itself, or a reordered copy in tmp.
+const double *engineOrder(const HostProblem &h, const double *z, size_t rows, std::vector<double> &tmp) {
+    if (!h.id3 && !h.id5) return z;
+    tmp.assign(z, z + rows * h.sIter);
+    for (size_t r = 0; r < rows; r++) spk::reorderIdentityBlocks(h, tmp.data() + r * h.sIter, true);
+    return tmp.data();
+}
+
+void referenceOrder(const HostProblem &h, double *z, size_t rows) {
+    if (!h.id3 && !h.id5) return;
+    for (size_t r = 0; r < rows; r++) spk::reorderIdentityBlocks(h, z + r * h.sIter, false);
+}
+
 int spk_batch_set_iterate(spk_batch *b, int k, const double *z) {
     try {
         if (k < 0 || k >= b->count) throw std::runtime_error("instance index out of range");
         check(hipStreamSynchronize(b->ctx->stream), "sync");
         const size_t n = b->probs[k]->hp.sIter;
+        std::vector<double> tmp;
+        z = engineOrder(b->probs[k]->hp, z, 1, tmp);
         if (b->lanes) {
             if (b->dtype == SPK_F64) laneColumn<double>(b, LR_ZIO, k, nullptr, z, n);
             else laneColumn<float>(b, LR_ZIO, k, nullptr, z, n);
@@ -1134,10 +1151,12 @@ int spk_batch_get_iterate(spk_batch *b, int k, double *z) {
         if (b->lanes) {
             if (b->dtype == SPK_F64) laneColumn<double>(b, LR_ZIO, k, z, nullptr, n);
             else laneColumn<float>(b, LR_ZIO, k, z, nullptr, n);
-            return 0;
+        } else if (b->dtype == SPK_F64) {
+            toHost<double>(b, z, corePtr<double>(b, k), n);
+        } else {
+            toHost<float>(b, z, corePtr<float>(b, k), n);
         }
-        if (b->dtype == SPK_F64) toHost<double>(b, z, corePtr<double>(b, k), n);
-        else toHost<float>(b, z, corePtr<float>(b, k), n);
+        referenceOrder(b->probs[k]->hp, z, 1);
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);
@@ -1148,6 +1167,8 @@ int spk_batch_set_iterates(spk_batch *b, const double *z) {
     if (b->lanes) {
         try {
             check(hipStreamSynchronize(b->ctx->stream), "sync");
+            std::vector<double> tmp;
+            z = engineOrder(b->probs[0]->hp, z, b->count, tmp);
             if (b->dtype == SPK_F64) laneUpload<double>(b, LR_ZIO, z, b->probs[0]->hp.sIter);
             else laneUpload<float>(b, LR_ZIO, z, b->probs[0]->hp.sIter);
             return 0;
@@ -1169,6 +1190,7 @@ int spk_batch_get_iterates(spk_batch *b, double *z) {
             check(hipStreamSynchronize(b->ctx->stream), "sync");
             if (b->dtype == SPK_F64) laneDownload<double>(b, LR_ZIO, z, b->probs[0]->hp.sIter);
             else laneDownload<float>(b, LR_ZIO, z, b->probs[0]->hp.sIter);
+            referenceOrder(b->probs[0]->hp, z, b->count);
             return 0;
         } catch (const std::exception &e) {
             return fail(b->ctx, e);
@@ -1710,12 +1732,15 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
         if (b->lanes) {
             const size_t n = b->probs[0]->hp.sIter;
             const bool f64 = b->dtype == SPK_F64;
+            std::vector<double> tin, tout;
+            const double *inE = engineOrder(b->probs[0]->hp, in, b->count, tin);
+            const double *outE = engineOrder(b->probs[0]->hp, out, b->count, tout);
             if (f64) {
-                laneUpload<double>(b, LR_ZIO, in, n);
-                laneUpload<double>(b, 1, out, n);
+                laneUpload<double>(b, LR_ZIO, inE, n);
+                laneUpload<double>(b, 1, outE, n);
             } else {
-                laneUpload<float>(b, LR_ZIO, in, n);
-                laneUpload<float>(b, 1, out, n);
+                laneUpload<float>(b, LR_ZIO, inE, n);
+                laneUpload<float>(b, 1, outE, n);
             }
             spk_options o;
             spk_default_options(&o);
@@ -1743,6 +1768,7 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
             check(hipStreamSynchronize(b->ctx->stream), "sync debug op");
             if (f64) laneDownload<double>(b, 1, out, n);
             else laneDownload<float>(b, 1, out, n);
+            referenceOrder(b->probs[0]->hp, out, b->count);
             std::vector<spk::Out> res(b->count);
             check(hipMemcpy(res.data(), b->dOut.p, sizeof(spk::Out) * b->count, hipMemcpyDeviceToHost), "results");
             if (scalars)
@@ -1755,12 +1781,15 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
         size_t pos = 0;
         for (int k = 0; k < b->count; k++) {
             const size_t n = b->probs[k]->hp.sIter;
+            std::vector<double> tin, tout;
+            const double *inE = engineOrder(b->probs[k]->hp, in + pos, 1, tin);
+            const double *outE = engineOrder(b->probs[k]->hp, out + pos, 1, tout);
             if (b->dtype == SPK_F64) {
-                toDevice<double>(b, corePtr<double>(b, k), in + pos, n);
-                toDevice<double>(b, corePtr<double>(b, k) + 2 * n, out + pos, n);
+                toDevice<double>(b, corePtr<double>(b, k), inE, n);
+                toDevice<double>(b, corePtr<double>(b, k) + 2 * n, outE, n);
             } else {
-                toDevice<float>(b, corePtr<float>(b, k), in + pos, n);
-                toDevice<float>(b, corePtr<float>(b, k) + 2 * n, out + pos, n);
+                toDevice<float>(b, corePtr<float>(b, k), inE, n);
+                toDevice<float>(b, corePtr<float>(b, k) + 2 * n, outE, n);
             }
             pos += n;
         }
@@ -1785,6 +1814,7 @@ int spk_batch_debug_op(spk_batch *b, int op, const double *in, double *out, doub
             const size_t n = b->probs[k]->hp.sIter;
             if (b->dtype == SPK_F64) toHost<double>(b, out + pos, corePtr<double>(b, k) + 2 * n, n);
             else toHost<float>(b, out + pos, corePtr<float>(b, k) + 2 * n, n);
+            referenceOrder(b->probs[k]->hp, out + pos, 1);
             if (scalars) {
                 scalars[2 * k] = res[k].w;
                 scalars[2 * k + 1] = res[k].err;

@@ -232,6 +232,56 @@ Table dedup(const std::vector<double> &stack, int rows, int cols, int count) {
     return t;
 }
 
+// PolyhedronWithIdentity (src/constraints.cuh:299-435): ilb <= z <= iub and glb <= Gamma z <= gub
+// on the same vector z (dimension nz), for every node of its part.  The engines run it as the
+// polyhedron whose matrix is [I; Gamma] (column-major, rows = nz + rows of Gamma) and whose bounds
+// are [ilb; glb], [iub; gub]: the same operator, adjoint and projection, with each node's dual
+// block ordered [identity part; Gamma part].  The reference orders the whole dual block as the
+// identity parts of every node, then the Gamma parts; iterates cross the host boundary in that
+// order (reorderIdentityBlocks).  No bound is relaxed at the root (unlike Rectangle's,
+// src/constraints.cuh:142-148).
+static void stackIdentity(const std::string &dir, const std::string &prefix, const std::vector<double> &G, int nz,
+                   int nodes, int &rows, Table &M, Table &LB, Table &UB) {
+    auto ilb = readFp(dir, prefix + "ILB"), iub = readFp(dir, prefix + "IUB");
+    auto glb = readFp(dir, prefix + "GLB"), gub = readFp(dir, prefix + "GUB");
+    expectSize(ilb, (size_t) nz, prefix + "ILB");
+    expectSize(iub, (size_t) nz, prefix + "IUB");
+    const int pg = (int) glb.size();
+    expectSize(gub, (size_t) pg, prefix + "GUB");
+    expectSize(G, (size_t) pg * nz, prefix + "Gamma");
+    rows = nz + pg;
+    std::vector<double> m((size_t) rows * nz, 0.);
+    for (int c = 0; c < nz; c++) {
+        m[(size_t) c * rows + c] = 1.;
+        for (int r = 0; r < pg; r++) m[(size_t) c * rows + nz + r] = G[(size_t) c * pg + r];
+    }
+    std::vector<double> lb = ilb, ub = iub;
+    lb.insert(lb.end(), glb.begin(), glb.end());
+    ub.insert(ub.end(), gub.begin(), gub.end());
+    M = single(m, rows, nz, nodes);
+    LB = single(lb, rows, 1, nodes);
+    UB = single(ub, rows, 1, nodes);
+}
+
+void reorderIdentityBlocks(const HostProblem &h, double *z, bool toEngine) {
+    auto one = [&](long off, int rows, int id, int nodes) {
+        if (!id) return;
+        const std::vector<double> tmp(z + off, z + off + (size_t) rows * nodes);
+        const int pg = rows - id;
+        for (int i = 0; i < nodes; i++)
+            for (int k = 0; k < rows; k++) {
+                // engine position i * rows + k <-> reference position: the identity part of node i
+                // (k < id), else row k - id of its Gamma part
+                const size_t eng = (size_t) i * rows + k;
+                const size_t ref = k < id ? (size_t) i * id + k : (size_t) nodes * id + (size_t) i * pg + (k - id);
+                if (toEngine) z[off + eng] = tmp[ref];
+                else z[off + ref] = tmp[eng];
+            }
+    };
+    one(h.sPrim + h.sI + h.sII, h.con3, h.id3, h.nl);
+    one(h.sPrim + h.sI + h.sII + h.sIII + h.sIV, h.con5, h.id5, h.nlf);
+}
+
 HostProblem loadProblem(const std::string &folderIn) {
     std::string dir = folderIn;
     if (!dir.empty() && dir.back() != '/') dir += '/';
@@ -266,12 +316,18 @@ HostProblem loadProblem(const std::string &folderIn) {
         if (s == "no") return (int) CON_NO;
         if (s == "rectangle") return (int) CON_RECT;
         if (s == "polyhedron") return (int) CON_POLY;
-        throw std::runtime_error("constraint type '" + s + "' is not supported (no, rectangle, polyhedron)");
+        if (s == "polyhedronWithIdentity") return (int) CON_PWI;
+        throw std::runtime_error("constraint type '" + s +
+                                 "' is not supported (no, rectangle, polyhedron, polyhedronWithIdentity)");
     };
     p.ncost = costKind(need(js, "cost.nonleaf"));
     p.lcost = costKind(need(js, "cost.leaf"));
     p.ncon = conKind(need(js, "constraint.nonleaf"));
     p.lcon = conKind(need(js, "constraint.leaf"));
+    // polyhedronWithIdentity runs as the polyhedron with matrix [I; Gamma] (see stackIdentity)
+    const bool pwiN = p.ncon == CON_PWI, pwiL = p.lcon == CON_PWI;
+    if (pwiN) p.ncon = CON_POLY;
+    if (pwiL) p.lcon = CON_POLY;
     if (need(js, "risk.type") != "avar") throw std::runtime_error("risk type must be avar");
     auto al = js.find("risk.alpha");
     if (al != js.end()) p.alpha = std::stod(al->second);
@@ -362,7 +418,12 @@ HostProblem loadProblem(const std::string &folderIn) {
         }
     }
     // constraints (src/constraints.cuh:120-150, 203-245)
-    if (p.ncon == CON_RECT) {
+    if (pwiN) {
+        auto G = readFp(dir, "nonleafConstraintGamma");
+        p.id3 = nx + nu;
+        stackIdentity(dir, "nonleafConstraint", G, p.id3, nl, p.con3, p.nG, p.nLB, p.nUB);
+        p.nGT = transposeTable(p.nG);
+    } else if (p.ncon == CON_RECT) {
         auto lb = readFp(dir, "nonleafConstraintILB"), ub = readFp(dir, "nonleafConstraintIUB");
         expectSize(lb, (size_t) nx + nu, "nonleafConstraintILB");
         expectSize(ub, (size_t) nx + nu, "nonleafConstraintIUB");
@@ -388,7 +449,12 @@ HostProblem loadProblem(const std::string &folderIn) {
         p.nLB = single(lb, p.con3, 1, nl);
         p.nUB = single(ub, p.con3, 1, nl);
     }
-    if (p.lcon == CON_RECT) {
+    if (pwiL) {
+        auto G = readFp(dir, "leafConstraintGamma");
+        p.id5 = nx;
+        stackIdentity(dir, "leafConstraint", G, p.id5, nlf, p.con5, p.lG, p.lLB, p.lUB);
+        p.lGT = transposeTable(p.lG);
+    } else if (p.lcon == CON_RECT) {
         auto lb = readFp(dir, "leafConstraintILB"), ub = readFp(dir, "leafConstraintIUB");
         expectSize(lb, (size_t) nx, "leafConstraintILB");
         expectSize(ub, (size_t) nx, "leafConstraintIUB");

@@ -18,7 +18,7 @@
 namespace spk {
 
 enum CostKind { COST_QUAD = 0, COST_QUADLIN = 1 };
-enum ConKind { CON_NO = 0, CON_RECT = 1, CON_POLY = 2 };
+enum ConKind { CON_NO = 0, CON_RECT = 1, CON_POLY = 2, CON_PWI = 3 };  // CON_PWI: as read; loaded as CON_POLY
 
 struct Table {                // distinct matrices (rows x cols, column major) + per-node index
     int rows = 0, cols = 0;
@@ -32,6 +32,7 @@ struct HostProblem {
     int N = 0, nl = 0, nlf = 0, S = 0, E = 0, nx = 0, nu = 0, yD = 0, nD = 0;
     int ncost = COST_QUAD, lcost = COST_QUAD, ncon = CON_NO, lcon = CON_NO;
     int con3 = 0, con5 = 0;   // per-node sizes of dual blocks III and V
+    int id3 = 0, id5 = 0;     // polyhedronWithIdentity: rows of the identity part (0 otherwise)
     bool precond = false;
     double stepSize = 0., alpha = 0.;
     std::vector<double> scaling;
@@ -55,6 +56,10 @@ struct HostProblem {
 
 // Reads a problem folder.  Throws std::runtime_error with a descriptive message on failure.
 HostProblem loadProblem(const std::string &folder);
+
+// polyhedronWithIdentity: one iterate z (size_iterate values) from the reference's order of the
+// constraint dual blocks to the engines' per-node order (toEngine) or back.  No-op otherwise.
+void reorderIdentityBlocks(const HostProblem &h, double *z, bool toEngine);
 
 // Minimal JSON reader for data.json: flattens nested objects into dotted keys.
 std::map<std::string, std::string> parseJsonFlat(const std::string &text);

@@ -36,6 +36,8 @@ def x0_for(name, nx, k=0):
         return configs.initial_states(7, nx, k + 1, "uniform")[k]
     if name == "affine":
         return configs.initial_states(8, nx, k + 1, "uniform")[k]
+    if name == "pwi":
+        return configs.initial_states(33, nx, k + 1, "uniform")[k]
     return configs.initial_states(2, nx, k + 1)[k]
 
 
@@ -50,7 +52,7 @@ def inert_zero(o, z):
 
 @pytest.mark.parametrize("name,wgs", [("fixture", 1), ("small", 1), ("mid", 1), ("poly", 1),
                                       ("mid", 7), ("poly", 3), ("mid", 128), ("nocon", 1), ("affine", 1),
-                                      ("affine", 5)])
+                                      ("affine", 5), ("pwi", 1), ("pwi", 3)])
 def test_operators_match_oracle(ctx, problem_dirs, name, wgs):
     """wgs > 1: grid mode, the instance spread over a cooperative launch of wgs workgroups."""
     d = problem_dirs[name]
@@ -83,7 +85,7 @@ def test_operators_match_oracle(ctx, problem_dirs, name, wgs):
     assert abs(sc[1] - np.max(np.abs(y / o.gamma - ell))) <= 1e-12 * np.max(np.abs(y / o.gamma))
 
 
-@pytest.mark.parametrize("name", ["fixture", "mid", "poly"])
+@pytest.mark.parametrize("name", ["fixture", "mid", "poly", "pwi"])
 def test_get_dual_matches_oracle(ctx, problem_dirs, name):
     """Cache::getDual (src/cache.cuh:371-376): the dual step from the iterate with a given primal
     point; with the primal prox of T(z) as that point it is the dual part of T(z)."""
@@ -104,7 +106,8 @@ def test_get_dual_matches_oracle(ctx, problem_dirs, name):
     ("fixture", 75, "spock", 1), ("small", 75, "spock", 1), ("small", 250, "cp", 1), ("fixture", 250, "cp", 1),
     ("mid", 50, "spock", 1), ("poly", 75, "spock", 1), ("poly", 250, "cp", 1),
     ("mid", 50, "spock", 64), ("poly", 75, "spock", 5), ("small", 250, "cp", 16), ("nocon", 75, "spock", 1),
-    ("affine", 75, "spock", 1), ("affine", 250, "cp", 1)])
+    ("affine", 75, "spock", 1), ("affine", 250, "cp", 1), ("pwi", 50, "spock", 1), ("pwi", 250, "cp", 1),
+    ("pwi", 50, "spock", 3)])
 def test_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method, wgs):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -137,7 +140,7 @@ def gpu_solve(ctx, problems, x0s, dtype=rt.F64, **opts):
     return b, res
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "poly", "affine"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly", "affine", "pwi"])
 def test_converged_solve_matches_oracle(ctx, problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)

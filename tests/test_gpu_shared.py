@@ -39,6 +39,8 @@ def x0s_for(name, nx, count):
         return list(configs.initial_states(7, nx, count, "uniform"))
     if name == "affine":
         return list(configs.initial_states(8, nx, count, "uniform"))
+    if name == "pwi":
+        return list(configs.initial_states(33, nx, count, "uniform"))
     return list(configs.initial_states(2, nx, count))
 
 
@@ -59,7 +61,7 @@ ENGINES = [17, 16, 0]
 
 
 @pytest.mark.parametrize("engine", ENGINES)
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "affine", "nocon"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "affine", "nocon", "pwi"])
 def test_shared_operators_match_oracle(ctx, problem_dirs, name, engine):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -106,7 +108,8 @@ def test_shared_operators_match_oracle(ctx, problem_dirs, name, engine):
 @pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("name,iters,method", [("fixture", 50, "spock"), ("small", 50, "spock"),
                                                ("mid", 50, "spock"), ("small", 250, "cp"), ("poly", 50, "spock"),
-                                               ("nocon", 50, "spock"), ("affine", 50, "spock")])
+                                               ("nocon", 50, "spock"), ("affine", 50, "spock"),
+                                               ("pwi", 50, "spock")])
 def test_shared_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method, engine):
     d = problem_dirs[name]
     o = Oracle(d)

@@ -49,7 +49,7 @@ def run(exe, folder):
     return subprocess.run([exe, folder], capture_output=True, text=True, timeout=120, env=env)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine", "pwi"])
 def test_loader_reads_every_family(loader_exe, problem_dirs, name):
     r = run(loader_exe, problem_dirs[name])
     assert r.returncode == 0, r.stdout + r.stderr
@@ -105,11 +105,78 @@ def test_missing_json_is_an_error(loader_exe, tmp_path):
 
 
 def test_unsupported_constraint_kind_is_named(loader_exe, problem_dirs, tmp_path):
-    """The reference's polyhedronWithIdentity constraint (src/constraints.cuh:299) is not on the
-    HIP path: the loader refuses it by name."""
+    """A constraint type the reference does not have (src/problem.cuh:74-92) is refused by name."""
     d = _copy(problem_dirs, tmp_path)
     info = json.load(open(d / "data.json"))
-    info["constraint"]["nonleaf"] = "polyhedronWithIdentity"
+    info["constraint"]["nonleaf"] = "ellipsoid"
     json.dump(info, open(d / "data.json", "w"))
     r = run(loader_exe, str(d))
-    assert r.returncode == 2 and "polyhedronWithIdentity" in r.stdout and "not supported" in r.stdout, r.stdout
+    assert r.returncode == 2 and "ellipsoid" in r.stdout and "not supported" in r.stdout, r.stdout
+
+
+REORDER = r'''
+#include <cstdio>
+#include <vector>
+#include "loader.hpp"
+int main(int argc, char **argv) {
+    spk::HostProblem p = spk::loadProblem(argv[1]);
+    std::vector<double> z(p.sIter);
+    FILE *f = std::fopen(argv[2], "rb");
+    if (std::fread(z.data(), sizeof(double), z.size(), f) != z.size()) return 3;
+    std::fclose(f);
+    spk::reorderIdentityBlocks(p, z.data(), true);
+    f = std::fopen(argv[3], "wb");
+    std::fwrite(z.data(), sizeof(double), z.size(), f);
+    std::fclose(f);
+    spk::reorderIdentityBlocks(p, z.data(), false);
+    f = std::fopen(argv[4], "wb");
+    std::fwrite(z.data(), sizeof(double), z.size(), f);
+    std::fclose(f);
+    std::printf("%d %d %d %d\n", p.con3, p.id3, p.con5, p.id5);
+    return 0;
+}
+'''
+
+
+def test_polyhedron_with_identity_dual_order(problem_dirs, tmp_path):
+    """polyhedronWithIdentity loads as the polyhedron [I; Gamma] (con3 = nx + nu + rows of Gamma),
+    and an iterate's constraint blocks move from the reference's order (identity parts of every
+    node, then the Gamma parts; src/constraints.cuh:340-373) to one block per node and back.  The
+    expected engine order is written out here index by index from the reference's layout."""
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    (tmp_path / "main.cpp").write_text(REORDER)
+    exe = tmp_path / "reorder"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    f"-I{CSRC}", str(tmp_path / "main.cpp"), os.path.join(CSRC, "loader.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=300)
+    from oracle.oracle import Oracle
+    d = problem_dirs["pwi"]
+    o = Oracle(d)
+    info = json.load(open(os.path.join(d, "data.json")))
+    nx, nu, nl, N = info["numStates"], info["numInputs"], info["numNonleafNodes"], info["numNodes"]
+    nlf = N - nl
+    pg3 = btio.read_flat(os.path.join(d, "nonleafConstraintGLB_d.bt")).size
+    pg5 = btio.read_flat(os.path.join(d, "leafConstraintGLB_d.bt")).size
+    z = np.random.default_rng(3).standard_normal(o.n_iter)
+    z.tofile(tmp_path / "in.bin")
+    r = subprocess.run([str(exe), d, str(tmp_path / "in.bin"), str(tmp_path / "eng.bin"), str(tmp_path / "back.bin")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    con3, id3, con5, id5 = (int(v) for v in r.stdout.split())
+    assert (con3, id3, con5, id5) == (nx + nu + pg3, nx + nu, nx + pg5, nx)
+    eng = np.fromfile(tmp_path / "eng.bin")
+    back = np.fromfile(tmp_path / "back.bin")
+    assert np.array_equal(back, z)
+    expect = z.copy()
+    o3 = o.n_prim + o.sI + o.sII
+    for i in range(nl):
+        expect[o3 + i * con3:o3 + i * con3 + id3] = z[o3 + i * id3:o3 + (i + 1) * id3]
+        g0 = o3 + nl * id3 + i * pg3
+        expect[o3 + i * con3 + id3:o3 + (i + 1) * con3] = z[g0:g0 + pg3]
+    o5 = o3 + o.sIII + o.sIV
+    for l in range(nlf):
+        expect[o5 + l * con5:o5 + l * con5 + id5] = z[o5 + l * id5:o5 + (l + 1) * id5]
+        g0 = o5 + nlf * id5 + l * pg5
+        expect[o5 + l * con5 + id5:o5 + (l + 1) * con5] = z[g0:g0 + pg5]
+    assert np.array_equal(eng, expect)
