@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libspock_oracle.so")
 
-_COST = {"quadratic": 0, "quadraticPlusLinear": 1}
+_COST = {"quadratic": 0, "quadraticPlusLinear": 1, "linear": 2}
 _CON = {"no": 0, "rectangle": 1, "polyhedron": 2, "polyhedronWithIdentity": 3}
 
 _dp = C.POINTER(C.c_double)

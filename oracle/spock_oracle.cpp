@@ -96,7 +96,7 @@ inline void gemvT(const double *M, int rows, int cols, const double *x, double *
 }
 
 enum { CON_NO = 0, CON_RECT = 1, CON_POLY = 2, CON_PWI = 3 };
-enum { COST_QUAD = 0, COST_QUADLIN = 1 };
+enum { COST_QUAD = 0, COST_QUADLIN = 1, COST_LIN = 2 };
 
 // ------------------------------------------------------------------- problem
 struct Problem {
@@ -111,6 +111,8 @@ struct Problem {
     bool affine = false;
     Vec e, Pe;  // affine dynamics: constant e_j and P_j e_j per node (src/dynamics.cuh:229-237)
     Vec sqrtQ, sqrtR, trans, q, r, lsqrtQ, ltrans, lq;
+    Vec grad, lgrad;  // linear costs: the gradient row [q' r'] per node (nonleaf), q' (leaf)
+    int d4 = 0, d6 = 0;  // entries per node of dual blocks IV and VI
     Vec nLB, nUB, nG, lLB, lUB, lG;  // per-node bounds (replicated) and matrices
     size_t sU, sX, sY, sT, sS, sPrim, sI, sII, sIII, sIV, sV, sVI, sDual, sIter;
     size_t oU, oX, oY, oT, oS;                  // offsets inside prim
@@ -152,11 +154,18 @@ struct Problem {
         NNtr = readFp(dir, "risk_NNtr");
         b = readFp(dir, "risk_b");
         if (precond) scaling = readFp(dir, "scaling");
-        sqrtQ = readFp(dir, "nonleafCost_sqrtQ");
-        sqrtR = readFp(dir, "nonleafCost_sqrtR");
-        trans = readFp(dir, "nonleafCost_translation");
-        lsqrtQ = readFp(dir, "leafCost_sqrtQ");
-        ltrans = readFp(dir, "leafCost_translation");
+        // src/costs.cuh:228-250: a linear cost reads only its gradient
+        if (ncost == COST_LIN) grad = readFp(dir, "nonleafCost_gradient");
+        else {
+            sqrtQ = readFp(dir, "nonleafCost_sqrtQ");
+            sqrtR = readFp(dir, "nonleafCost_sqrtR");
+            trans = readFp(dir, "nonleafCost_translation");
+        }
+        if (lcost == COST_LIN) lgrad = readFp(dir, "leafCost_gradient");
+        else {
+            lsqrtQ = readFp(dir, "leafCost_sqrtQ");
+            ltrans = readFp(dir, "leafCost_translation");
+        }
         if (ncost == COST_QUADLIN) { q = readFp(dir, "nonleafCost_q"); r = readFp(dir, "nonleafCost_r"); }
         else { q.assign((size_t) nx * N, 0.); r.assign((size_t) nu * N, 0.); }
         if (lcost == COST_QUADLIN) lq = readFp(dir, "leafCost_q");
@@ -222,8 +231,12 @@ struct Problem {
         // Iterate layout (src/cache.cuh:482-581)
         sU = (size_t) nu * nl; sX = (size_t) nx * N; sY = (size_t) yD * nl; sT = N; sS = N;
         sPrim = sU + sX + sY + sT + sS;
-        sI = (size_t) yD * nl; sII = nl; sIII = (size_t) con3 * nl; sIV = (size_t) (nxu + 2) * N;
-        sV = (size_t) con5 * nlf; sVI = (size_t) (nx + 2) * nlf;
+        // a linear cost's block is [cost of every node; epigraph variable of every node]
+        // (src/costs.cuh:232, 248, 254-258)
+        d4 = ncost == COST_LIN ? 2 : nxu + 2;
+        d6 = lcost == COST_LIN ? 2 : nx + 2;
+        sI = (size_t) yD * nl; sII = nl; sIII = (size_t) con3 * nl; sIV = (size_t) d4 * N;
+        sV = (size_t) con5 * nlf; sVI = (size_t) d6 * nlf;
         sDual = sI + sII + sIII + sIV + sV + sVI;
         sIter = sPrim + sDual;
         oU = 0; oX = oU + sU; oY = oX + sX; oT = oY + sY; oS = oT + sT;
@@ -285,23 +298,36 @@ void opL(const Problem &p, const double *prim, double *dual) {
             gemv(p.nG.data(), rows, nxu, idPart, gPart, false);
         }
     }
-    const int d4 = nxu + 2;
-    for (int k = 0; k < d4; k++) IV[k] = 0.;
-    for (int j = 1; j < p.N; j++) {
-        const long a = p.anc[j];
-        double *iv = IV + (size_t) j * d4;
-        const double *xa = x + (size_t) a * nx, *ua = u + (size_t) a * nu;
-        gemv(p.sqrtQ.data() + (size_t) j * nx * nx, nx, nx, xa, iv, false);
-        gemv(p.sqrtR.data() + (size_t) j * nu * nu, nu, nu, ua, iv + nx, false);
-        double half = t[j] * 0.5;
-        if (p.ncost == COST_QUADLIN) {
-            double qx = 0., ru = 0.;
-            for (int k = 0; k < nx; k++) qx += p.q[(size_t) j * nx + k] * xa[k];
-            for (int k = 0; k < nu; k++) ru += p.r[(size_t) j * nu + k] * ua[k];
-            half = half - 0.5 * qx - 0.5 * ru;
+    if (p.ncost == COST_LIN) {  // src/costs.cuh:261-267: sum_j = g_j [x_anc; u_anc] (node 0: 0), set = t
+        IV[0] = 0.;
+        for (int j = 1; j < p.N; j++) {
+            const long a = p.anc[j];
+            const double *g = p.grad.data() + (size_t) j * nxu;
+            double acc = 0.;
+            for (int k = 0; k < nx; k++) acc += g[k] * x[(size_t) a * nx + k];
+            for (int k = 0; k < nu; k++) acc += g[nx + k] * u[(size_t) a * nu + k];
+            IV[j] = acc;
         }
-        iv[nxu] = half;
-        iv[nxu + 1] = half;
+        for (int j = 0; j < p.N; j++) IV[p.N + j] = t[j];
+    } else {
+        const int d4 = nxu + 2;
+        for (int k = 0; k < d4; k++) IV[k] = 0.;
+        for (int j = 1; j < p.N; j++) {
+            const long a = p.anc[j];
+            double *iv = IV + (size_t) j * d4;
+            const double *xa = x + (size_t) a * nx, *ua = u + (size_t) a * nu;
+            gemv(p.sqrtQ.data() + (size_t) j * nx * nx, nx, nx, xa, iv, false);
+            gemv(p.sqrtR.data() + (size_t) j * nu * nu, nu, nu, ua, iv + nx, false);
+            double half = t[j] * 0.5;
+            if (p.ncost == COST_QUADLIN) {
+                double qx = 0., ru = 0.;
+                for (int k = 0; k < nx; k++) qx += p.q[(size_t) j * nx + k] * xa[k];
+                for (int k = 0; k < nu; k++) ru += p.r[(size_t) j * nu + k] * ua[k];
+                half = half - 0.5 * qx - 0.5 * ru;
+            }
+            iv[nxu] = half;
+            iv[nxu + 1] = half;
+        }
     }
     for (int l = 0; l < p.nlf; l++) {
         const double *xl = x + (size_t) (p.nl + l) * nx;
@@ -313,6 +339,13 @@ void opL(const Problem &p, const double *prim, double *dual) {
             double *idPart = V + (size_t) l * p.id5, *gPart = V + (size_t) p.nlf * p.id5 + (size_t) l * rows;
             for (int k = 0; k < nx; k++) idPart[k] = xl[k];
             gemv(p.lG.data(), rows, nx, xl, gPart, false);
+        }
+        if (p.lcost == COST_LIN) {  // src/costs.cuh:269-272
+            double acc = 0.;
+            for (int k = 0; k < nx; k++) acc += p.lgrad[k] * xl[k];
+            VI[l] = acc;
+            VI[p.nlf + l] = s[p.nl + l];
+            continue;
         }
         double *vi = VI + (size_t) l * (nx + 2);
         gemv(p.lsqrtQ.data(), nx, nx, xl, vi, false);
@@ -330,7 +363,7 @@ void opL(const Problem &p, const double *prim, double *dual) {
 // ------------------------------------------------------------------ operator L*
 // src/operator.cuh:63-82 with src/costs.cuh:162-195, :418-458 and src/constraints.cuh:171-191, :265-286
 void opLadj(const Problem &p, const double *dual, double *prim) {
-    const int nx = p.nx, nu = p.nu, nxu = nx + nu, yD = p.yD, d4 = nxu + 2;
+    const int nx = p.nx, nu = p.nu, nxu = nx + nu, yD = p.yD, d4 = p.d4;
     double *u = prim + p.oU, *x = prim + p.oX, *y = prim + p.oY, *t = prim + p.oT, *s = prim + p.oS;
     const double *I = dual + p.oI, *II = dual + p.oII, *III = dual + p.oIII, *IV = dual + p.oIV,
         *V = dual + p.oV, *VI = dual + p.oVI;
@@ -361,9 +394,15 @@ void opLadj(const Problem &p, const double *dual, double *prim) {
             for (int k = 0; k < nx; k++) xi[k] = 0.;
             for (int k = 0; k < nu; k++) ui[k] = 0.;
         }
-        // children accumulate in child order (src/costs.cuh:172-177)
+        // children accumulate in child order (src/costs.cuh:172-177; linear: :274-285)
         for (long c = 0; c < p.numCh[i]; c++) {
             const long j = p.chFrom[i] + c;
+            if (p.ncost == COST_LIN) {
+                const double *g = p.grad.data() + (size_t) j * nxu;
+                for (int k = 0; k < nx; k++) xi[k] += g[k] * IV[j];
+                for (int k = 0; k < nu; k++) ui[k] += g[nx + k] * IV[j];
+                continue;
+            }
             const double *iv = IV + (size_t) j * d4;
             double w[1024];
             gemv(p.sqrtQ.data() + (size_t) j * nx * nx, nx, nx, iv, w, false);
@@ -378,10 +417,14 @@ void opLadj(const Problem &p, const double *dual, double *prim) {
             (void) cd;
         }
     }
-    t[0] = 0.;
-    for (int j = 1; j < p.N; j++) {
-        const double *iv = IV + (size_t) j * d4;
-        t[j] = (iv[nxu] + iv[nxu + 1]) * 0.5;
+    if (p.ncost == COST_LIN) {  // t <- set, every node (src/costs.cuh:286)
+        for (int j = 0; j < p.N; j++) t[j] = IV[p.N + j];
+    } else {
+        t[0] = 0.;
+        for (int j = 1; j < p.N; j++) {
+            const double *iv = IV + (size_t) j * d4;
+            t[j] = (iv[nxu] + iv[nxu + 1]) * 0.5;
+        }
     }
     for (int l = 0; l < p.nlf; l++) {
         double *xl = x + (size_t) (p.nl + l) * nx;
@@ -394,6 +437,11 @@ void opLadj(const Problem &p, const double *dual, double *prim) {
             gemvT(p.lG.data(), rows, nx, gPart, xl, false);
             for (int k = 0; k < nx; k++) xl[k] += idPart[k];
         } else for (int k = 0; k < nx; k++) xl[k] = 0.;
+        if (p.lcost == COST_LIN) {  // src/costs.cuh:289-292
+            for (int k = 0; k < nx; k++) xl[k] += p.lgrad[k] * VI[l];
+            s[p.nl + l] = VI[p.nlf + l];
+            continue;
+        }
         const double *vi = VI + (size_t) l * (nx + 2);
         double w[1024];
         gemv(p.lsqrtQ.data(), nx, nx, vi, w, false);
@@ -522,6 +570,13 @@ void projSoc(double *v, int dim) {
     v[dim - 1] = i1b * v[dim - 1] + (1 - i1b) * (1 - i2) * (i3 * (sc * nrm - 1) + 1);
 }
 
+// A linear cost's "projection" (src/costs.cuh:296-301 with k_projectRectangle,
+// src/definitions.cu:39-48, lower bound -inf, upper bound the epigraph variable): the cost entry
+// is clamped to at most the epigraph entry, which stays as it is.  This is not the Euclidean
+// projection onto {(c, t) : c <= t} (that one would move both to their mean); it is restated as
+// the reference computes it.
+double projBelow(double c, double t) { return c >= t ? t : c; }
+
 // Dual prox on the dual workspace w (already scaled by 1/gamma and translated):
 // w <- Pi_C(w) block by block, src/cache.cuh:685-698
 void projectDual(const Problem &p, double *w) {
@@ -535,18 +590,28 @@ void projectDual(const Problem &p, double *w) {
         }
     for (int i = 0; i < p.nl; i++) II[i] = std::max(0., II[i]);  // src/cones.cuh:134-137
     for (size_t k = 0; k < p.sIII; k++) III[k] = projBox(III[k], p.nLB[k], p.nUB[k]);
-    const int d4 = p.nx + p.nu + 2;
-    for (int j = 0; j < p.N; j++) projSoc(IV + (size_t) j * d4, d4);
+    if (p.ncost == COST_LIN) {
+        for (int j = 0; j < p.N; j++) IV[j] = projBelow(IV[j], IV[p.N + j]);
+    } else {
+        const int d4 = p.nx + p.nu + 2;
+        for (int j = 0; j < p.N; j++) projSoc(IV + (size_t) j * d4, d4);
+    }
     for (size_t k = 0; k < p.sV; k++) V[k] = projBox(V[k], p.lLB[k], p.lUB[k]);
-    for (int l = 0; l < p.nlf; l++) projSoc(VI + (size_t) l * (p.nx + 2), p.nx + 2);
+    if (p.lcost == COST_LIN) {
+        for (int l = 0; l < p.nlf; l++) VI[l] = projBelow(VI[l], VI[p.nlf + l]);
+    } else {
+        for (int l = 0; l < p.nlf; l++) projSoc(VI + (size_t) l * (p.nx + 2), p.nx + 2);
+    }
 }
 
-void translateDual(const Problem &p, double *w) {  // src/costs.cuh:197-199, 460-462
+void translateDual(const Problem &p, double *w) {  // src/costs.cuh:197-199, 460-462; linear: none (:294)
     const int d4 = p.nx + p.nu + 2, d6 = p.nx + 2;
-    for (int j = 0; j < p.N; j++)
-        for (int k = 0; k < d4; k++) w[p.oIV + (size_t) j * d4 + k] += p.trans[(size_t) j * d4 + k];
-    for (int l = 0; l < p.nlf; l++)
-        for (int k = 0; k < d6; k++) w[p.oVI + (size_t) l * d6 + k] += p.ltrans[k];
+    if (p.ncost != COST_LIN)
+        for (int j = 0; j < p.N; j++)
+            for (int k = 0; k < d4; k++) w[p.oIV + (size_t) j * d4 + k] += p.trans[(size_t) j * d4 + k];
+    if (p.lcost != COST_LIN)
+        for (int l = 0; l < p.nlf; l++)
+            for (int k = 0; k < d6; k++) w[p.oVI + (size_t) l * d6 + k] += p.ltrans[k];
 }
 
 // ------------------------------------------------------------------ CP operator T

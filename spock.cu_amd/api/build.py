@@ -86,6 +86,24 @@ class CostQuadraticPlusLinear(CostQuadratic):
         self.r = None if leaf else np.asarray(r, dtype=np.float64).reshape(-1, 1)
 
 
+class CostLinear:
+    """Linear stage cost q'x + r'u (leaf: q'x) (``api/python/spock/build.py:351-490``).  Its cone
+    block per node is the pair (q'x + r'u, t) of the node's cost and epigraph variable."""
+    kind = "linear"
+
+    def __init__(self, q, r=None, leaf=False):
+        self.q = np.asarray(q, dtype=np.float64).reshape(-1, 1)
+        self.r = None if leaf else np.asarray(r, dtype=np.float64).reshape(-1, 1)
+        self.Q = self.R = None
+        self.leaf = leaf
+
+    def node_zero(self):
+        self.q = np.zeros_like(self.q)
+        if self.r is not None:
+            self.r = np.zeros_like(self.r)
+        return self
+
+
 class CostSet:
     """A cost replicated over the nodes it applies to, with conditioning and translations
     (``api/python/spock/build.py:184-330`` and ``:470-701``)."""
@@ -95,13 +113,14 @@ class CostSet:
         self.kind = costs[-1].kind
         self.n = len(costs)
         self.start = 0 if leaf else 1
-        self.Q0 = [c.Q.copy() for c in costs]
+        lin = self.kind == "linear"
+        self.Q0 = None if lin else [c.Q.copy() for c in costs]
         self.q0 = [c.q.copy() for c in costs]
-        self.R0 = None if leaf else [c.R.copy() for c in costs]
+        self.R0 = None if (leaf or lin) else [c.R.copy() for c in costs]
         self.r0 = None if leaf else [c.r.copy() for c in costs]
-        self.Q = [m.copy() for m in self.Q0]
+        self.Q = None if lin else [m.copy() for m in self.Q0]
         self.q = [m.copy() for m in self.q0]
-        self.R = None if leaf else [m.copy() for m in self.R0]
+        self.R = None if (leaf or lin) else [m.copy() for m in self.R0]
         self.r = None if leaf else [m.copy() for m in self.r0]
         self._refresh()
 
@@ -113,7 +132,22 @@ class CostSet:
     def is_quadratic_plus_linear(self):
         return self.kind == "quadraticPlusLinear"
 
+    @property
+    def is_linear(self):
+        return self.kind == "linear"
+
+    def block(self, nx, nu):
+        """Entries of the cost's dual block per node: the cone [v; a; b] of dimension nx + nu + 2
+        (leaf nx + 2), or the pair (cost, epigraph variable) of a linear cost."""
+        return 2 if self.is_linear else nx + (0 if self.leaf else nu) + 2
+
     def _refresh(self):
+        if self.is_linear:  # gradient rows [q' r'] (leaf q'), src/costs.cuh:228-250
+            self.sqrtQ = self.sqrtR = None
+            self.dim_per_node = 2
+            self.translation = [np.zeros((2, 1)) for _ in range(self.n)]
+            self.gradient = [np.vstack([self.q[i]] + ([] if self.leaf else [self.r[i]])).T for i in range(self.n)]
+            return
         self.sqrtQ = [_sqrtm(m) for m in self.Q]
         self.sqrtR = None if self.leaf else [_sqrtm(m) for m in self.R]
         nx = self.Q[0].shape[0]
@@ -139,25 +173,29 @@ class CostSet:
     def scaling_states(self, sx):
         for e in range(sx.size):
             for i in range(self.start, self.n):
-                sx[e] = max(sx[e], np.sqrt(self.Q0[i][e, e]))
-                if self.is_quadratic_plus_linear:
+                if not self.is_linear:
+                    sx[e] = max(sx[e], np.sqrt(self.Q0[i][e, e]))
+                if not self.is_quadratic:  # api/python/spock/build.py:417-423
                     sx[e] = max(sx[e], abs(self.q0[i][e, 0]))
         return sx
 
     def scaling_inputs(self, su):
         for e in range(su.size):
             for i in range(self.start, self.n):
-                su[e] = max(su[e], np.sqrt(self.R0[i][e, e]))
-                if self.is_quadratic_plus_linear:
+                if not self.is_linear:
+                    su[e] = max(su[e], np.sqrt(self.R0[i][e, e]))
+                if not self.is_quadratic:
                     su[e] = max(su[e], abs(self.r0[i][e, 0]))
         return su
 
     def condition(self, sx_inv, su_inv=None):
         for i in range(self.start, self.n):
-            self.Q[i] = sx_inv.T @ self.Q0[i] @ sx_inv
+            if not self.is_linear:
+                self.Q[i] = sx_inv.T @ self.Q0[i] @ sx_inv
             self.q[i] = np.diagonal(sx_inv).reshape(-1, 1) * self.q0[i]
             if not self.leaf:
-                self.R[i] = su_inv.T @ self.R0[i] @ su_inv
+                if not self.is_linear:
+                    self.R[i] = su_inv.T @ self.R0[i] @ su_inv
                 self.r[i] = np.diagonal(su_inv).reshape(-1, 1) * self.r0[i]
         self._refresh()
         return self

@@ -157,6 +157,31 @@ def polyhedron_identity_small(folder, seed=44, dt="d"):
             .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
 
 
+def linear_cost_markov(folder, seed=61, dt="d"):
+    """The reference's linear-cost problem (``tests/python/testProblemFactory.py:13-78``): the
+    3-state Markov chain (P, v) of that test, horizon 4 branching until stage 3 (the same tree),
+    nx=3, nu=2, per-event nonleaf costs linear with q = 10, 20, 30 and r = 1, 2, 3 (all entries),
+    leaf cost 5 I, |x| <= 6 and |u| <= 0.3 at nonleaf nodes, |x| <= 0.1 at leaves, AV@R 0.95.  The
+    dynamics are seeded and kept mild (A = I + 0.1 N(0, 1), B = 0.5 N(0, 1), scaled by 1, 1.05,
+    0.95 per event) instead of that test's unseeded A, 2A, 3A, so the leaf bound can be met."""
+    rng = np.random.default_rng(seed)
+    P = np.array([[0.1, 0.8, 0.1], [0.4, 0.6, 0.], [0, 0.3, 0.7]])
+    v = np.array([0.5, 0.4, 0.1])
+    tr = tree_mod.MarkovChain(P, v, 4, 3, dt=dt).build()
+    nx, nu = 3, 2
+    A = np.eye(nx) + 0.1 * rng.standard_normal((nx, nx))
+    B = 0.5 * rng.standard_normal((nx, nu))
+    dyn = [build.Dynamics(k * A, k * B) for k in (1.0, 1.05, 0.95)]
+    costs = [build.CostLinear(10.0 * k * np.ones(nx), 1.0 * k * np.ones(nu)) for k in (1, 2, 3)]
+    leaf = build.CostQuadratic(5 * np.eye(nx), leaf=True)
+    lim = np.vstack((6.0 * np.ones((nx, 1)), 0.3 * np.ones((nu, 1))))
+    ncon = build.Rectangle(-lim, lim)
+    lcon = build.Rectangle(-0.1 * np.ones((nx, 1)), 0.1 * np.ones((nx, 1)))
+    return (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(ncon).with_constraint_leaf(lcon)
+            .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
+
+
 def unconstrained_markov(folder, seed=7, dt="d"):
     """The 31-node family (2 events, horizon 4; nx=3, nu=2; AV@R 0.95) with no state or input
     constraints at any node (the reference's ``no`` constraint)."""
@@ -201,5 +226,5 @@ def initial_states(seed, nx, count, kind="normal"):
 
 def generate(name, folder, **kw):
     os.makedirs(folder, exist_ok=True)
-    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov, "poly": polyhedral_small, "nocon": unconstrained_markov, "affine": affine_markov, "pwi": polyhedron_identity_small,
+    return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov, "poly": polyhedral_small, "nocon": unconstrained_markov, "affine": affine_markov, "pwi": polyhedron_identity_small, "linear": linear_cost_markov,
             "large": polyhedral_iid}[name](folder, **kw)

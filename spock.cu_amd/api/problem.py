@@ -128,17 +128,19 @@ class Problem:
         self.size_prim = nu * nl + nx * N + self.yD * nl + 2 * N
         con_nl = 0 if self.ncon.kind == "no" else self.ncon.dim_per_node
         con_l = 0 if self.lcon.kind == "no" else self.lcon.dim_per_node
-        self.size_dual = self.yD * nl + nl + con_nl * nl + (nx + nu + 2) * N + con_l * nlf + (nx + 2) * nlf
+        self.d4, self.d6 = self.ncost.block(nx, nu), self.lcost.block(nx, 0)
+        self.size_dual = self.yD * nl + nl + con_nl * nl + self.d4 * N + con_l * nlf + self.d6 * nlf
 
     def _stacks(self):
         t = self.tree
         anc = t.ancestors_arr.copy()
         anc[0] = 0
+        nc, lc = self.ncost, self.lcost
         return dict(
             anc=anc,
-            sQ=np.array(self.ncost.sqrtQ), sR=np.array(self.ncost.sqrtR),
-            q=np.array(self.ncost.q)[:, :, 0], r=np.array(self.ncost.r)[:, :, 0],
-            lsQ=np.array(self.lcost.sqrtQ), lq=np.array(self.lcost.q)[:, :, 0],
+            sQ=None if nc.is_linear else np.array(nc.sqrtQ), sR=None if nc.is_linear else np.array(nc.sqrtR),
+            q=np.array(nc.q)[:, :, 0], r=np.array(nc.r)[:, :, 0],
+            lsQ=None if lc.is_linear else np.array(lc.sqrtQ), lq=np.array(lc.q)[:, :, 0],
             b=np.array(self.b_pad)[:, :, 0],
         )
 
@@ -159,20 +161,28 @@ class Problem:
         xu = np.hstack((x[:nl], u))
         III = self._con_op(self.ncon, xu)
         xa, ua = x[st["anc"]], u[np.minimum(st["anc"], nl - 1)]
-        IV = np.zeros((N, nx + nu + 2))
-        IV[:, :nx] = np.einsum("nij,nj->ni", st["sQ"], xa)
-        IV[:, nx:nx + nu] = np.einsum("nij,nj->ni", st["sR"], ua)
-        half = 0.5 * tt - 0.5 * (np.einsum("ni,ni->n", st["q"], xa) + np.einsum("ni,ni->n", st["r"], ua))
-        IV[:, nx + nu] = half
-        IV[:, nx + nu + 1] = half
-        IV[0] = 0.0
+        lin = np.einsum("ni,ni->n", st["q"], xa) + np.einsum("ni,ni->n", st["r"], ua)
+        if self.ncost.is_linear:  # [q'x + r'u of every node; t] (src/costs.cuh:261-267)
+            lin[0] = 0.0
+            IV = np.concatenate([lin, tt])
+        else:
+            IV = np.zeros((N, nx + nu + 2))
+            IV[:, :nx] = np.einsum("nij,nj->ni", st["sQ"], xa)
+            IV[:, nx:nx + nu] = np.einsum("nij,nj->ni", st["sR"], ua)
+            half = 0.5 * tt - 0.5 * lin
+            IV[:, nx + nu] = half
+            IV[:, nx + nu + 1] = half
+            IV[0] = 0.0
         xl = x[nl:]
         V = self._con_op(self.lcon, xl)
-        VI = np.zeros((nlf, nx + 2))
-        VI[:, :nx] = np.einsum("nij,nj->ni", st["lsQ"], xl)
-        hs = 0.5 * s[nl:] - 0.5 * np.einsum("ni,ni->n", st["lq"], xl)
-        VI[:, nx] = hs
-        VI[:, nx + 1] = hs
+        if self.lcost.is_linear:  # [q'x of every leaf; s] (src/costs.cuh:269-272)
+            VI = np.concatenate([np.einsum("ni,ni->n", st["lq"], xl), s[nl:]])
+        else:
+            VI = np.zeros((nlf, nx + 2))
+            VI[:, :nx] = np.einsum("nij,nj->ni", st["lsQ"], xl)
+            hs = 0.5 * s[nl:] - 0.5 * np.einsum("ni,ni->n", st["lq"], xl)
+            VI[:, nx] = hs
+            VI[:, nx + 1] = hs
         return np.concatenate([I.ravel(), II, III.ravel(), IV.ravel(), V.ravel(), VI.ravel()])
 
     def adj(self, dual):
@@ -186,10 +196,10 @@ class Problem:
         II = d[o:o + nl]; o += nl
         c3 = 0 if self.ncon.kind == "no" else self.ncon.dim_per_node
         III = d[o:o + c3 * nl].reshape(nl, c3); o += c3 * nl
-        IV = d[o:o + (nx + nu + 2) * N].reshape(N, nx + nu + 2); o += (nx + nu + 2) * N
+        IV = d[o:o + self.d4 * N].reshape(-1, N if self.ncost.is_linear else self.d4); o += self.d4 * N
         c5 = 0 if self.lcon.kind == "no" else self.lcon.dim_per_node
         V = d[o:o + c5 * nlf].reshape(nlf, c5); o += c5 * nlf
-        VI = d[o:o + (nx + 2) * nlf].reshape(nlf, nx + 2)
+        VI = d[o:o + self.d6 * nlf].reshape(-1, nlf if self.lcost.is_linear else self.d6)
         s = np.zeros(N)
         s[:nl] = II
         y = I - st["b"] * II[:, None]
@@ -197,19 +207,27 @@ class Problem:
         x = np.zeros((N, nx))
         u = xu[:, nx:].copy()
         x[:nl] = xu[:, :nx]
-        cd = IV[:, nx + nu] + IV[:, nx + nu + 1]
-        cx = np.einsum("nij,nj->ni", st["sQ"], IV[:, :nx]) - 0.5 * st["q"] * cd[:, None]
-        cu = np.einsum("nij,nj->ni", st["sR"], IV[:, nx:nx + nu]) - 0.5 * st["r"] * cd[:, None]
+        if self.ncost.is_linear:  # src/costs.cuh:274-287: t is the second half, whole
+            cx, cu = st["q"] * IV[0][:, None], st["r"] * IV[0][:, None]
+            tt = IV[1].copy()
+        else:
+            cd = IV[:, nx + nu] + IV[:, nx + nu + 1]
+            cx = np.einsum("nij,nj->ni", st["sQ"], IV[:, :nx]) - 0.5 * st["q"] * cd[:, None]
+            cu = np.einsum("nij,nj->ni", st["sR"], IV[:, nx:nx + nu]) - 0.5 * st["r"] * cd[:, None]
+            tt = 0.5 * cd
+            tt[0] = 0.0
         cx[0] = 0.0
         cu[0] = 0.0
         np.add.at(x, st["anc"][1:], cx[1:])
         np.add.at(u, st["anc"][1:], cu[1:])
-        tt = 0.5 * cd
-        tt[0] = 0.0
         x[nl:] = self._con_adj(self.lcon, V, nlf, nx)
-        cdl = VI[:, nx] + VI[:, nx + 1]
-        x[nl:] += np.einsum("nij,nj->ni", st["lsQ"], VI[:, :nx]) - 0.5 * st["lq"] * cdl[:, None]
-        s[nl:] = 0.5 * cdl
+        if self.lcost.is_linear:  # src/costs.cuh:289-292
+            x[nl:] += st["lq"] * VI[0][:, None]
+            s[nl:] = VI[1]
+        else:
+            cdl = VI[:, nx] + VI[:, nx + 1]
+            x[nl:] += np.einsum("nij,nj->ni", st["lsQ"], VI[:, :nx]) - 0.5 * st["lq"] * cdl[:, None]
+            s[nl:] = 0.5 * cdl
         return np.concatenate([u.ravel(), x.ravel(), y.ravel(), tt, s])
 
     def trajectory_cost(self, prim):
@@ -230,17 +248,23 @@ class Problem:
         c3 = 0 if self.ncon.kind == "no" else self.ncon.dim_per_node
         c5 = 0 if self.lcon.kind == "no" else self.lcon.dim_per_node
         o4 = self.yD * nl + nl + c3 * nl
-        o6 = o4 + (nx + nu + 2) * N + c5 * nlf
+        o6 = o4 + self.d4 * N + c5 * nlf
 
         def smallest_epigraph(block):
             v, a, b = block[:-2], block[-2], block[-1]
             return (v @ v) / (b - a) - (a + b)
         stage = np.zeros(N)
-        for j in range(1, N):
+        for j in range(1, N):  # a linear cost's epigraph bound is the cost itself
+            if self.ncost.is_linear:
+                stage[j] = w[o4 + j]
+                continue
             blk = w[o4 + j * (nx + nu + 2):o4 + (j + 1) * (nx + nu + 2)] + self.ncost.translation[j].ravel()
             stage[j] = smallest_epigraph(blk)
         togo = np.zeros(N)
         for k in range(nlf):
+            if self.lcost.is_linear:
+                togo[nl + k] = w[o6 + k]
+                continue
             tr = self.lcost.translation[min(k, len(self.lcost.translation) - 1)].ravel()
             togo[nl + k] = smallest_epigraph(w[o6 + k * (nx + 2):o6 + (k + 1) * (nx + 2)] + tr)
         for i in range(nl - 1, -1, -1):  # children carry larger indices than their parent
@@ -287,14 +311,16 @@ class Problem:
             "uncond_dynamics_A": _stack([d.A0 for d in self.dyn]),
             "uncond_dynamics_B": _stack([d.B0 for d in self.dyn]),
             "uncond_dynamics_c": _stack([d.c0 for d in self.dyn]),
-            "uncond_cost_nonleaf_Q": _stack(nc.Q0),
-            "uncond_cost_nonleaf_R": _stack(nc.R0),
-            "uncond_cost_leaf_Q": _stack([lc.Q0[-1]]),
         }
-        if nc.is_quadratic_plus_linear:
+        if not nc.is_linear:
+            out["uncond_cost_nonleaf_Q"] = _stack(nc.Q0)
+            out["uncond_cost_nonleaf_R"] = _stack(nc.R0)
+        if not lc.is_linear:
+            out["uncond_cost_leaf_Q"] = _stack([lc.Q0[-1]])
+        if not nc.is_quadratic:
             out["uncond_cost_nonleaf_q"] = _stack(nc.q0)
             out["uncond_cost_nonleaf_r"] = _stack(nc.r0)
-        if lc.is_quadratic_plus_linear:
+        if not lc.is_quadratic:
             out["uncond_cost_leaf_q"] = _stack([lc.q0[-1]])
         for con, txt in ((self.ncon, "uncond_nonleafConstraint"), (self.lcon, "uncond_leafConstraint")):
             rect = con if con.kind == "rectangle" else getattr(con, "rect", None)
@@ -348,15 +374,21 @@ class Problem:
         if self.preconditioned:
             tens["scaling"] = self.scaling.reshape(-1)
         nc = self.ncost
-        tens["nonleafCost_sqrtQ"] = _stack(nc.sqrtQ)
-        tens["nonleafCost_sqrtR"] = _stack(nc.sqrtR)
-        tens["nonleafCost_translation"] = _stack(nc.translation)
+        if nc.is_linear:  # api/python/spock/problem.py:510-514
+            tens["nonleafCost_gradient"] = _stack(nc.gradient)
+        else:
+            tens["nonleafCost_sqrtQ"] = _stack(nc.sqrtQ)
+            tens["nonleafCost_sqrtR"] = _stack(nc.sqrtR)
+            tens["nonleafCost_translation"] = _stack(nc.translation)
         if nc.is_quadratic_plus_linear:
             tens["nonleafCost_q"] = _stack(nc.q)
             tens["nonleafCost_r"] = _stack(nc.r)
         lc = self.lcost
-        tens["leafCost_sqrtQ"] = _stack([lc.sqrtQ[-1]])
-        tens["leafCost_translation"] = _stack([lc.translation[-1]])
+        if lc.is_linear:  # api/python/spock/problem.py:541-545
+            tens["leafCost_gradient"] = _stack([lc.gradient[-1]])
+        else:
+            tens["leafCost_sqrtQ"] = _stack([lc.sqrtQ[-1]])
+            tens["leafCost_translation"] = _stack([lc.translation[-1]])
         if lc.is_quadratic_plus_linear:
             tens["leafCost_q"] = _stack([lc.q[-1]])
         for con, txt in ((self.ncon, "nonleafConstraint"), (self.lcon, "leafConstraint")):
@@ -410,6 +442,8 @@ class Factory:
     def _copy_cost(self, c):
         if c.kind == "quadratic":
             return build.CostQuadratic(c.Q, c.R, c.leaf)
+        if c.kind == "linear":
+            return build.CostLinear(c.q, c.r, c.leaf)
         return build.CostQuadraticPlusLinear(c.Q, c.q, c.R, c.r, c.leaf)
 
     def with_cost_nonleaf(self, cost):

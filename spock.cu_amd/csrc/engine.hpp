@@ -114,7 +114,15 @@ struct Out {
 };
 
 enum Method { METHOD_SPOCK = 0, METHOD_CP = 1 };
-enum DevKinds { CON_NO_DEV = 0, CON_RECT_DEV = 1, CON_POLY_DEV = 2, COST_QUAD_DEV = 0, COST_QUADLIN_DEV = 1 };
+enum DevKinds { CON_NO_DEV = 0, CON_RECT_DEV = 1, CON_POLY_DEV = 2, COST_QUAD_DEV = 0, COST_QUADLIN_DEV = 1,
+                COST_LIN_DEV = 2 };
+
+// A linear cost's dual "projection" (src/costs.cuh:296-301: k_projectRectangle of
+// src/definitions.cu:39-48 with lower bound -inf and the epigraph entry as upper bound): the cost
+// entry is clamped to at most the epigraph entry, which is kept.  Not the Euclidean projection
+// onto {c <= t}; restated as the reference computes it.
+template <typename T>
+__host__ __device__ __forceinline__ T projBelow(T c, T t) { return c >= t ? t : c; }
 enum Status { ST_NOTRUN = -1, ST_CONVERGED = 0, ST_OUTOFITER = 1, ST_OUTOFTIME = 2 };
 
 constexpr int MAXR = 4;  // row chunks of 64 per wave task (rows <= 256)

@@ -264,6 +264,17 @@ __device__ __forceinline__ void adjNode(const Inst<T> &P, const T *dual, int n, 
         const int c0 = P.chFrom[n], nc = P.numCh[n];
         for (int ci = 0; ci < nc; ci++) {  // children accumulate in order (src/costs.cuh:172-177)
             const int j = c0 + ci;
+            if (P.ncost == COST_LIN_DEV) {  // x_i += q_j sum_j, u_i += r_j sum_j (src/costs.cuh:274-285)
+                const T sj = dual[P.oIV + j];
+                const T *qj = P.q.at(j), *rj = P.r.at(j);
+#pragma unroll
+                for (int rc = 0; rc < R; rc++) {
+                    const int r = lane + 64 * rc;
+                    if (r < nx) o.x[rc] += qj[r] * sj;
+                    if (r < nu) o.u[rc] += rj[r] * sj;
+                }
+                continue;
+            }
             const T *iv = dual + P.oIV + (long) j * d4;
             const T a = iv[nxu], bb = iv[nxu + 1];
             loadPtr(v, iv, nx, lane);
@@ -306,24 +317,37 @@ __device__ __forceinline__ void adjNode(const Inst<T> &P, const T *dual, int n, 
             loadPtr(v, dual + b5, P.con5, lane);
             mvAcc(o.x, P.lGT.at(l), nx, nx, P.con5, v, lane);
         }
-        const T *vi = dual + P.oVI + (long) l * (nx + 2);
-        const T a = vi[nx], bb = vi[nx + 1];
-        loadPtr(v, vi, nx, lane);
-        zero(w);
-        mvAcc(w, P.lsQ.at(l), nx, nx, nx, v, lane);
-#pragma unroll
-        for (int rc = 0; rc < R; rc++) o.x[rc] += w[rc];
-        if (P.lcost == COST_QUADLIN_DEV) {
+        if (P.lcost == COST_LIN_DEV) {  // x_l += q sum_l, s_l = set_l (src/costs.cuh:289-292)
+            const T sl = dual[P.oVI + l];
             const T *lq = P.lq.at(l);
 #pragma unroll
             for (int rc = 0; rc < R; rc++) {
                 const int r = lane + 64 * rc;
-                if (r < nx) o.x[rc] = o.x[rc] - T(0.5) * lq[r] * a - T(0.5) * lq[r] * bb;
+                if (r < nx) o.x[rc] += lq[r] * sl;
             }
+            o.s = dual[P.oVI + P.nlf + l];
+        } else {
+            const T *vi = dual + P.oVI + (long) l * (nx + 2);
+            const T a = vi[nx], bb = vi[nx + 1];
+            loadPtr(v, vi, nx, lane);
+            zero(w);
+            mvAcc(w, P.lsQ.at(l), nx, nx, nx, v, lane);
+#pragma unroll
+            for (int rc = 0; rc < R; rc++) o.x[rc] += w[rc];
+            if (P.lcost == COST_QUADLIN_DEV) {
+                const T *lq = P.lq.at(l);
+#pragma unroll
+                for (int rc = 0; rc < R; rc++) {
+                    const int r = lane + 64 * rc;
+                    if (r < nx) o.x[rc] = o.x[rc] - T(0.5) * lq[r] * a - T(0.5) * lq[r] * bb;
+                }
+            }
+            o.s = (a + bb) * T(0.5);
         }
-        o.s = (a + bb) * T(0.5);
     }
-    if (n >= 1) {
+    if (P.ncost == COST_LIN_DEV) {
+        o.t = dual[P.oIV + P.N + n];  // t <- set, every node (src/costs.cuh:286)
+    } else if (n >= 1) {
         const T *iv = dual + P.oIV + (long) n * d4;
         o.t = (iv[nxu] + iv[nxu + 1]) * T(0.5);
     }
@@ -371,7 +395,22 @@ __device__ __forceinline__ void opNode(const Inst<T> &P, int n, int lane, PV pv,
         }
     }
     // IV
-    {
+    if (P.ncost == COST_LIN_DEV) {  // (sum_n, set_n) = (q_n' x_anc + r_n' u_anc, t_n), node 0: (0, t_0)
+        T sum = T(0);              // (src/costs.cuh:261-267)
+        if (n >= 1) {
+            const int a = P.anc[n];
+            const T *qn = P.q.at(n), *rn = P.r.at(n);
+            T pq = T(0), pr = T(0);
+#pragma unroll
+            for (int c = 0; c < V; c++) {
+                const int k = lane + 64 * c;
+                if (k < nx) pq += qn[k] * pv(P.oX + (long) a * nx + k);
+                if (k < nu) pr += rn[k] * pv(P.oU + (long) a * nu + k);
+            }
+            sum = waveSum(pq) + waveSum(pr);
+        }
+        h.blockLin(P.oIV + n, P.oIV + P.N + n, sum, pv(P.oT + n));
+    } else {
         zero(acc);
         zero(acc2);
         T hv = T(0);
@@ -414,6 +453,17 @@ __device__ __forceinline__ void opNode(const Inst<T> &P, int n, int lane, PV pv,
             zero(acc);
             mvAcc(acc, P.lG.at(l), P.con5, P.con5, nx, v, lane);
             h.blockElem(P.oV + (long) l * P.con5, acc, P.con5, P.lLB.at(l), P.lUB.at(l));
+        }
+        if (P.lcost == COST_LIN_DEV) {  // (q' x_l, s_l) (src/costs.cuh:269-272)
+            const T *lq = P.lq.at(l);
+            T pq = T(0);
+#pragma unroll
+            for (int c = 0; c < V; c++) {
+                const int k = lane + 64 * c;
+                if (k < nx) pq += lq[k] * v[c];
+            }
+            h.blockLin(P.oVI + l, P.oVI + P.nlf + l, waveSum(pq), pv(P.oS + n));
+            return;
         }
         zero(acc);
         zero(acc2);
@@ -461,6 +511,12 @@ struct StoreL {
         if (lane == 0) {
             out[base + nxr + nur] = hv;
             out[base + nxr + nur + 1] = hv;
+        }
+    }
+    __device__ void blockLin(long eSum, long eSet, T sum, T set) {
+        if (lane == 0) {
+            out[eSum] = sum;
+            out[eSet] = set;
         }
     }
 };
@@ -544,6 +600,15 @@ struct ProxL {
             cd[kh + 1] = (w2 - p2) * P.gamma;
         }
     }
+    // linear cost: the cost entry clamped to at most the epigraph entry, which is kept
+    // (src/costs.cuh:296-301 with k_projectRectangle, lower bound -inf)
+    __device__ void blockLin(long eSum, long eSet, T sum, T set) {
+        if (lane == 0) {
+            const T ws = wOf(sum, eSum, T(0)), wt = wOf(set, eSet, T(0));
+            cd[eSum] = (ws - projBelow(ws, wt)) * P.gamma;
+            cd[eSet] = (wt - wt) * P.gamma;
+        }
+    }
 };
 
 // Reduction handler for ell(y) = [L* y_d; L y_p]: accumulates sum x . (y - gamma ell) and
@@ -580,6 +645,12 @@ struct ReduceL {
         if (lane == 0) {
             acc(base + nxr + nur, hv);
             acc(base + nxr + nur + 1, hv);
+        }
+    }
+    __device__ void blockLin(long eSum, long eSet, T sum, T set) {
+        if (lane == 0) {
+            acc(eSum, sum);
+            acc(eSet, set);
         }
     }
 };

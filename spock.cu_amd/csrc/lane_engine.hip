@@ -239,6 +239,14 @@ __device__ __forceinline__ void adjNode(LP<T> P, const LV<T> &d, int n, E emit) 
                 }
                 for (int ci = 0; ci < nc; ci++) {  // children accumulate in order (src/costs.cuh:172-177)
                     const int j = c0 + ci;
+                    if (P.ncost == COST_LIN_DEV) {  // q_j sum_j, r_j sum_j (src/costs.cuh:274-285)
+                        const T sj = d[P.oIV + j];
+                        CP<T> gj = (part == 0 ? tab(P.q, j) : tab(P.r, j)) + r0;
+#pragma unroll
+                        for (int r = 0; r < NY; r++)
+                            if (r < rr) y[r] += gj[r] * sj;
+                        continue;
+                    }
                     const long iv = P.oIV + (long) j * d4;
                     if (part == 0) mv(y, tab(P.sQ, j) + r0, rr, nx, nx, [&](int cc) { return d[iv + cc]; });
                     else mv(y, tab(P.sR, j) + r0, rr, nu, nu, [&](int cc) { return d[iv + nx + cc]; });
@@ -256,8 +264,9 @@ __device__ __forceinline__ void adjNode(LP<T> P, const LV<T> &d, int n, E emit) 
     } else {
         const int l = n - P.nl;
         const long b5 = P.oV + (long) l * P.con5;
-        const long vi = P.oVI + (long) l * (nx + 2);
-        const T a = d[vi + nx], bb = d[vi + nx + 1];
+        const bool lin = P.lcost == COST_LIN_DEV;  // block VI = (sum of every leaf; set of every leaf)
+        const long vi = P.oVI + (long) l * (lin ? 1 : nx + 2);
+        const T a = lin ? T(0) : d[vi + nx], bb = lin ? T(0) : d[vi + nx + 1];
         for (int r0 = 0; r0 < nx; r0 += NY) {
             const int rr = min(NY, nx - r0);
             zeroY(y);
@@ -268,7 +277,15 @@ __device__ __forceinline__ void adjNode(LP<T> P, const LV<T> &d, int n, E emit) 
             } else if (P.lcon == CON_POLY_DEV) {
                 mv(y, tab(P.lGT, l) + r0, rr, nx, P.con5, [&](int cc) { return d[b5 + cc]; });
             }
-            mv(y, tab(P.lsQ, l) + r0, rr, nx, nx, [&](int cc) { return d[vi + cc]; });
+            if (lin) {  // q sum_l (src/costs.cuh:289-292)
+                const T sl = d[vi];
+                CP<T> lq = tab(P.lq, l) + r0;
+#pragma unroll
+                for (int r = 0; r < NY; r++)
+                    if (r < rr) y[r] += lq[r] * sl;
+            } else {
+                mv(y, tab(P.lsQ, l) + r0, rr, nx, nx, [&](int cc) { return d[vi + cc]; });
+            }
             if (P.lcost == COST_QUADLIN_DEV) {
                 CP<T> lq = tab(P.lq, l) + r0;
 #pragma unroll
@@ -277,9 +294,11 @@ __device__ __forceinline__ void adjNode(LP<T> P, const LV<T> &d, int n, E emit) 
             }
             each(y, rr, [&](int r, T v) { emit(0, r0 + r, v); });
         }
-        emit(3, 0, (a + bb) * T(0.5));
+        emit(3, 0, lin ? d[P.oVI + P.nlf + l] : (a + bb) * T(0.5));
     }
-    if (n >= 1) {
+    if (P.ncost == COST_LIN_DEV) {
+        emit(4, 0, d[P.oIV + P.N + n]);  // t <- set, every node (src/costs.cuh:286)
+    } else if (n >= 1) {
         const long iv = P.oIV + (long) n * d4;
         emit(4, 0, (d[iv + nxu] + d[iv + nxu + 1]) * T(0.5));
     } else {
@@ -322,7 +341,18 @@ __device__ __forceinline__ void opNode(LP<T> P, int n, PV pv, H &h) {
             }
         }
     }
-    {  // IV
+    if (P.ncost == COST_LIN_DEV) {  // IV: (q_n' x_anc + r_n' u_anc, t_n), node 0 (0, t_0) (src/costs.cuh:261-267)
+        T sum = T(0);
+        if (n >= 1) {
+            const int a = cst(P.anc)[n];
+            CP<T> qn = tab(P.q, n), rn = tab(P.r, n);
+            T pq = T(0), pr = T(0);
+            for (int k = 0; k < nx; k++) pq += qn[k] * pv(P.oX + (long) a * nx + k);
+            for (int k = 0; k < nu; k++) pr += rn[k] * pv(P.oU + (long) a * nu + k);
+            sum = pq + pr;
+        }
+        h.lin(P.oIV + n, P.oIV + P.N + n, sum, pv(P.oT + n));
+    } else {  // IV
         const long base = P.oIV + (long) n * (nxu + 2);
         CP<T> tr = tab(P.trans, n);
         h.socBegin();
@@ -369,6 +399,13 @@ __device__ __forceinline__ void opNode(LP<T> P, int n, PV pv, H &h) {
                 mv(y, tab(P.lG, l) + r0, rr, P.con5, nx, [&](int cc) { return pv(P.oX + (long) n * nx + cc); });
                 each(y, rr, [&](int r, T v) { h.elemBox(b5 + r0 + r, v, lb[r0 + r], ub[r0 + r]); });
             }
+        }
+        if (P.lcost == COST_LIN_DEV) {  // VI: (q' x_l, s_l) (src/costs.cuh:269-272)
+            CP<T> lq = tab(P.lq, l);
+            T pq = T(0);
+            for (int k = 0; k < nx; k++) pq += lq[k] * pv(P.oX + (long) n * nx + k);
+            h.lin(P.oVI + l, P.oVI + P.nlf + l, pq, pv(P.oS + n));
+            return;
         }
         const long base = P.oVI + (long) l * (nx + 2);
         CP<T> tr = tab(P.ltrans, l);
@@ -442,6 +479,11 @@ struct Prox {
         cd[k1] = (w1 - p1) * P.gamma;
         cd[k2] = (w2 - p2) * P.gamma;
     }
+    __device__ __forceinline__ void lin(long eSum, long eSet, T sum, T set) {
+        const T ws = wOf(sum, eSum, T(0)), wt = wOf(set, eSet, T(0));
+        cd[eSum] = (ws - projBelow(ws, wt)) * P.gamma;
+        cd[eSet] = (wt - wt) * P.gamma;
+    }
 };
 
 // ell(y) = [L* y_d; L y_p] reductions: sum x.(y - gamma ell) and max |y/gamma - ell|
@@ -465,6 +507,10 @@ struct Red {
         acc(P.sPrim + base + dim - 2, hv);
         acc(P.sPrim + base + dim - 1, hv);
     }
+    __device__ __forceinline__ void lin(long eSum, long eSet, T sum, T set) {
+        acc(P.sPrim + eSum, sum);
+        acc(P.sPrim + eSet, set);
+    }
 };
 
 template <typename T>
@@ -479,6 +525,10 @@ struct Store {
     __device__ __forceinline__ void socEnd(long base, int dim, T hv, CP<T>) {
         out[base + dim - 2] = hv;
         out[base + dim - 1] = hv;
+    }
+    __device__ __forceinline__ void lin(long eSum, long eSet, T sum, T set) {
+        out[eSum] = sum;
+        out[eSet] = set;
     }
 };
 

@@ -310,7 +310,8 @@ HostProblem loadProblem(const std::string &folderIn) {
     auto costKind = [](const std::string &s) {
         if (s == "quadratic") return (int) COST_QUAD;
         if (s == "quadraticPlusLinear") return (int) COST_QUADLIN;
-        throw std::runtime_error("cost type '" + s + "' is not supported (quadratic, quadraticPlusLinear)");
+        if (s == "linear") return (int) COST_LIN;
+        throw std::runtime_error("cost type '" + s + "' is not supported (quadratic, linear, quadraticPlusLinear)");
     };
     auto conKind = [](const std::string &s) {
         if (s == "no") return (int) CON_NO;
@@ -391,18 +392,41 @@ HostProblem loadProblem(const std::string &folderIn) {
     // risk (src/risks.cuh:26-33)
     p.NN = stackTable("risk_NNtr", p.nD, p.nD, nl);
     p.b = stackTable("risk_b", p.yD, 1, nl);
-    // costs (src/costs.cuh:85-110, 331-368): nonleaf per node, leaf single matrix replicated
-    p.sQ = stackTable("nonleafCost_sqrtQ", nx, nx, N);
-    p.sR = stackTable("nonleafCost_sqrtR", nu, nu, N);
-    p.trans = stackTable("nonleafCost_translation", nx + nu + 2, 1, N);
+    // costs (src/costs.cuh:85-110, 331-368): nonleaf per node, leaf single matrix replicated.  A
+    // linear cost (src/costs.cuh:228-250) reads only its gradient [q' r'] per node; its q and r
+    // parts fill the q and r tables and the quadratic tables are zero.
+    if (p.ncost == COST_LIN) {
+        auto g = readFp(dir, "nonleafCost_gradient");
+        expectSize(g, (size_t) (nx + nu) * N, "nonleafCost_gradient");
+        std::vector<double> qv((size_t) nx * N), rv((size_t) nu * N);
+        for (int n = 0; n < N; n++) {
+            for (int k = 0; k < nx; k++) qv[(size_t) n * nx + k] = g[(size_t) n * (nx + nu) + k];
+            for (int k = 0; k < nu; k++) rv[(size_t) n * nu + k] = g[(size_t) n * (nx + nu) + nx + k];
+        }
+        p.q = dedup(qv, nx, 1, N);
+        p.r = dedup(rv, nu, 1, N);
+        p.sQ = single(std::vector<double>((size_t) nx * nx, 0.), nx, nx, N);
+        p.sR = single(std::vector<double>((size_t) nu * nu, 0.), nu, nu, N);
+        p.trans = single(std::vector<double>((size_t) nx + nu + 2, 0.), nx + nu + 2, 1, N);
+    } else {
+        p.sQ = stackTable("nonleafCost_sqrtQ", nx, nx, N);
+        p.sR = stackTable("nonleafCost_sqrtR", nu, nu, N);
+        p.trans = stackTable("nonleafCost_translation", nx + nu + 2, 1, N);
+    }
     if (p.ncost == COST_QUADLIN) {
         p.q = stackTable("nonleafCost_q", nx, 1, N);
         p.r = stackTable("nonleafCost_r", nu, 1, N);
-    } else {
+    } else if (p.ncost == COST_QUAD) {
         p.q = single(std::vector<double>(nx, 0.), nx, 1, N);
         p.r = single(std::vector<double>(nu, 0.), nu, 1, N);
     }
-    {
+    if (p.lcost == COST_LIN) {
+        auto g = readFp(dir, "leafCost_gradient");
+        expectSize(g, (size_t) nx, "leafCost_gradient");
+        p.lq = single(g, nx, 1, nlf);
+        p.lsQ = single(std::vector<double>((size_t) nx * nx, 0.), nx, nx, nlf);
+        p.ltrans = single(std::vector<double>((size_t) nx + 2, 0.), nx + 2, 1, nlf);
+    } else {
         auto m = readFp(dir, "leafCost_sqrtQ");
         expectSize(m, (size_t) nx * nx, "leafCost_sqrtQ");
         p.lsQ = single(m, nx, nx, nlf);
@@ -475,8 +499,10 @@ HostProblem loadProblem(const std::string &folderIn) {
     // iterate layout (src/cache.cuh:482-581)
     p.sU = (long) nu * nl; p.sX = (long) nx * N; p.sY = (long) p.yD * nl; p.sT = N; p.sS = N;
     p.sPrim = p.sU + p.sX + p.sY + p.sT + p.sS;
-    p.sI = (long) p.yD * nl; p.sII = nl; p.sIII = (long) p.con3 * nl; p.sIV = (long) (nx + nu + 2) * N;
-    p.sV = (long) p.con5 * nlf; p.sVI = (long) (nx + 2) * nlf;
+    // a linear cost's block is (cost of every node; epigraph variable of every node)
+    p.sI = (long) p.yD * nl; p.sII = nl; p.sIII = (long) p.con3 * nl;
+    p.sIV = (long) (p.ncost == COST_LIN ? 2 : nx + nu + 2) * N;
+    p.sV = (long) p.con5 * nlf; p.sVI = (long) (p.lcost == COST_LIN ? 2 : nx + 2) * nlf;
     p.sDual = p.sI + p.sII + p.sIII + p.sIV + p.sV + p.sVI;
     p.sIter = p.sPrim + p.sDual;
     return p;

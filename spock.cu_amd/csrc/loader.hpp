@@ -17,7 +17,7 @@
 
 namespace spk {
 
-enum CostKind { COST_QUAD = 0, COST_QUADLIN = 1 };
+enum CostKind { COST_QUAD = 0, COST_QUADLIN = 1, COST_LIN = 2 };
 enum ConKind { CON_NO = 0, CON_RECT = 1, CON_POLY = 2, CON_PWI = 3 };  // CON_PWI: as read; loaded as CON_POLY
 
 struct Table {                // distinct matrices (rows x cols, column major) + per-node index

@@ -258,6 +258,7 @@ __device__ __forceinline__ void adjNode(LP<T> P, const TV<T> &d, int n, E emit) 
                 [&](auto &acc, int r0, int rr) {
                     if (P.ncon == CON_POLY_DEV)
                         mma(acc, tab(P.nGT, n) + part * nx + r0, rr, nxu, P.con3, [&](int cc) { return d[b3 + cc]; });
+                    if (P.ncost == COST_LIN_DEV) return;  // elementwise below
                     for (int ci = 0; ci < nc; ci++) {  // children accumulate (src/costs.cuh:172-177)
                         const long iv = P.oIV + (long) (c0 + ci) * d4;
                         if (part == 0)
@@ -268,6 +269,12 @@ __device__ __forceinline__ void adjNode(LP<T> P, const TV<T> &d, int n, E emit) 
                 },
                 [&](int r, T v) {
                     if (P.ncon == CON_RECT_DEV) v += d[b3 + part * nx + r];
+                    if (P.ncost == COST_LIN_DEV) {  // q_j sum_j, r_j sum_j (src/costs.cuh:274-285)
+                        for (int ci = 0; ci < nc; ci++) {
+                            const int j = c0 + ci;
+                            v += (part == 0 ? tab(P.q, j) : tab(P.r, j))[r] * d[P.oIV + j];
+                        }
+                    }
                     if (P.ncost == COST_QUADLIN_DEV) {
                         for (int ci = 0; ci < nc; ci++) {
                             const int j = c0 + ci;
@@ -283,27 +290,31 @@ __device__ __forceinline__ void adjNode(LP<T> P, const TV<T> &d, int n, E emit) 
     } else {
         const int l = n - P.nl;
         const long b5 = P.oV + (long) l * P.con5;
-        const long vi = P.oVI + (long) l * (nx + 2);
-        const T a = d[vi + nx], bb = d[vi + nx + 1];
+        const bool lin = P.lcost == COST_LIN_DEV;  // block VI = (sum of every leaf; set of every leaf)
+        const long vi = P.oVI + (long) l * (lin ? 1 : nx + 2);
+        const T a = lin ? T(0) : d[vi + nx], bb = lin ? T(0) : d[vi + nx + 1];
         gemv<T>(
             nx,
             [&](auto &acc, int r0, int rr) {
                 if (P.lcon == CON_POLY_DEV)
                     mma(acc, tab(P.lGT, l) + r0, rr, nx, P.con5, [&](int cc) { return d[b5 + cc]; });
-                mma(acc, tab(P.lsQ, l) + r0, rr, nx, nx, [&](int cc) { return d[vi + cc]; });
+                if (!lin) mma(acc, tab(P.lsQ, l) + r0, rr, nx, nx, [&](int cc) { return d[vi + cc]; });
             },
             [&](int r, T v) {
                 if (P.lcon == CON_RECT_DEV) v += d[b5 + r];
+                if (lin) v += tab(P.lq, l)[r] * d[vi];  // q sum_l (src/costs.cuh:289-292)
                 if (P.lcost == COST_QUADLIN_DEV) {
                     const T qr = tab(P.lq, l)[r];
                     v = v - T(0.5) * qr * a - T(0.5) * qr * bb;
                 }
                 emit(0, r, v);
             });
-        if (first) emit(3, 0, (a + bb) * T(0.5));
+        if (first) emit(3, 0, lin ? d[P.oVI + P.nlf + l] : (a + bb) * T(0.5));
     }
     if (first) {
-        if (n >= 1) {
+        if (P.ncost == COST_LIN_DEV) {
+            emit(4, 0, d[P.oIV + P.N + n]);  // t <- set, every node (src/costs.cuh:286)
+        } else if (n >= 1) {
             const long iv = P.oIV + (long) n * d4;
             emit(4, 0, (d[iv + nxu] + d[iv + nxu + 1]) * T(0.5));
         } else {
@@ -349,7 +360,18 @@ __device__ __forceinline__ void opNode(LP<T> P, int n, PV pv, H &h) {
                 [&](int r, T v) { h.elemBox(base + r, v, lb[r], ub[r]); });
         }
     }
-    {  // IV
+    if (P.ncost == COST_LIN_DEV) {  // IV: (q_n' x_anc + r_n' u_anc, t_n), node 0 (0, t_0) (src/costs.cuh:261-267)
+        T sum = T(0);
+        if (n >= 1) {
+            const int a = cst(P.anc)[n];
+            CP<T> qn = tab(P.q, n), rn = tab(P.r, n);
+            T pq = T(0), pr = T(0);
+            forRows<T>(nx, [&](int k) { pq += qn[k] * pv(P.oX + (long) a * nx + k); });
+            forRows<T>(nu, [&](int k) { pr += rn[k] * pv(P.oU + (long) a * nu + k); });
+            sum = sumG(pq) + sumG(pr);
+        }
+        if (g == 0) h.lin(P.oIV + n, P.oIV + P.N + n, sum, pv(P.oT + n));
+    } else {  // IV
         const long base = P.oIV + (long) n * (nxu + 2);
         CP<T> tr = tab(P.trans, n);
         h.socBegin();
@@ -399,6 +421,14 @@ __device__ __forceinline__ void opNode(LP<T> P, int n, PV pv, H &h) {
                     mma(acc, tab(P.lG, l) + r0, rr, P.con5, nx, [&](int cc) { return pv(P.oX + (long) n * nx + cc); });
                 },
                 [&](int r, T v) { h.elemBox(b5 + r, v, lb[r], ub[r]); });
+        }
+        if (P.lcost == COST_LIN_DEV) {  // VI: (q' x_l, s_l) (src/costs.cuh:269-272)
+            CP<T> lq = tab(P.lq, l);
+            T pq = T(0);
+            forRows<T>(nx, [&](int k) { pq += lq[k] * pv(P.oX + (long) n * nx + k); });
+            pq = sumG(pq);
+            if (g == 0) h.lin(P.oVI + l, P.oVI + P.nlf + l, pq, pv(P.oS + n));
+            return;
         }
         const long base = P.oVI + (long) l * (nx + 2);
         CP<T> tr = tab(P.ltrans, l);
@@ -478,6 +508,11 @@ struct Prox {
             cd[k2] = (w2 - p2) * P.gamma;
         }
     }
+    __device__ __forceinline__ void lin(long eSum, long eSet, T sum, T set) {
+        const T ws = wOf(sum, eSum, T(0)), wt = wOf(set, eSet, T(0));
+        cd[eSum] = (ws - projBelow(ws, wt)) * P.gamma;
+        cd[eSet] = (wt - wt) * P.gamma;
+    }
 };
 
 // ell(y) = [L* y_d; L y_p] reductions: sum x.(y - gamma ell) and max |y/gamma - ell|
@@ -503,6 +538,10 @@ struct Red {
         acc(P.sPrim + base + dim - 2, hv);
         acc(P.sPrim + base + dim - 1, hv);
     }
+    __device__ __forceinline__ void lin(long eSum, long eSet, T sum, T set) {
+        acc(P.sPrim + eSum, sum);
+        acc(P.sPrim + eSet, set);
+    }
 };
 
 template <typename T>
@@ -519,6 +558,10 @@ struct Store {
         const int dim = n1 + n2 + 2;
         out[base + dim - 2] = hv;
         out[base + dim - 1] = hv;
+    }
+    __device__ __forceinline__ void lin(long eSum, long eSet, T sum, T set) {
+        out[eSum] = sum;
+        out[eSet] = set;
     }
 };
 

@@ -42,7 +42,7 @@ def problem_dirs(tmp_path_factory):
     from spock_amd.api import configs
     base = tmp_path_factory.mktemp("problems")
     out = {}
-    for name in ("fixture", "small", "mid", "poly", "nocon", "affine", "pwi"):
+    for name in ("fixture", "small", "mid", "poly", "nocon", "affine", "pwi", "linear"):
         d = str(base / name)
         configs.generate(name, d)
         out[name] = d

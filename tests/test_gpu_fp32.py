@@ -76,7 +76,7 @@ def check_ops(o, b, zs, ys, x0s, count, name):
         assert errs[key] < BOUND_SWEEP, (key, errs[key])
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine", "pwi"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine", "pwi", "linear"])
 def test_fp32_per_instance_operators_match_fp64_oracle(ctx, problem_dirs, name):
     d = problem_dirs[name]
     o = Oracle(d)
@@ -92,7 +92,7 @@ def test_fp32_per_instance_operators_match_fp64_oracle(ctx, problem_dirs, name):
 
 
 @pytest.mark.parametrize("engine", [17, 16, 0])
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine", "pwi"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine", "pwi", "linear"])
 def test_fp32_shared_operators_match_fp64_oracle(ctx, problem_dirs, name, engine):
     """engine 17, 16: the tile engine (matrix-core products; 1024- and 512-thread builds); 0: the
     lane engine."""

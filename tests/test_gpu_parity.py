@@ -38,6 +38,8 @@ def x0_for(name, nx, k=0):
         return configs.initial_states(8, nx, k + 1, "uniform")[k]
     if name == "pwi":
         return configs.initial_states(33, nx, k + 1, "uniform")[k]
+    if name == "linear":
+        return configs.initial_states(61, nx, k + 1, "small")[k]
     return configs.initial_states(2, nx, k + 1)[k]
 
 
@@ -52,7 +54,7 @@ def inert_zero(o, z):
 
 @pytest.mark.parametrize("name,wgs", [("fixture", 1), ("small", 1), ("mid", 1), ("poly", 1),
                                       ("mid", 7), ("poly", 3), ("mid", 128), ("nocon", 1), ("affine", 1),
-                                      ("affine", 5), ("pwi", 1), ("pwi", 3)])
+                                      ("affine", 5), ("pwi", 1), ("pwi", 3), ("linear", 1), ("linear", 3)])
 def test_operators_match_oracle(ctx, problem_dirs, name, wgs):
     """wgs > 1: grid mode, the instance spread over a cooperative launch of wgs workgroups."""
     d = problem_dirs[name]
@@ -85,7 +87,7 @@ def test_operators_match_oracle(ctx, problem_dirs, name, wgs):
     assert abs(sc[1] - np.max(np.abs(y / o.gamma - ell))) <= 1e-12 * np.max(np.abs(y / o.gamma))
 
 
-@pytest.mark.parametrize("name", ["fixture", "mid", "poly", "pwi"])
+@pytest.mark.parametrize("name", ["fixture", "mid", "poly", "pwi", "linear"])
 def test_get_dual_matches_oracle(ctx, problem_dirs, name):
     """Cache::getDual (src/cache.cuh:371-376): the dual step from the iterate with a given primal
     point; with the primal prox of T(z) as that point it is the dual part of T(z)."""
@@ -107,7 +109,7 @@ def test_get_dual_matches_oracle(ctx, problem_dirs, name):
     ("mid", 50, "spock", 1), ("poly", 75, "spock", 1), ("poly", 250, "cp", 1),
     ("mid", 50, "spock", 64), ("poly", 75, "spock", 5), ("small", 250, "cp", 16), ("nocon", 75, "spock", 1),
     ("affine", 75, "spock", 1), ("affine", 250, "cp", 1), ("pwi", 50, "spock", 1), ("pwi", 250, "cp", 1),
-    ("pwi", 50, "spock", 3)])
+    ("pwi", 50, "spock", 3), ("linear", 50, "spock", 1), ("linear", 100, "cp", 1), ("linear", 50, "spock", 3)])
 def test_fixed_iterations_match_oracle(ctx, problem_dirs, name, iters, method, wgs):
     d = problem_dirs[name]
     o = Oracle(d)

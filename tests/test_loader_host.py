@@ -49,7 +49,7 @@ def run(exe, folder):
     return subprocess.run([exe, folder], capture_output=True, text=True, timeout=120, env=env)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine", "pwi"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine", "pwi", "linear"])
 def test_loader_reads_every_family(loader_exe, problem_dirs, name):
     r = run(loader_exe, problem_dirs[name])
     assert r.returncode == 0, r.stdout + r.stderr

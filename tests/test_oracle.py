@@ -41,7 +41,7 @@ def inert_zero(o, z):
     return z
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine"])
+@pytest.mark.parametrize("name", ["fixture", "small", "mid", "poly", "nocon", "affine", "linear"])
 def test_adjoint(problem_dirs, name):
     o = Oracle(problem_dirs[name])
     rng = np.random.default_rng(0)
@@ -52,7 +52,7 @@ def test_adjoint(problem_dirs, name):
     assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))
 
 
-@pytest.mark.parametrize("name", ["small", "poly", "nocon", "affine", "pwi"])
+@pytest.mark.parametrize("name", ["small", "poly", "nocon", "affine", "pwi", "linear"])
 def test_operators_match_the_generator(tmp_path, name):
     """The oracle's L and L* (a C++ transcription of src/operator.cuh) against the generator's
     (spock_amd.api.problem, written separately from the Python reference) on random vectors, for
@@ -72,7 +72,7 @@ def test_operators_match_the_generator(tmp_path, name):
         assert o.gamma * np.linalg.norm(Ld, 2) == pytest.approx(0.99, abs=2e-3)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "poly", "nocon", "affine"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly", "nocon", "affine", "linear"])
 def test_step_size_is_099_over_norm(problem_dirs, name):
     o = Oracle(problem_dirs[name])
     Ld = nr.dense_operator(o.L, o.n_prim)
@@ -142,6 +142,29 @@ def test_soc_matches_textbook_projection():
             assert abs((v - p) @ p) <= 1e-9 * max(1.0, np.linalg.norm(v) ** 2)  # orthogonality
 
 
+def test_linear_cost_projection_is_the_reference_clamp(problem_dirs):
+    """The linear cost's dual block is (q'x + r'u of every node; t of every node), and its
+    "projection" is k_projectRectangle with lower bound -inf and the t entry as the upper bound
+    (src/costs.cuh:296-301, src/definitions.cu:39-48): a cost entry above its t becomes t, t is kept.
+    The Euclidean projection onto {c <= t} would move both to their mean instead; the oracle
+    follows the reference, not the textbook."""
+    o = Oracle(problem_dirs["linear"])
+    N = o.info["numNodes"]
+    assert o.sIV == 2 * N
+    oIV = o.sI + o.sII + o.sIII
+    cases = [(3.0, 1.0, 1.0), (-2.0, 1.0, -2.0), (1.0, 1.0, 1.0), (0.5, -4.0, -4.0)]
+    w = np.zeros(o.n_dual)
+    for j, (c, t, _) in enumerate(cases):
+        w[oIV + j], w[oIV + N + j] = c, t
+    p = o.project_dual(w)
+    for j, (c, t, want) in enumerate(cases):
+        assert p[oIV + j] == want and p[oIV + N + j] == t
+    assert (p[oIV + 0], p[oIV + N + 0]) != (2.0, 2.0)  # the Euclidean projection of (3, 1)
+    rest = np.ones(o.n_dual, dtype=bool)
+    rest[[oIV + j for j in range(len(cases))] + [oIV + N + j for j in range(len(cases))]] = False
+    assert np.all(p[rest] == 0.0)
+
+
 def test_box_known_answer():
     """tests/cuda/testKernels.cu:14-36 (with an infinite first bound)."""
     n, ub, oob = 1025, 4.0, 0.1
@@ -188,7 +211,7 @@ def test_dotM_definition(problem_dirs, name):
     assert o.dotM(x, y) == pytest.approx(ref, rel=1e-12, abs=1e-12)
 
 
-@pytest.mark.parametrize("name", ["fixture", "small", "poly", "nocon", "affine"])
+@pytest.mark.parametrize("name", ["fixture", "small", "poly", "nocon", "affine", "linear"])
 def test_T_lands_on_the_dynamics(problem_dirs, name):
     """The primal part of T(z) is a projection onto the dynamics with x_0 = x0."""
     d = problem_dirs[name]
