@@ -3,27 +3,31 @@
 Workload: the 229-node Markov-tree RAOCP (3 events, horizon 10, branching stops at stage 3,
 nx = 50, nu = 20, box constraints, AV@R 0.9, leaf cost from the discrete Riccati equation,
 preconditioned), synthetic data from the package's seeded generator, and a stream of held-out
-initial states x0 ~ N(0, 0.25 I).
+initial states x0 ~ N(0, 0.5^2 I).
 
-Acceptance: the reference's residual ||r/gamma - [L* r_d; L r_p]||_inf (src/cache.cuh:910-961)
-at the ABSOLUTE tolerance 1e-3 times min(scaling) (toleranceAbsolute(1e-3), relative 0,
-src/cache.cuh:302-306), tested every 25 iterations of a solve's own count.  Every accepted solution
-is certified in FP64.
+Acceptance: the reference's residual ||r/gamma - [L* r_d; L r_p]||_inf (src/cache.cuh:910-961),
+tested every 25 iterations of a solve's own count, against the threshold the plain method sets
+itself for that x0 under the reference's builder tol(1e-3) (absolute and relative 1e-3, the
+setting of its timed drivers, examples/random/main.cu:41-45): max(1e-3, 1e-3 e0) x min(scaling),
+e0 the error of the cold first iterate (src/cache.cuh:926-927).  Warm, polish and fallback solves
+all accept against that same per-x0 threshold, and every accepted solution is certified in FP64.
 
 The method (continuous batching, the way a solver service runs): a pool of --slots instance slots
 (4,096: the held-out batch) solved in lockstep by the tile engine in FP32.  Each slot's solve starts
 from the learned warm start (the MLP's prediction for its x0) and ends when the residual test
 passes; the slot is then refilled with the next held-out x0 while the other slots run on.  Every
-FP32 solution is polished in FP64 in a second pool (warm-started from it, accepted against the
-same threshold; already accepted when its FP64 residual passes at its first check).  A warm solve
-that reaches its FP32 iteration cap falls back to a cold FP64 solve in the same FP64 pool.
-A step is one chunk of --step-iters lockstep iterations of the FP32 pool plus one chunk of the FP64
-pool, with the refills between them; warmup steps (longer) fill the pipeline first.  value = held-out
-solutions accepted in FP64 during the timed steps / timed wall time.
+FP32 solution is polished in FP64 in a second pool (warm-started from it; already accepted when its
+FP64 residual passes at its first check).  A warm solve that reaches its FP32 iteration cap falls
+back to a cold FP64 solve in the same FP64 pool.  A step is one chunk of --step-iters lockstep
+iterations of the FP32 pool plus one chunk of the FP64 pool, with the refills between them; warmup
+steps (longer) fill the pipeline first.  value = held-out solutions accepted in FP64 during the
+timed steps / timed wall time.
 
-Before the timed part: the MLP is trained on cold FP32 solves of training initial states (to the
-reference's tol(1e-3)), the engines' schedules are autotuned, and the plain baseline is measured:
-FP64 SPOCK on one instance per launch (grid mode), cold.
+Before the timed part: the thresholds of the stream, the MLP trained on cold FP32 solves of
+training initial states, the engines' schedules autotuned, and the plain baseline's seconds per
+iteration: FP64 SPOCK on one instance per launch (grid mode), cold.  After it: the cost check of
+sampled accepted solutions against plain FP64 SPOCK (cold, same thresholds), whose iteration counts
+scale the plain and CPU per-iteration times to solves/s.
 
 Prints one JSON line; ``--gpus`` above 1 is refused: this build runs on one GPU.
 """
@@ -57,6 +61,20 @@ def _pkg():
     return rt, configs
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tile_solve_x1024_229node_4096states_traffic.json")
+
+
+def measured_traffic(roof):
+    """HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE and WRITE_SIZE (two
+    --pmc runs of `bench.py --kernel-only`, each scaled by what the counter reads for the 1 GiB
+    device copy of that run), as recorded under profiles/; None when that summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
 def bytes_per_T(p, word, share=1):
     """Algorithmic bytes one application of the CP operator T moves for one instance: the per-node
     factors (ABKTr, APB per child; K, R~^-1 per nonleaf), the kernel projectors and the per-event
@@ -84,22 +102,26 @@ def s0_of(p, z):
 
 class Pool:
     """Continuous batching over a shared batch: every slot holds a job (an initial state of the
-    stream) or is idle; solves start in free slots (refills) between chunks of lockstep iterations."""
+    stream) or is idle; solves start in free slots (refills) between chunks of lockstep iterations.
+    Each job carries its own acceptance threshold (a given tolerance: a solve whose residual passes
+    at its first check ends there)."""
 
-    def __init__(self, rt, ctx, prob, dtype, slots, tol_scaled, max_iters, schedule=None):
+    def __init__(self, rt, ctx, prob, dtype, slots, max_iters, schedule=None):
         self.b = rt.Batch(ctx, [prob] * slots, dtype, shared=True)
         if schedule is not None:
             self.b.set_schedule(*schedule)
-        # the threshold as a given tolerance: a solve whose residual passes at its first check ends there
-        self.b.set_tolerances([tol_scaled] * slots)
+        self.tols = np.ones(slots)
+        self.b.set_tolerances(self.tols)
         self.b.pool_open(rt.options(tol_abs=0.0, tol_rel=0.0, max_iters=max_iters, chunk_iters=25))
         self.slots = slots
         self.job = np.full(slots, -1, dtype=np.int64)  # job id, -1 idle, -2 ended and waiting to be freed
         self.kernel_s = 0.0
         self.launches = 0
 
-    def admit(self, slots, jobs, x0s, mlp=None, src=None, src_slots=None):
+    def admit(self, slots, jobs, x0s, taus, mlp=None, src=None, src_slots=None):
         slots = np.asarray(slots, dtype=np.int64)
+        self.tols[slots] = taus
+        self.b.set_tolerances(self.tols)
         if src is not None:
             self.b.refill_from(slots, x0s, src, src_slots)
         elif mlp is not None:
@@ -124,6 +146,25 @@ class Pool:
         return np.flatnonzero(self.job == -1)
 
 
+def thresholds(rt, ctx, prob, x0s, tol, chunk):
+    """The acceptance threshold the plain method sets itself for each x0.  The reference's builder
+    tol(1e-3) (absolute and relative 1e-3, the setting of its timed drivers, examples/random/main.cu:41-45)
+    makes a cold solve accept at max(tol_abs, tol_rel e0) x min(scaling), e0 the error of its first
+    iterate (src/cache.cuh:926-927, :302-306).  One FP64 iteration of a cold shared batch per chunk of
+    x0 reports it (results()[k]["tol"])."""
+    out = np.empty(len(x0s))
+    b = rt.Batch(ctx, [prob] * chunk, rt.F64, shared=True)
+    for a in range(0, len(x0s), chunk):
+        xs = np.asarray(x0s[a:a + chunk])
+        pad = np.zeros((chunk, xs.shape[1]))
+        pad[:len(xs)] = xs
+        b.set_x0(list(pad))
+        b.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=0, chunk_iters=1))
+        res, _ = b.results()
+        out[a:a + len(xs)] = [r["tol"] for r in res[:len(xs)]]
+    return out
+
+
 def cost_gaps(prob, gen, z, z_ref, mask):
     """Relative gaps of two costs of iterates z against z_ref, over the rows in mask: the root
     epigraph variable s0, and J, the nested AV@R cost of the states and inputs alone."""
@@ -139,17 +180,17 @@ def cost_gaps(prob, gen, z, z_ref, mask):
     return out
 
 
-def plain_reference(rt, ctx, prob, x0s, tol_scaled, max_iters, seconds, schedule):
-    """Plain FP64 SPOCK, cold, of initial states x0s to the same absolute threshold, as one shared
-    FP64 batch (the same arithmetic as one instance per launch) run in chunks until every solve has
-    ended or `seconds` have passed: (status per row, iterates)."""
+def plain_reference(rt, ctx, prob, x0s, tol, max_iters, seconds, schedule):
+    """Plain FP64 SPOCK, cold, of initial states x0s at the reference's tol(1e-3) (it sets its own
+    threshold at its first check), as one shared FP64 batch (the arithmetic of one instance per
+    launch) run in chunks until every solve has ended or `seconds` have passed: (status per row,
+    iterates, iterations)."""
     b = rt.Batch(ctx, [prob] * len(x0s), rt.F64, shared=True)
     if schedule is not None:
         b.set_schedule(*schedule)
     b.set_x0(list(x0s))
-    b.set_tolerances([tol_scaled] * len(x0s))
     t = time.perf_counter()
-    b.solve(rt.options(tol_abs=0.0, tol_rel=0.0, max_iters=max_iters, chunk_iters=2000))
+    b.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=max_iters, chunk_iters=1000))
     while True:
         res, _ = b.results()
         running = sum(r["status"] == -1 for r in res)
@@ -274,27 +315,31 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--slots", type=int, default=4096, help="FP32 pool slots (the held-out batch)")
     ap.add_argument("--polish-slots", type=int, default=1024, help="FP64 pool slots (polish and fallback)")
-    ap.add_argument("--step-iters", type=int, default=100, help="lockstep iterations per timed step")
-    ap.add_argument("--warmup-iters", type=int, default=1000, help="lockstep iterations per warmup step")
+    ap.add_argument("--step-iters", type=int, default=200, help="FP32 pool iterations per timed step")
+    ap.add_argument("--warmup-iters", type=int, default=500, help="FP32 pool iterations per warmup step")
     ap.add_argument("--polish-iters", type=int, default=25, help="FP64 pool iterations per step")
-    ap.add_argument("--max-iters", type=int, default=60000, help="FP32 cap of a warm solve before the fallback")
-    ap.add_argument("--fallback-iters", type=int, default=200000, help="iteration cap of the FP64 pool")
+    ap.add_argument("--max-iters", type=int, default=6000, help="FP32 cap of a warm solve before the fallback")
+    ap.add_argument("--fallback-iters", type=int, default=30000, help="iteration cap of the FP64 pool")
     ap.add_argument("--train", type=int, default=4096, help="training initial states solved for the MLP")
-    ap.add_argument("--train-iters", type=int, default=6000, help="iteration cap of the training solves")
+    ap.add_argument("--train-iters", type=int, default=4000, help="iteration cap of the training solves")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--cold", action="store_true", help="no learned warm start (the pool starts cold)")
     ap.add_argument("--tol", type=float, default=1e-3)
-    ap.add_argument("--plain-iters", type=int, default=1500, help="bounded sample of the plain single-instance solve")
+    ap.add_argument("--plain-iters", type=int, default=800, help="bounded sample of the plain single-instance solve")
     ap.add_argument("--plain-workgroups", type=int, default=64)
-    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--cpu-seconds", type=float, default=5.0)
     ap.add_argument("--cpu-workers", type=int, default=0, help="host processes of the CPU baseline (0: min(16, cpus))")
     ap.add_argument("--tune-iters", type=int, default=25)
-    ap.add_argument("--budget", type=float, default=420.0,
+    ap.add_argument("--budget", type=float, default=470.0,
                     help="wall-clock seconds after which the timed loop stops at the end of a step (reported)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cost-sample", type=int, default=256, help="accepted solutions kept for the cost check")
-    ap.add_argument("--cost-seconds", type=float, default=40.0, help="time allowed for the plain FP64 reference solves")
+    ap.add_argument("--cost-sample", type=int, default=128, help="accepted solutions kept for the cost check")
+    ap.add_argument("--cost-seconds", type=float, default=60.0, help="time allowed for the plain FP64 reference solves")
     ap.add_argument("--no-cost", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the dominant kernel: one launch of 50 cold iterations of --slots FP32 instances "
+                         "(schedule 17, 1024 threads), then a 1 GiB device copy whose counters calibrate "
+                         "FETCH_SIZE / WRITE_SIZE; prints one JSON line (for rocprofv3) and exits")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
@@ -304,15 +349,31 @@ def main():
     with tempfile.TemporaryDirectory() as folder:
         gen = configs.generate("mid", folder)
         prob = rt.Problem(ctx, folder)
-        tol_scaled = args.tol * float(np.min(prob.scaling))
+        tol_abs_scaled = args.tol * float(np.min(prob.scaling))
         stream = configs.initial_states(51, prob.nx, 16 * args.slots, "normal")  # held-out x0 stream
-        log(f"problem: {prob.N} nodes, iterate {prob.n_iter}; tolerance {tol_scaled:.3e} (scaled)")
+        log(f"problem: {prob.N} nodes, iterate {prob.n_iter}; absolute part of the threshold {tol_abs_scaled:.3e} "
+            f"(scaled)")
+        if args.kernel_only:
+            import torch
+            roof = kernel_roofline(rt, ctx, prob, stream[:args.slots], (17, 1024), 50)
+            src = torch.ones(256 * 2 ** 20, dtype=torch.float32, device="cuda")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            print(json.dumps({"kernel_roofline": roof, "calibration_copy_bytes_each_way": src.numel() * 4}),
+                  flush=True)
+            return
 
         cpu = None
         if not args.no_cpu:
             workers = args.cpu_workers or min(16, os.cpu_count() or 1)
             cpu = cpu_baseline(folder, stream[0], args.cpu_seconds, 1.0, workers)  # per iteration; scaled below
             log(f"cpu baseline: {cpu['sample']}")
+
+        # the plain method's own threshold for every held-out x0
+        taus = thresholds(rt, ctx, prob, stream, args.tol, min(args.slots, 4096))
+        log(f"thresholds of the held-out stream: median {np.median(taus):.3e}, min {taus.min():.3e}, "
+            f"max {taus.max():.3e} (scaled)")
 
         # schedules of the FP32 and FP64 pools (the autotuner times the engines on the held-out batch)
         schedules = {}
@@ -344,8 +405,8 @@ def main():
             mlp, n_train, train_loss = train_warm_start(rt, ctx, prob, configs, args, s32)
 
         # the pools: FP32 (warm-started solves of the held-out stream) and FP64 (polish, fallback)
-        p32 = Pool(rt, ctx, prob, rt.F32, args.slots, tol_scaled, args.max_iters, s32)
-        p64 = Pool(rt, ctx, prob, rt.F64, args.polish_slots, tol_scaled, args.fallback_iters, s64)
+        p32 = Pool(rt, ctx, prob, rt.F32, args.slots, args.max_iters, s32)
+        p64 = Pool(rt, ctx, prob, rt.F64, args.polish_slots, args.fallback_iters, s64)
         nxt = [0]
         jobs = {}
 
@@ -359,7 +420,7 @@ def main():
             return ids
 
         ids = take(args.slots)
-        p32.admit(np.arange(args.slots), ids, stream[ids], mlp=mlp)
+        p32.admit(np.arange(args.slots), ids, stream[ids], taus[ids], mlp=mlp)
         solutions = {}  # job -> accepted FP64 iterate, for the cost check (the first --cost-sample)
         p64_queue = []  # (job, FP32 slot): FP32 solutions waiting for an FP64 slot (their FP32 slot held)
         fb_queue = []   # jobs whose warm FP32 solve failed: cold FP64 solves
@@ -391,7 +452,8 @@ def main():
                 polish.append((free64.pop(0), j, k))
             if polish:
                 js = np.array([j for _, j, _ in polish])
-                p64.admit([sl for sl, _, _ in polish], js, stream[js], src=p32.b, src_slots=[k for _, _, k in polish])
+                p64.admit([sl for sl, _, _ in polish], js, stream[js], taus[js], src=p32.b,
+                          src_slots=[k for _, _, k in polish])
                 for _, _, k in polish:
                     p32.job[k] = -1
             cold = []
@@ -399,11 +461,11 @@ def main():
                 cold.append((free64.pop(0), fb_queue.pop(0)))
             if cold:
                 js = np.array([j for _, j in cold])
-                p64.admit([sl for sl, _ in cold], js, stream[js])
+                p64.admit([sl for sl, _ in cold], js, stream[js], taus[js])
             free32 = p32.free()
             if free32.size:
                 new = take(free32.size)
-                p32.admit(free32, new, stream[new], mlp=mlp)
+                p32.admit(free32, new, stream[new], taus[new], mlp=mlp)
             return accepted
 
         for w in range(args.warmup):
@@ -423,50 +485,56 @@ def main():
                 break
         ctx.lib.spk_ctx_synchronize(ctx.h)
         elapsed = time.perf_counter() - t0
+        done = [v for v in jobs.values() if v["status"] is not None]
+        acc = [v for v in done if v["status"] == 0]
+        fp32_it = np.array([v["fp32_iters"] for v in acc if v["fp32_iters"] is not None] or [0])
+        fp64_it = np.array([v["fp64_iters"] for v in acc if v["fp64_iters"] is not None] or [0])
+        mean_iters = float(np.mean([(v["fp32_iters"] or 0) + (v["fp64_iters"] or 0) for v in acc])) if acc else None
         del p32, p64
-        # cost check: every sampled accepted solution against plain FP64 SPOCK, cold, same threshold
-        cost = None
+        # cost check: sampled accepted solutions against plain FP64 SPOCK, cold, at its own threshold;
+        # its iteration counts are the plain method's iterations per solve
+        cost, plain_iters = None, None
         if solutions and not args.no_cost:
             js = np.array(sorted(solutions))
             left = args.budget + args.cost_seconds - (time.perf_counter() - T_START)
-            status, z_ref, it_ref = plain_reference(rt, ctx, prob, stream[js], tol_scaled, args.fallback_iters,
+            status, z_ref, it_ref = plain_reference(rt, ctx, prob, stream[js], args.tol, args.fallback_iters,
                                                     max(10.0, min(args.cost_seconds, left)), s64)
             z = np.array([solutions[j] for j in js])
             ok = status == 0
             g = cost_gaps(prob, gen, z, z_ref, ok)
-            cost = {"sampled": int(js.size), "compared": int(ok.sum()),
-                    "s0": g["s0"], "J": g["J"],
-                    "plain_iterations_mean": float(it_ref[ok].mean()) if ok.any() else None,
-                    "reference": "plain FP64 SPOCK, cold, same x0, same absolute threshold (a shared FP64 batch: the "
-                                 "arithmetic of one instance per launch); solves not ended in the time allowed "
+            plain_iters = float(it_ref[ok].mean()) if ok.any() else None
+            cost = {"sampled": int(js.size), "compared": int(ok.sum()), "s0": g["s0"], "J": g["J"],
+                    "plain_iterations_mean": plain_iters,
+                    "reference": "plain FP64 SPOCK, cold, same x0, its own tol(1e-3) threshold (a shared FP64 batch: "
+                                 "the arithmetic of one instance per launch); solves not ended in the time allowed "
                                  "are not compared",
                     "costs": "s0: the root epigraph variable of the iterate; J: the nested AV@R cost of its states "
                              "and inputs alone; relative gaps"}
             log(f"cost check: {cost['compared']} of {cost['sampled']} compared; s0 gap mean "
                 f"{g['s0']['mean']}, max {g['s0']['max']}, within 1e-3: {g['s0']['within_1e-3']}; J gap mean "
-                f"{g['J']['mean']}, max {g['J']['max']}, within 1e-3: {g['J']['within_1e-3']}")
-        done = [v for v in jobs.values() if v["status"] is not None]
-        acc = [v for v in done if v["status"] == 0]
-        fp32_it = np.array([v["fp32_iters"] for v in acc if v["fp32_iters"] is not None] or [0])
-        fp64_it = np.array([v["fp64_iters"] for v in acc if v["fp64_iters"] is not None] or [0])
-        mean_iters = float(np.mean(fp32_it + fp64_it)) if acc else None
+                f"{g['J']['mean']}, max {g['J']['max']}, within 1e-3: {g['J']['within_1e-3']}; plain iterations "
+                f"{plain_iters}")
 
     value = accepted / elapsed if elapsed > 0 else 0.0
-    plain_rate = None
-    if mean_iters:
-        plain_rate = 1.0 / (best_plain["seconds_per_iteration"] * mean_iters)
-    if cpu is not None and mean_iters:
-        for key in ("value",):
-            cpu[key] = cpu[key] / mean_iters
-        cpu["single_core"]["value"] /= mean_iters
+    per_solve = plain_iters or mean_iters
+    plain_rate = 1.0 / (best_plain["seconds_per_iteration"] * per_solve) if per_solve else None
+    if cpu is not None and per_solve:
+        cpu["value"] = cpu["value"] / per_solve
+        cpu["single_core"]["value"] /= per_solve
+        cpu["sample"] += f"; scaled by {per_solve:.0f} iterations per plain solve (measured in the cost check)"
     line = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": steps_done,
         "steps_requested": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / max(1, steps_done),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "RAOCP Markov tree 229 nodes (3 events, horizon 10, branching stops at stage 3), "
-                               "nx=50, nu=20, box, AV@R 0.9; held-out x0~N(0,0.25I)",
+                               "nx=50, nu=20, box, AV@R 0.9; held-out x0~N(0,0.5^2 I)",
                    "slots": args.slots, "polish_slots": args.polish_slots, "step_iters": args.step_iters,
-                   "tol_abs": args.tol, "tol_scaled": tol_scaled, "check_period": 25,
+                   "acceptance": "per x0, the threshold plain FP64 SPOCK sets itself under the reference's "
+                                 "builder tol(1e-3) (absolute and relative 1e-3, as its timed drivers, "
+                                 "examples/random/main.cu:41-45): max(1e-3, 1e-3 e0) x min(scaling), e0 the "
+                                 "error of the cold first iterate; checked every 25 iterations",
+                   "threshold_scaled": {"median": float(np.median(taus)), "min": float(taus.min()),
+                                        "max": float(taus.max())},
                    "method": ("learned warm start + " if mlp is not None else "cold start + ")
                              + "FP32 tile engine (continuous batching) + FP64 polish; cold FP64 fallback",
                    "mlp": {"hidden": 512, "layers": 3, "train_solutions": n_train, "final_loss": train_loss}
@@ -478,18 +546,19 @@ def main():
                                           "fp32_mean": float(fp32_it.mean()) if acc else None,
                                           "fp64_mean": float(fp64_it.mean()) if acc else None},
         "plain_fp64_single": {"solves_per_s": plain_rate, "variants": list(plain.values()),
+                              "iterations_per_solve": per_solve,
                               "basis": "measured seconds per iteration of the plain path (cold, bounded sample) "
-                                       "times the mean iterations the batched method needed per accepted solve; "
-                                       "a cold plain solve needs more, so this rate is an upper bound"},
+                                       "times the iterations plain FP64 SPOCK took per solve on the cost-check "
+                                       "sample (cold, same thresholds)"},
         "speedup_vs_plain": (value / plain_rate) if plain_rate else None,
+        "cost_check": cost,
         "schedule": schedules,
         "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": roof["achieved"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_tile_solve (FP32, %s)" % (s32,), "detail": roof,
+                     "frac": roof["achieved"] / HBM_PEAK_GBS, "traffic": measured_traffic(roof),
+                     "kernel": "k_tile_solve_x1024 (FP32, %s)" % (s32,), "detail": roof,
                      "unit_note": "per instance: T bytes = factors/16 + 2|z| + 2(nx N + nu nl) words, ell bytes "
                                   "= 2|z| + shared/16 words (FP32); summed over the instances' T and [L*; L] "
                                   "applications of one 50-iteration launch, divided by its kernel time"},
-        "cost_check": cost,
         "cpu_baseline": cpu,
         "wall_s": time.perf_counter() - T_START,
     }

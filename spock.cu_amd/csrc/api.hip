@@ -1095,7 +1095,8 @@ int spk_batch_set_tolerances(spk_batch *b, const double *tol) {
                 if (!(t >= 0.)) throw std::runtime_error("tolerances must be >= 0");
             if (b->lanes) {
                 check(hipStreamSynchronize(b->ctx->stream), "sync");
-                b->ltol.alloc(b->Bpad * b->elSize());
+                // the same buffer across calls: launches and captured graphs keep its address
+                if (b->ltol.bytes != b->Bpad * b->elSize()) b->ltol.alloc(b->Bpad * b->elSize());
                 std::vector<double> t(b->tolFixed);
                 t.resize(b->Bpad, 0.);
                 if (b->dtype == SPK_F64) toDevice<double>(b, static_cast<double *>(b->ltol.p), t.data(), t.size());
