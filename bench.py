@@ -75,6 +75,25 @@ def measured_traffic(roof):
         return None
 
 
+def calibration_copy(nbytes=1 << 30):
+    """A plain 1 GiB device-to-device copy through the HIP runtime the library links (hipMemcpy's
+    copy kernel): the known-size reference that scales FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    src, dst = C.c_void_p(), C.c_void_p()
+    calls = [("hipMalloc", lambda: hip.hipMalloc(C.byref(src), C.c_size_t(nbytes))),
+             ("hipMalloc", lambda: hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes))),
+             ("hipMemset", lambda: hip.hipMemset(src, 1, C.c_size_t(nbytes))),
+             ("hipMemcpy", lambda: hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)),  # device to device
+             ("hipDeviceSynchronize", lambda: hip.hipDeviceSynchronize()),
+             ("hipFree", lambda: hip.hipFree(src)), ("hipFree", lambda: hip.hipFree(dst))]
+    for name, call in calls:
+        rc = call()
+        if rc != 0:
+            raise RuntimeError(f"{name} failed with HIP error {rc}")
+    return nbytes
+
+
 def bytes_per_T(p, word, share=1):
     """Algorithmic bytes one application of the CP operator T moves for one instance: the per-node
     factors (ABKTr, APB per child; K, R~^-1 per nonleaf), the kernel projectors and the per-event
@@ -92,6 +111,21 @@ def bytes_per_ell(p, word, share=1):
     """Algorithmic bytes of one [L*; L] application inside a norm or error check."""
     shared = 2 * p.nx * p.nx + p.nu * p.nu + 2 * (p.nx + p.nu) + 2 * p.nx
     return word * (2 * p.n_iter + shared / share)
+
+
+def bytes_of_passes(p, word, results):
+    """Algorithmic bytes of SuperMann's iterate-sized vector work outside T and [L*; L], summed over
+    the instances' results (src/cache.cuh:817-904, 1041-1099): per outer iteration the residual
+    with the new Anderson column (read z, T(z), r_prev, z_prev and the m - 1 = 2 older Y columns;
+    write r and the new Y and S columns: 9 vectors) and the direction (read r_prev, z_prev and
+    the 3 S columns, write d: 6); per line-search step the trial point (read d, z_prev, write z
+    and tau d: 4) and its residual (read z, T(z), tau d, write r: 4); a K1 candidate copies one
+    vector (2), a K2 candidate combines two (3), a K3 restore copies one (2)."""
+    v = 0
+    for r in results:
+        steps = r["num_T"] - r["iters"]  # T applications of the line search
+        v += 15 * r["iters"] + 8 * steps + 2 * r["k1"] + 3 * r["k2"] + 2 * r["k3"]
+    return word * p.n_iter * v
 
 
 def s0_of(p, z):
@@ -303,9 +337,44 @@ def kernel_roofline(rt, ctx, prob, x0s, schedule, iters):
     nE = sum(r["num_ell"] for r in res)
     bt, bl = bytes_per_T(prob, 4, 16), bytes_per_ell(prob, 4, 16)
     alg = nT * bt + nE * bl
+    passes = bytes_of_passes(prob, 4, res)
     return {"seconds": sec, "T": nT, "ell": nE, "bytes_per_T": bt, "bytes_per_ell": bl, "alg_bytes": alg,
             "achieved": alg / sec / 1e9, "ms_per_iteration": 1e3 * sec / max(1, res[0]["iters"]),
-            "iterations": res[0]["iters"]}
+            "iterations": res[0]["iters"], "pass_bytes": passes,
+            "achieved_with_passes": (alg + passes) / sec / 1e9}
+
+
+def absolute_study(rt, ctx, prob, configs, args, stream, tol_abs_scaled):
+    """FP32 solves of held-out x0 at the absolute threshold: the first half of the slots warm-started
+    by the MLP, the second half cold from the same x0, in one pool (separate workgroups), for args.absolute_study
+    seconds; one progress line per 2,000 iterations with each half's accepted count and residual
+    percentiles of its running solves."""
+    mlp, n_train, loss = train_warm_start(rt, ctx, prob, configs, args, (17, 1024))
+    n = args.slots
+    half = n // 2
+    q = rt.Batch(ctx, [prob] * n, rt.F32, shared=True)
+    q.set_schedule(17, 1024)
+    q.set_tolerances([tol_abs_scaled] * n)
+    q.pool_open(rt.options(tol_abs=0.0, tol_rel=0.0, max_iters=0, chunk_iters=2000))
+    mlp.predict_lanes(stream[:half], q, np.arange(half))
+    q.refill(np.arange(half, n), stream[:n - half])  # the same x0, cold
+    t0, it = time.perf_counter(), 0
+    while time.perf_counter() - t0 < args.absolute_study:
+        q.resume()
+        res, sec = q.results()
+        it += 2000
+        st = np.array([r["status"] for r in res])
+        err = np.array([r["err"] for r in res])
+        parts = []
+        for name, sl in (("warm", slice(0, half)), ("cold", slice(half, n))):
+            run = st[sl] == -1
+            e = err[sl][run]
+            pc = np.nanpercentile(e, [10, 50, 90]) if e.size else [0, 0, 0]
+            parts.append(f"{name}: accepted {int((st[sl] == 0).sum())}/{st[sl].size}, residual p10/p50/p90 "
+                         f"{pc[0]:.2e}/{pc[1]:.2e}/{pc[2]:.2e}")
+        log(f"iteration {it} ({1e3 * sec / 2000:.1f} ms/iteration): " + "; ".join(parts))
+        if not (st == -1).any():
+            break
 
 
 def main():
@@ -336,6 +405,11 @@ def main():
     ap.add_argument("--cost-sample", type=int, default=128, help="accepted solutions kept for the cost check")
     ap.add_argument("--cost-seconds", type=float, default=60.0, help="time allowed for the plain FP64 reference solves")
     ap.add_argument("--no-cost", action="store_true")
+    ap.add_argument("--absolute-study", type=float, default=0.0, metavar="SECONDS",
+                    help="instead of the bench: the learned warm start (trained as in the bench) for half of "
+                         "--slots held-out x0 and cold starts of the same x0 in the other half, solved in FP32 at the "
+                         "absolute threshold toleranceAbsolute(--tol) for SECONDS, with progress every 2,000 "
+                         "iterations")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the dominant kernel: one launch of 50 cold iterations of --slots FP32 instances "
                          "(schedule 17, 1024 threads), then a 1 GiB device copy whose counters calibrate "
@@ -354,14 +428,13 @@ def main():
         log(f"problem: {prob.N} nodes, iterate {prob.n_iter}; absolute part of the threshold {tol_abs_scaled:.3e} "
             f"(scaled)")
         if args.kernel_only:
-            import torch
             roof = kernel_roofline(rt, ctx, prob, stream[:args.slots], (17, 1024), 50)
-            src = torch.ones(256 * 2 ** 20, dtype=torch.float32, device="cuda")
-            dst = torch.empty_like(src)
-            dst.copy_(src)
-            torch.cuda.synchronize()
-            print(json.dumps({"kernel_roofline": roof, "calibration_copy_bytes_each_way": src.numel() * 4}),
-                  flush=True)
+            nbytes = calibration_copy()
+            print(json.dumps({"kernel_roofline": roof, "calibration_copy_bytes_each_way": nbytes}), flush=True)
+            return
+
+        if args.absolute_study > 0:
+            absolute_study(rt, ctx, prob, configs, args, stream, tol_abs_scaled)
             return
 
         cpu = None
@@ -555,10 +628,15 @@ def main():
         "schedule": schedules,
         "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": roof["achieved"] / HBM_PEAK_GBS, "traffic": measured_traffic(roof),
+                     "achieved_with_passes": roof["achieved_with_passes"],
+                     "frac_with_passes": roof["achieved_with_passes"] / HBM_PEAK_GBS,
                      "kernel": "k_tile_solve_x1024 (FP32, %s)" % (s32,), "detail": roof,
                      "unit_note": "per instance: T bytes = factors/16 + 2|z| + 2(nx N + nu nl) words, ell bytes "
                                   "= 2|z| + shared/16 words (FP32); summed over the instances' T and [L*; L] "
-                                  "applications of one 50-iteration launch, divided by its kernel time"},
+                                  "applications of one 50-iteration launch, divided by its kernel time. "
+                                  "achieved_with_passes adds SuperMann's iterate-sized vector passes "
+                                  "(bytes_of_passes); traffic is FETCH_SIZE + WRITE_SIZE of the same launch "
+                                  "from rocprofv3, scaled by a 1 GiB copy"},
         "cpu_baseline": cpu,
         "wall_s": time.perf_counter() - T_START,
     }
