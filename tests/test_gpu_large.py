@@ -2,10 +2,12 @@
 quadratic-plus-linear costs, 200 polyhedral rows, AV@R 0.95) in grid mode: T against the oracle
 (1e-12) and an FP32 solve to 1e-3 whose FP64 polish is accepted against the same threshold.
 
-Skipped: a run of this workload with 256 and 512 cooperating workgroups ended in a segmentation
-fault at process exit, after every result had been printed (T within 6.6e-16 of the oracle, 5.8
-ms per outer iteration, the FP32 solve converged in 2,575 iterations).  Its cause is not
-found, so the case stays off the GPU until it is."""
+An earlier probe of this workload (256 and 512 cooperating workgroups) ended in a segmentation
+fault at process exit, after every result had been printed.  The cause was in the C ABI: at
+interpreter exit Python may finalize a context before the batches made on it, and
+spk_batch_destroy then synchronized the freed context's stream.  Handles are now reference
+counted (a batch or problem keeps its context alive; tests/test_handles_host.py destroys them in
+every order under AddressSanitizer), so these tests run again."""
 import numpy as np
 import pytest
 
@@ -14,7 +16,7 @@ from oracle.oracle import Oracle
 from spock_amd import runtime as rt
 from spock_amd.api import configs
 
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason="exit fault of the 256/512-workgroup run not explained yet")]
+pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
