@@ -266,8 +266,7 @@ def cpu_baseline(folder, x0, budget_s, iters_per_solve, workers):
             "sample": f"oracle FP64 SPOCK from held-out x0#0, {r['iters']} iterations per process, one process "
                       f"({1e3 * per_iter1:.2f} ms/iteration)" + (f" and {workers} processes in parallel "
                                                                  f"({1e3 * per_iter:.2f} ms/iteration aggregate)"
-                                                                 if multi else "")
-                      + f"; scaled by {iters_per_solve:.0f} iterations per solve"}
+                                                                 if multi else "")}
 
 
 def _oracle_iters(folder, x0, n):
