@@ -15,6 +15,8 @@
  *   spk_batch_results     <- status(), solveIter(), solveTime() (src/cache.cuh:357-361)
  *   spk_batch_get_iterate <- solution() (src/cache.cuh:378)
  *   spk_batch_reset       <- Cache::reset (src/cache.cuh:436-479)
+ *   spk_batch_set_debug   <- CacheBuilder::enableDebug (src/cache.cuh:1204-1209) and the debug
+ *   spk_batch_debug_records  caches printToJson writes (src/cache.cuh:947-959, :1274-1311)
  *   spk_batch_debug_op    <- the operator pieces the reference's gtest suite reaches through
  *                            friend functions (LinearOperator::op/adj, Dynamics::project,
  *                            projectPrimalWorkspaceOnKernels, iter(), dotM;
@@ -50,7 +52,10 @@ enum spk_dtype { SPK_F64 = 0, SPK_F32 = 1 };
 enum spk_error { SPK_OK = 0, SPK_ERR = -1, SPK_ERR_COOPERATIVE_LAUNCH = -2 };
 enum spk_method { SPK_SPOCK = 0, SPK_CP = 1 };
 /* solve status codes, identical to enum CacheStatus (src/cache.cuh:68-73) */
-enum spk_status { SPK_NOT_RUN = -1, SPK_CONVERGED = 0, SPK_OUT_OF_ITER = 1, SPK_OUT_OF_TIME = 2 };
+enum spk_status { SPK_NOT_RUN = -1, SPK_CONVERGED = 0, SPK_OUT_OF_ITER = 1, SPK_OUT_OF_TIME = 2,
+                  /* debug traces on: a candidate entry was not finite and the solve stopped there,
+                     where the reference's isFinite throws (src/cache.cuh:52-62, :912) */
+                  SPK_NONFINITE = 3 };
 /* Anderson least squares: Householder QR as the reference (geqrf/ormqr/trsm), or the Gram
  * matrix kept incrementally and solved by Cholesky in FP64 (one fused pass per iteration). */
 enum spk_ls_method { SPK_LS_HOUSEHOLDER = 0, SPK_LS_GRAM = 1 };
@@ -136,6 +141,21 @@ int spk_batch_set_workgroups(spk_batch *b, int workgroups);
 /* zero every iterate and workspace (Cache::reset, src/cache.cuh:436-479); the initial states set by
  * spk_batch_set_x0 are inputs and stay, where the reference re-uploads them on every runSpock */
 int spk_batch_reset(spk_batch *b);
+/* debug traces (per-instance batches): max_records > 0 makes every later solve record, at each
+ * error check (every check_period outer iterations, from 0), one record of SPK_DEBUG_WORDS doubles
+ *   {outer iteration, calls to L and L* of this solve (two per application of T), err0, err1, err2, err3}
+ * as the reference's debug caches (src/cache.cuh:947-959): err1 and err2 the max-norms of the
+ * primal and dual parts of r/gamma - [L*; L] r, err3 = ||z - T(z)||_2, and err0 as the reference
+ * computes it, e <- cand_prim - e then ||e||_2 with e zeroed only by spk_batch_reset (the L* product
+ * next to it in the reference never reaches e).  Every outer iteration also checks the candidate
+ * for non-finite entries and stops the solve with status SPK_NONFINITE at the first one.  Records
+ * beyond max_records are counted but not kept.  0 turns the traces off. */
+#define SPK_DEBUG_WORDS 6
+int spk_batch_set_debug(spk_batch *b, long max_records);
+/* the records of instance inst's last solve: up to cap of them into records (cap x SPK_DEBUG_WORDS),
+ * *count = the records the solve made (may exceed what was kept), *nonfinite = the index of the
+ * first non-finite candidate entry when the solve ended with SPK_NONFINITE, else -1 */
+int spk_batch_debug_records(spk_batch *b, int inst, long cap, double *records, long *count, long *nonfinite);
 /* initial states, unscaled, concatenated over instances (nx_k values each) */
 int spk_batch_set_x0(spk_batch *b, const double *x0);
 /* per-instance acceptance tolerances in the solver's scaled units (spk_result.tol of an earlier

@@ -11,6 +11,8 @@
 //   getDual / reset / sizePrimal / sizeDual     src/cache.cuh:357-412, 436-479
 //   CacheBuilder<T>::build / make_unique        src/cache.cuh:1215-1244
 //   enum CacheStatus and toString               src/cache.cuh:68-88
+//   CacheBuilder::enableDebug, the debug caches  src/cache.cuh:947-959, 1088-1130, 1204-1209,
+//   and printToJson (log/cacheSp.json, ...Cp)    :1274-1311
 // Errors are std::invalid_argument with the library's message, as the reference throws.
 // T = double solves in FP64, T = float in FP32 (the reference instantiates both,
 // src/definitions.cu:24-443).  solution(), solutionPrim() and solutionDual() return a DTensor<T>&
@@ -20,8 +22,13 @@
 // std::vector<T>.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <filesystem>
+#include <iostream>
 #include <fstream>
 #include <memory>
 #include <sstream>
@@ -133,6 +140,16 @@ public:
     size_t numNonleafNodes() const { return field("numNonleafNodes"); }
     size_t numStages() const { return field("numStages"); }
     size_t numEvents() const { return field("numEvents"); }
+    // the largest number of children of any node of each stage, 0 for the last (src/tree.cuh:171-178)
+    std::vector<size_t> childMax() const {
+        const std::vector<size_t> nc = spock_amd::parseFromFile<size_t>(m_path + "numChildren_u.bt");
+        const std::vector<size_t> from = spock_amd::parseFromFile<size_t>(m_path + "stageFrom_u.bt");
+        const std::vector<size_t> to = spock_amd::parseFromFile<size_t>(m_path + "stageTo_u.bt");
+        std::vector<size_t> out(numStages(), 0);
+        for (size_t st = 0; st + 1 < numStages() && st < from.size() && st < to.size(); st++)
+            for (size_t n = from[st]; n <= to[st] && n < nc.size(); n++) out[st] = std::max(out[st], nc[n]);
+        return out;
+    }
 
 private:
     size_t field(const char *key) const {
@@ -188,11 +205,28 @@ public:
     static constexpr size_t kAutoGridMin = 50000, kAutoGridMax = 100000;
 
     // workgroups: how many workgroups solve the instance together (0: automatic, see above)
-    Cache(ScenarioTree<T> &, ProblemData<T> &data, const spk_options &opts, int workgroups = 0)
-        : m_data(&data), m_opts(opts) {
+    // debug: CacheBuilder::enableDebug (src/cache.cuh:1204-1209, 312-318)
+    Cache(ScenarioTree<T> &tree, ProblemData<T> &data, const spk_options &opts, int workgroups = 0,
+          bool debug = false)
+        : m_data(&data), m_opts(opts), m_debug(debug) {
         spk_problem *p = data.handle();
         detail::check(spk_batch_create(detail::context(), 1, &p, sizeof(T) == 8 ? SPK_F64 : SPK_F32, &m_b),
                       "Cache");
+        if (m_debug) {
+            const size_t len = (size_t) std::max<long>(m_opts.max_iters, 0);
+            m_cacheCallsToL.assign(len, 0);
+            m_cacheError0.assign(len, T(0));
+            m_cacheError1.assign(len, T(0));
+            m_cacheError2.assign(len, T(0));
+            m_cacheError3.assign(len, T(0));
+            const long period = m_opts.check_period > 0 ? m_opts.check_period : 25;
+            m_records = m_opts.max_iters / period + 2;
+            detail::check(spk_batch_set_debug(m_b, m_records), "Cache");
+            m_horizon = tree.numStages() - 1;
+            const std::vector<size_t> cm = tree.childMax();
+            m_branchFactor = (size_t) (std::find(cm.begin(), cm.end(), (size_t) 1) - cm.begin());
+            m_numEvents = tree.numEvents();
+        }
         if (workgroups == 0) {
             const size_t n = data.sizePrimal() + data.sizeDual();
             workgroups = (n >= kAutoGridMin && n <= kAutoGridMax) ? 64 : 1;
@@ -202,9 +236,14 @@ public:
         detail::check(spk_batch_set_workgroups(m_b, workgroups), "Cache");
     }
     ~Cache() { spk_batch_destroy(m_b); }
-    Cache(Cache &&o) noexcept : m_data(o.m_data), m_opts(o.m_opts), m_b(o.m_b), m_autoGrid(o.m_autoGrid),
-                                m_workgroups(o.m_workgroups), m_gridFallback(o.m_gridFallback), m_exit(o.m_exit),
-                                m_iters(o.m_iters), m_time(o.m_time) {
+    Cache(Cache &&o) noexcept
+        : m_data(o.m_data), m_opts(o.m_opts), m_debug(o.m_debug), m_b(o.m_b), m_autoGrid(o.m_autoGrid),
+          m_workgroups(o.m_workgroups), m_gridFallback(o.m_gridFallback), m_exit(o.m_exit), m_iters(o.m_iters),
+          m_time(o.m_time), m_tol(o.m_tol), m_records(o.m_records), m_callsToL(o.m_callsToL),
+          m_horizon(o.m_horizon), m_branchFactor(o.m_branchFactor), m_numEvents(o.m_numEvents),
+          m_cacheCallsToL(std::move(o.m_cacheCallsToL)), m_cacheError0(std::move(o.m_cacheError0)),
+          m_cacheError1(std::move(o.m_cacheError1)), m_cacheError2(std::move(o.m_cacheError2)),
+          m_cacheError3(std::move(o.m_cacheError3)) {
         o.m_b = nullptr;
     }
     Cache(const Cache &) = delete;
@@ -247,6 +286,7 @@ public:
         detail::check(spk_batch_get_iterate(m_b, 0, z.data()), "getDual");
         std::copy(primal.begin(), primal.end(), out.begin());
         detail::check(spk_batch_debug_op(m_b, SPK_OP_DUAL, z.data(), out.data(), nullptr), "getDual");
+        if (m_debug) m_callsToL += 1;  // modifyDual's L(true) (src/cache.cuh:771-776)
         return std::vector<T>(out.begin() + sizePrimal(), out.end());
     }
 
@@ -272,9 +312,60 @@ public:
         return u;
     }
 
+    // zeroes the iterates and, as the reference, the callsL, err0, err1 and err2 caches but not err3
+    // nor the running count of calls to L (src/cache.cuh:437-478)
     void reset() {
         detail::check(spk_batch_reset(m_b), "reset");
         m_exit = notRun;
+        std::fill(m_cacheCallsToL.begin(), m_cacheCallsToL.end(), 0);
+        std::fill(m_cacheError0.begin(), m_cacheError0.end(), T(0));
+        std::fill(m_cacheError1.begin(), m_cacheError1.end(), T(0));
+        std::fill(m_cacheError2.begin(), m_cacheError2.end(), T(0));
+    }
+
+    // the debug caches, one entry per outer iteration (zero where no error check ran)
+    const std::vector<size_t> &cacheCallsToL() const { return m_cacheCallsToL; }
+    const std::vector<T> &cacheError0() const { return m_cacheError0; }
+    const std::vector<T> &cacheError1() const { return m_cacheError1; }
+    const std::vector<T> &cacheError2() const { return m_cacheError2; }
+    const std::vector<T> &cacheError3() const { return m_cacheError3; }
+
+    // log/cache<file>.json under the working directory (src/cache.cuh:1274-1311)
+    void printToJson(const std::string &file) {
+        namespace fs = std::filesystem;
+        const fs::path path = fs::current_path() / "log" / ("cache" + file + ".json");
+        fs::create_directories(path.parent_path());
+        std::ofstream f(path);
+        if (!f.is_open()) throw std::runtime_error("[Cache::printToJson] Failed to open file for writing.\n");
+        auto num = [&](double v) {
+            if (!std::isfinite(v)) { f << "null"; return; }
+            char buf[40];
+            std::snprintf(buf, sizeof(buf), "%.17g", v);
+            f << buf;
+        };
+        auto arr = [&](const char *name, const auto &v) {
+            f << ",\"" << name << "\":[";
+            for (size_t k = 0; k < v.size(); k++) {
+                if (k) f << ",";
+                num((double) v[k]);
+            }
+            f << "]";
+        };
+        f << "{\"maxIters\":" << m_opts.max_iters << ",\"tol\":";
+        num((double) m_tol);
+        f << ",\"horizon\":" << m_horizon << ",\"branchFactor\":" << m_branchFactor << ",\"numEvents\":"
+          << m_numEvents << ",\"numStates\":" << m_data->numStates() << ",\"numInputs\":" << m_data->numInputs()
+          << ",\"sizeCache\":" << m_opts.max_iters;
+        arr("callsL", m_cacheCallsToL);
+        arr("err0", m_cacheError0);
+        arr("err1", m_cacheError1);
+        arr("err2", m_cacheError2);
+        arr("err3", m_cacheError3);
+        arr("states", states());
+        arr("inputs", inputs());
+        f << "}";
+        f.close();
+        if (!f.good()) throw std::runtime_error("[Cache::printToJson] Can't write the JSON string to the file.\n");
     }
 
 private:
@@ -314,12 +405,61 @@ private:
         detail::check(spk_batch_results(m_b, &r, nullptr), "results");
         m_exit = r.status;
         m_iters = (size_t) r.iters;
+        m_tol = (T) r.tol;
         m_time = (T) std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+        if (m_debug) debugReport(method, r);
         return m_exit;
+    }
+
+    // the debug caches from the solve's records, then the reference's JSON and console summary
+    // (src/cache.cuh:947-959, 988-1001, 1101-1130)
+    void debugReport(int method, const spk_result &r) {
+        std::vector<double> rec((size_t) m_records * SPK_DEBUG_WORDS);
+        long made = 0, bad = -1;
+        detail::check(spk_batch_debug_records(m_b, 0, m_records, rec.data(), &made, &bad), "debug");
+        const size_t len = m_cacheError0.size();
+        for (long i = 0; i < std::min(made, m_records); i++) {
+            const double *q = &rec[(size_t) i * SPK_DEBUG_WORDS];
+            const size_t idx = (size_t) q[0];
+            if (idx >= len) continue;  // the check at maxIters itself (the reference writes past its caches)
+            m_cacheCallsToL[idx] = m_callsToL + (size_t) q[1];
+            m_cacheError0[idx] = (T) q[2];
+            m_cacheError1[idx] = (T) q[3];
+            m_cacheError2[idx] = (T) q[4];
+            m_cacheError3[idx] = (T) q[5];
+        }
+        m_callsToL += 2 * (size_t) r.num_T;
+        if (r.status == SPK_NONFINITE)
+            throw std::invalid_argument("[isFinite] DTensor has a non-finite entry at (" + std::to_string(bad) +
+                                        ").\n");
+        printToJson(method == SPK_SPOCK ? "Sp" : "Cp");
+        if (method == SPK_CP) {
+            if (m_exit == converged)
+                std::cout << "\nConverged in " << m_iters << " iterations, to a tolerance of " << m_tol << "\n";
+            else if (m_exit == outOfIter)
+                std::cout << "\nOut of iterations (" << m_opts.max_iters << " iters).\n";
+            else if (m_exit == outOfTime)
+                std::cout << "\nOut of time (" << m_opts.max_time_secs << " secs).\n";
+            else
+                std::cout << "\nExited.\n";
+            return;
+        }
+        const std::string counts = ", K1: " + std::to_string(r.k1) + ", K2: " + std::to_string(r.k2) +
+                                   ", bt: " + std::to_string(r.k2bt) + ", K3: " + std::to_string(r.k3) + "].\n";
+        if (m_exit == converged)
+            std::cout << "\nConverged in " << m_iters << " iters and " << m_time << " secs, to a tolerance of "
+                      << m_tol << ", [K0: " << r.k0 << counts;
+        else if (m_exit == outOfIter)
+            std::cout << "\nMax iterations (" << m_opts.max_iters << ") reached [K0: " << r.k0 << counts;
+        else if (m_exit == outOfTime)
+            std::cout << "\nMax time (" << m_opts.max_time_secs << "s) reached [K0: " << r.k0 << counts;
+        else
+            std::cout << "\nExited.\n";
     }
 
     ProblemData<T> *m_data;
     spk_options m_opts;
+    bool m_debug = false;
     spk_batch *m_b = nullptr;
     bool m_autoGrid = false;  // grid mode chosen by size, not asked for
     int m_workgroups = 1;
@@ -328,6 +468,14 @@ private:
     int m_exit = notRun;
     size_t m_iters = 0;
     T m_time = 0;
+    T m_tol = 0;
+    // debug caches; m_callsToL runs over every solve of this cache, as the reference's counter,
+    // which nothing resets (src/cache.cuh:111)
+    long m_records = 0;
+    size_t m_callsToL = 0;
+    size_t m_horizon = 0, m_branchFactor = 0, m_numEvents = 0;
+    std::vector<size_t> m_cacheCallsToL;
+    std::vector<T> m_cacheError0, m_cacheError1, m_cacheError2, m_cacheError3;
 };
 
 template <typename T>
@@ -343,17 +491,25 @@ public:
     CacheBuilder &maxIters(size_t iters) { m_opts.max_iters = (long) iters; return *this; }
     CacheBuilder &maxItersInner(size_t iters) { m_opts.max_inner = (int) iters; return *this; }
     CacheBuilder &andersonBuffer(size_t buffer) { m_opts.anderson_buffer = (int) buffer; return *this; }
-    CacheBuilder &enableDebug(bool = true) { return *this; }  // per-iteration traces: not provided
+    // debug caches and JSON dump; with no iteration cap set yet, caps at 1,000 (src/cache.cuh:1204-1209)
+    CacheBuilder &enableDebug(bool enable = true) {
+        if (enable && m_opts.max_iters == 0) m_opts.max_iters = 1000;
+        m_debug = enable;
+        return *this;
+    }
     // workgroups per solve (no counterpart in the reference): 0 automatic, 1, or n > 1 (grid mode)
     CacheBuilder &workgroups(int n) { m_wgs = n; return *this; }
-    Cache<T> build() { return Cache<T>(m_tree, m_data, m_opts, m_wgs); }
-    std::unique_ptr<Cache<T>> make_unique() { return std::make_unique<Cache<T>>(m_tree, m_data, m_opts, m_wgs); }
+    Cache<T> build() { return Cache<T>(m_tree, m_data, m_opts, m_wgs, m_debug); }
+    std::unique_ptr<Cache<T>> make_unique() {
+        return std::make_unique<Cache<T>>(m_tree, m_data, m_opts, m_wgs, m_debug);
+    }
 
 private:
     ScenarioTree<T> &m_tree;
     ProblemData<T> &m_data;
     spk_options m_opts;
     int m_wgs = 0;
+    bool m_debug = false;
 };
 
 }  // namespace spock_amd

@@ -49,6 +49,8 @@ def _lib():
     lib.orc_lstsq_gram.argtypes = [_dp, C.c_long, C.c_int, _dp, _dp]
     lib.orc_solve.restype = C.c_int
     lib.orc_solve.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _lp, _dp, _lp, _dp, _ip, _dp, C.c_long]
+    lib.orc_solve_debug.restype = C.c_int
+    lib.orc_solve_debug.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _lp, _dp, _lp, _dp, _dp, _dp, C.c_long, _lp]
     return lib
 
 
@@ -171,6 +173,30 @@ class Oracle:
         if trace_len:
             res["branch"], res["w"] = tb, tw
         return res
+
+
+    def solve_debug(self, x0, err_acc, method="spock", warm=None, tol_abs=1e-3, tol_rel=0.0, max_iters=0,
+                    max_inner=8, anderson=3, check_period=25, cap=100000):
+        """solve() with the reference's debug caches (src/cache.cuh:947-959).  err_acc (n_prim,
+        float64, updated in place) is err0's running vector: zeros after a reset, carried from solve
+        to solve otherwise.  Returns solve()'s fields plus records (rows {idx, callsL, err0, err1,
+        err2, err3}), num_T and nonfinite (first non-finite candidate entry, status 3; else -1)."""
+        assert err_acc.dtype == np.float64 and err_acc.size == self.n_prim and err_acc.flags.c_contiguous
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        z = np.zeros(self.n_iter)
+        out = np.zeros(7, dtype=np.int64)
+        outd = np.zeros(2)
+        dopts = np.array([tol_abs, tol_rel, 0.0, 0.0])
+        iopts = np.array([max_iters, max_inner, anderson, check_period, 0, 0], dtype=np.int64)
+        wp = _d(np.ascontiguousarray(warm, dtype=np.float64)) if warm is not None else None
+        rec = np.zeros((cap, 6))
+        info = np.zeros(3, dtype=np.int64)
+        lib().orc_solve_debug(self.h, 0 if method == "spock" else 1, _d(x0), wp, _d(dopts),
+                              iopts.ctypes.data_as(_lp), _d(z), out.ctypes.data_as(_lp), _d(outd), _d(err_acc),
+                              _d(rec), cap, info.ctypes.data_as(_lp))
+        return dict(z=z, iters=int(out[0]), status=int(out[1]), k0=int(out[2]), k1=int(out[3]), k2=int(out[4]),
+                    k2bt=int(out[5]), k3=int(out[6]), err=float(outd[0]), tol=float(outd[1]),
+                    records=rec[:min(int(info[0]), cap)], num_T=int(info[2]), nonfinite=int(info[1]))
 
 
 def lstsq(Y, r):

@@ -769,6 +769,12 @@ struct Solver {
     int status = -1;
     long k0 = 0, k1 = 0, k2 = 0, k2bt = 0, k3 = 0;
     Trace trace;
+    // debug caches (src/cache.cuh:947-959): on when errAcc is set.  nT counts applications of T
+    // (each calls L and L* once, the calls the reference counts, src/cache.cuh:704-716, :750, :773)
+    long nT = 0;
+    double *errAcc = nullptr;     // err0's running vector e (sPrim), kept by the caller across solves
+    std::vector<double> records;  // {idx, callsL, err0, err1, err2, err3} per error check
+    long nonfinite = -1;
 
     Solver(const Problem &pp, const Opts &oo) : p(pp), o(oo) {
         size_t n = p.sIter;
@@ -790,11 +796,37 @@ struct Solver {
     // computeError (src/cache.cuh:910-961); returns true when the solve ends
     bool computeError(long idx, bool &initFlag, const std::chrono::steady_clock::time_point &t0, Vec &ellR) {
         size_t n = p.sIter;
+        if (errAcc) {  // isFinite(candidate) (src/cache.cuh:52-62, :912): the reference throws
+            for (size_t k = 0; k < n; k++)
+                if (!std::isfinite(cand[k])) {
+                    nonfinite = (long) k;
+                    status = 3;
+                    iters = idx;
+                    return true;
+                }
+        }
         for (size_t k = 0; k < n; k++) r[k] = z[k] - cand[k];
         if (idx % o.checkPeriod != 0) return false;
         ell(p, r.data(), ellR.data());
         double mx = 0.;
         for (size_t k = 0; k < n; k++) mx = std::max(mx, std::fabs(r[k] * p.gammaRecip - ellR[k]));
+        if (errAcc) {
+            // err1/err2: max-norms of the primal and dual parts of r/gamma - ell(r); err3 = ||z - cand||;
+            // err0: e <- cand_prim - e, ||e|| (the reference's m_d_err is never given the L* product
+            // computed beside it, so e alternates over the candidates since the last reset)
+            double e1 = 0., e2 = 0., s3 = 0., s0 = 0.;
+            for (size_t k = 0; k < n; k++) {
+                const double v = std::fabs(r[k] * p.gammaRecip - ellR[k]);
+                if (k < p.sPrim) e1 = std::max(e1, v); else e2 = std::max(e2, v);
+                s3 += r[k] * r[k];
+            }
+            for (size_t k = 0; k < p.sPrim; k++) {
+                errAcc[k] = cand[k] - errAcc[k];
+                s0 += errAcc[k] * errAcc[k];
+            }
+            const double rec[] = {(double) idx, (double) (2 * nT), std::sqrt(s0), e1, e2, std::sqrt(s3)};
+            records.insert(records.end(), rec, rec + 6);
+        }
         if (initFlag && o.tolFixed > 0.) {
             tol = o.tolFixed;
             initFlag = false;
@@ -865,7 +897,11 @@ struct Solver {
         for (long it = 0; it < hardCap; it++) {
             if (it != 0) z = cand;
             opT(p, x0s.data(), z.data(), cand.data());
-            if (computeError(it, initFlag, t0, ellR)) { z = cand; break; }
+            nT++;
+            if (computeError(it, initFlag, t0, ellR)) {
+                if (status != 3) z = cand;
+                break;
+            }
             backup = cand;
             // normM(r): w = sqrt(r' (r - gamma ell(r)))  (src/cache.cuh:740-742)
             if (o.adjDot) {
@@ -892,6 +928,7 @@ struct Solver {
             for (long in = 0; in < o.maxInner; in++) {  // line search (src/cache.cuh:1063-1099)
                 for (size_t k = 0; k < n; k++) { dirS[k] = dir[k] * tau; z[k] = zPrev[k] + dirS[k]; }
                 opT(p, x0s.data(), z.data(), cand.data());
+                nT++;
                 for (size_t k = 0; k < n; k++) r[k] = z[k] - cand[k];
                 wTilde = std::sqrt(o.adjDot ? dotMAdj(p, r.data(), r.data()) : dotM(p, r.data(), r.data()));
                 if (w <= wSafe && wTilde <= c1 * w) {  // K1
@@ -929,8 +966,9 @@ struct Solver {
         Vec ellR(p.sIter);
         for (long it = 0; it < 10000000; it++) {
             opT(p, x0s.data(), z.data(), cand.data());
+            nT++;
             bool done = computeError(it, initFlag, t0, ellR);
-            z = cand;
+            if (!(done && status == 3)) z = cand;
             if (done) break;
         }
         return status;
@@ -1019,6 +1057,33 @@ int orc_solve(void *h, int method, const double *x0, const double *warm, const d
             if (traceW) traceW[i] = s.trace.w[i];
         }
     }
+    return st;
+}
+
+// orc_solve with the debug caches on (CacheBuilder::enableDebug, src/cache.cuh:947-959):
+// errAcc (sPrim, in/out) is err0's running vector, zero after a reset; records (cap x 6) receive
+// {idx, callsL, err0, err1, err2, err3} per error check; info = {records made, first non-finite
+// candidate entry or -1, applications of T}.  Status 3: a non-finite candidate entry stopped the solve.
+int orc_solve_debug(void *h, int method, const double *x0, const double *warm, const double *dopts,
+                    const long *iopts, double *zOut, long *out, double *outD, double *errAcc, double *records,
+                    long cap, long *info) {
+    auto &p = *static_cast<Problem *>(h);
+    Opts o{dopts[0], dopts[1], dopts[2], iopts[0], iopts[1], iopts[2], iopts[3] > 0 ? iopts[3] : 25, iopts[4],
+           dopts[3], iopts[5]};
+    Solver s(p, o);
+    s.errAcc = errAcc;
+    int st = method == 0 ? s.runSpock(x0, warm, false) : s.runCp(x0, warm);
+    std::copy(s.z.begin(), s.z.end(), zOut);
+    if (out) {
+        long v[] = {s.iters, (long) st, s.k0, s.k1, s.k2, s.k2bt, s.k3};
+        std::memcpy(out, v, sizeof(v));
+    }
+    if (outD) { outD[0] = s.lastErr; outD[1] = s.tol; }
+    const long made = (long) s.records.size() / 6;
+    std::copy(s.records.begin(), s.records.begin() + 6 * std::min(made, cap), records);
+    info[0] = made;
+    info[1] = s.nonfinite;
+    info[2] = s.nT;
     return st;
 }
 

@@ -214,6 +214,84 @@ def affine_markov(folder, seed=8, dt="d"):
             .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
 
 
+def server_air_condition(folder, horizon=5, dt="d"):
+    """The reference's server air-conditioning example (examples/serverAirCondition/main.py:1-86):
+    2 events, Markov transitions [[.5, .5], [.5, .5]] from [.3, .7], branching at every stage;
+    nx = nu = 50; A_w tridiagonal with 0.01 off the diagonal and 1 + (1 + j/50) w/2 on it, B = I;
+    Q = 0.1 I, R = I, leaf 0.1 I; |x| <= 1, |u| <= 1.5 (leaf |x| <= 1); AV@R 0.95; preconditioned.
+    The initial state 0.1 * ones is written as initialState, as the example does."""
+    E, nu = 2, 50
+    nx = nu
+    tr = tree_mod.MarkovChain(np.array([[0.5, 0.5], [0.5, 0.5]]), np.array([0.3, 0.7]), horizon, horizon,
+                              dt=dt).build()
+    off = 0.01 * np.ones(nx - 1)
+    dyn = []
+    for w in range(E):
+        A = np.diag(off, -1) + np.diag(off, 1)
+        for j in range(nx):
+            A[j, j] = 1.0 + (1.0 + j / nx) * w / E
+        dyn.append(build.Dynamics(A, np.eye(nu)))
+    costs = [build.CostQuadratic(0.1 * np.eye(nx), np.eye(nu)) for _ in range(E)]
+    leaf = build.CostQuadratic(0.1 * np.eye(nx), leaf=True)
+    x_ub, u_ub = np.ones(nx), 1.5 * np.ones(nu)
+    ncon = build.Rectangle(np.hstack((-x_ub, -u_ub)), np.hstack((x_ub, u_ub)))
+    lcon = build.Rectangle(-x_ub, x_ub)
+    prob = (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(ncon).with_constraint_leaf(lcon)
+            .with_risk(build.AVaR(0.95)).with_preconditioning().generate_problem(folder))
+    tr.write_to_file_fp(folder, "initialState", 0.1 * np.ones(nx))
+    return prob
+
+
+def random_example(folder, seed=None, lo=100, hi=1000, count="states_and_inputs", dt="d"):
+    """The reference's random problem generator (examples/andersonBuffer/main.py:15-137 and
+    examples/random/main.py, which differ only in how the variables are counted): sizes are drawn
+    until lo < variables < hi, with horizon U{5..15}, branching stops at U{1..3}, events U{2..10},
+    nu U{10..300} and nx = 2 nu; an i.i.d. tree; A_w = I + N(0, .01^2), B_w = B + N(0, .01^2) with
+    B ~ N(0, 1); Q_w = (Q + N(0, .01^2))(.)' and R_w likewise from diagonal Q ~ U[0, .1], R ~ U[0, 100];
+    leaf cost Q; boxes |x| <= U[1, 2], |u| <= U[0, .1]; AV@R alpha ~ U[0, 1]; preconditioned.  The
+    initial state, U[-ub/2, ub/2] per state, is written as initialState.  count: "states_and_inputs"
+    counts nodes * (nx + nu) (the andersonBuffer study), "random" nodes * nx + nonleaf nodes * nu.
+    seed=None draws as the reference does (unseeded); a seed makes the draw reproducible."""
+    rng = np.random.default_rng(seed)
+    num_vars = np.inf
+    while not (lo < num_vars < hi):
+        horizon = int(rng.integers(5, 15, endpoint=True))
+        stopping = int(rng.integers(1, 3, endpoint=True))
+        E = int(rng.integers(2, 10, endpoint=True))
+        nu = int(rng.integers(10, 300, endpoint=True))
+        nx = 2 * nu
+        nodes = (E ** (stopping + 1) - 1) / (E - 1) + E ** stopping * (horizon - stopping)
+        if count == "random":
+            num_vars = nodes * nx + (nodes - E ** stopping) * nu
+        else:
+            num_vars = nodes * (nx + nu)
+    r = rng.uniform(size=E)
+    tr = tree_mod.IidProcess(r / r.sum(), horizon, stopping, dt=dt).build()
+    B_base = rng.normal(0., 1., size=(nx, nu))
+    dyn = [build.Dynamics(np.eye(nx) + rng.normal(0., .01, size=(nx, nx)),
+                          B_base + rng.normal(0., .01, size=(nx, nu))) for _ in range(E)]
+    Q_base = np.diag(rng.uniform(0., .1, size=nx))
+    R_base = np.diag(rng.uniform(0., 100., size=nu))
+    costs = []
+    for _ in range(E):
+        Qw = Q_base + rng.normal(0., .01, size=(nx, nx))
+        Rw = R_base + rng.normal(0., .01, size=(nu, nu))
+        costs.append(build.CostQuadratic(Qw @ Qw.T, Rw @ Rw.T))
+    leaf = build.CostQuadratic(Q_base, leaf=True)
+    x_ub = rng.uniform(1., 2., nx)
+    u_ub = rng.uniform(0., .1, nu)
+    ncon = build.Rectangle(np.hstack((-x_ub, -u_ub)), np.hstack((x_ub, u_ub)))
+    lcon = build.Rectangle(-x_ub, x_ub)
+    alpha = rng.uniform(0., 1.)
+    prob = (problem.Factory(tr, nx, nu).with_dynamics_events(dyn).with_cost_nonleaf_events(costs)
+            .with_cost_leaf(leaf).with_constraint_nonleaf(ncon).with_constraint_leaf(lcon)
+            .with_risk(build.AVaR(alpha)).with_preconditioning().generate_problem(folder))
+    x0 = np.array([rng.uniform(-.5 * x_ub[k], .5 * x_ub[k]) for k in range(nx)])
+    tr.write_to_file_fp(folder, "initialState", x0)
+    return prob
+
+
 def initial_states(seed, nx, count, kind="normal"):
     """Initial states: ``normal`` ~ N(0, 0.5^2 I), ``small`` ~ N(0, 0.05^2 I), ``uniform`` ~ U[-1, 1]."""
     rng = np.random.default_rng([seed, 1])
@@ -227,4 +305,5 @@ def initial_states(seed, nx, count, kind="normal"):
 def generate(name, folder, **kw):
     os.makedirs(folder, exist_ok=True)
     return {"fixture": fixture_problem, "small": small_markov, "mid": mid_markov, "poly": polyhedral_small, "nocon": unconstrained_markov, "affine": affine_markov, "pwi": polyhedron_identity_small, "linear": linear_cost_markov,
-            "large": polyhedral_iid}[name](folder, **kw)
+            "large": polyhedral_iid, "server": server_air_condition,
+            "random": random_example}[name](folder, **kw)

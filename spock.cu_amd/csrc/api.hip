@@ -140,6 +140,10 @@ struct spk_batch {
     bool tuned = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool pending = false;
+    // debug trace of per-instance batches (spk_batch_set_debug): records, err0's running vectors
+    long dbgCap = 0;
+    DevBuf dbgRec, dbgE;
+    std::vector<size_t> dbgEOff;
     size_t elSize() const { return dtype == SPK_F64 ? 8 : 4; }
 };
 
@@ -293,6 +297,11 @@ void fillInsts(spk_batch *b, const spk_options &o, int warm, T *anders, const st
         I.dynInverse = o.dyn_inverse;
         I.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
         I.gpart = gpart;
+        if (b->dbgCap > 0) {
+            I.dbg = static_cast<double *>(b->dbgRec.p) + (size_t) k * b->dbgCap * spk::DBG_WORDS;
+            I.dbgCap = b->dbgCap;
+            I.dbgE = static_cast<T *>(b->dbgE.p) + b->dbgEOff[k];
+        }
     }
 }
 
@@ -1041,6 +1050,60 @@ int spk_batch_set_workgroups(spk_batch *b, int workgroups) {
     }
 }
 
+static_assert(spk::DBG_WORDS == SPK_DEBUG_WORDS, "debug record layout");
+
+int spk_batch_set_debug(spk_batch *b, long max_records) {
+    try {
+        if (max_records < 0) throw std::invalid_argument("spk_batch_set_debug: max_records must be >= 0");
+        if (b->lanes)
+            throw std::invalid_argument("spk_batch_set_debug: debug traces are recorded by per-instance batches "
+                                        "(spk_batch_create), not shared ones");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        b->dbgCap = max_records;
+        if (max_records == 0) {
+            b->dbgRec.release();
+            b->dbgE.release();
+            b->dbgEOff.clear();
+            return 0;
+        }
+        b->dbgRec.alloc((size_t) b->count * max_records * spk::DBG_WORDS * sizeof(double));
+        check(hipMemset(b->dbgRec.p, 0, b->dbgRec.bytes), "zero debug records");
+        b->dbgEOff.resize(b->count);
+        size_t off = 0;
+        for (int k = 0; k < b->count; k++) {
+            b->dbgEOff[k] = off;
+            off += alignUp((size_t) b->probs[k]->hp.sPrim, 64);
+        }
+        b->dbgE.alloc(off * b->elSize());
+        check(hipMemset(b->dbgE.p, 0, b->dbgE.bytes), "zero debug vectors");
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_debug_records(spk_batch *b, int inst, long cap, double *records, long *count, long *nonfinite) {
+    try {
+        if (b->lanes || b->dbgCap == 0)
+            throw std::invalid_argument("spk_batch_debug_records: debug traces are off (spk_batch_set_debug)");
+        if (inst < 0 || inst >= b->count) throw std::invalid_argument("spk_batch_debug_records: no such instance");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        spk::Out o;
+        check(hipMemcpy(&o, static_cast<spk::Out *>(b->dOut.p) + inst, sizeof(o), hipMemcpyDeviceToHost),
+              "debug result");
+        const long n = std::min(std::min(o.nDbg, b->dbgCap), std::max(cap, 0L));
+        if (n > 0)
+            check(hipMemcpy(records, static_cast<double *>(b->dbgRec.p) + (size_t) inst * b->dbgCap * spk::DBG_WORDS,
+                            (size_t) n * spk::DBG_WORDS * sizeof(double), hipMemcpyDeviceToHost),
+                  "debug records");
+        if (count) *count = o.nDbg;
+        if (nonfinite) *nonfinite = o.nonfinite;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
 int spk_batch_reset(spk_batch *b) {
     try {
         check(hipStreamSynchronize(b->ctx->stream), "sync");
@@ -1050,6 +1113,7 @@ int spk_batch_reset(spk_batch *b) {
         }
         check(hipMemset(b->core.p, 0, b->coreElems * b->elSize()), "reset");
         if (b->anders.p) check(hipMemset(b->anders.p, 0, b->andElems * b->elSize()), "reset");
+        if (b->dbgE.p) check(hipMemset(b->dbgE.p, 0, b->dbgE.bytes), "reset");  // err0's vector (src/cache.cuh:469)
         // the initial states are inputs of the next solve (spk_batch_set_x0) and stay; the
         // reference zeroes its copy because runSpock(x0) uploads x0 again (src/cache.cuh:448,1011)
         return 0;

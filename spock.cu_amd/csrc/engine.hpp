@@ -37,7 +37,16 @@ struct Inst {
     int maxInner, andBuff, checkPeriod, method, warm, lsMethod, dynInverse;
     double maxTicks;  // wall-clock cap in device ticks (0 = none)
     T *gpart;         // grid mode: 2 x GRID_MAXK x GRID_MAXWG reduction partials
+    // debug trace (enableDebug, src/cache.cuh:947-959); dbg == nullptr: off.  One record of
+    // DBG_WORDS doubles per error check, at most dbgCap; dbgE (sPrim) carries err0's running
+    // vector from check to check and from solve to solve
+    double *dbg;
+    long dbgCap;
+    T *dbgE;
 };
+
+// A debug record: outer iteration, calls to L and L* so far, err0, err1, err2, err3.
+constexpr int DBG_WORDS = 6;
 
 // Grid mode limits: values per grid reduction and workgroups per cooperative launch.
 constexpr int GRID_MAXK = 16;
@@ -111,6 +120,8 @@ struct Out {
     double err, tol, w;
     long k0, k1, k2, k2bt, k3, nT, nEll;
     long ticks[NUM_PHASES];  // device wall-clock ticks spent per phase (thread 0's view)
+    long nDbg;               // debug records written by the last solve
+    long nonfinite;          // debug: first non-finite entry of the candidate (-1: none)
 };
 
 enum Method { METHOD_SPOCK = 0, METHOD_CP = 1 };
@@ -123,7 +134,9 @@ enum DevKinds { CON_NO_DEV = 0, CON_RECT_DEV = 1, CON_POLY_DEV = 2, COST_QUAD_DE
 // onto {c <= t}; restated as the reference computes it.
 template <typename T>
 __host__ __device__ __forceinline__ T projBelow(T c, T t) { return c >= t ? t : c; }
-enum Status { ST_NOTRUN = -1, ST_CONVERGED = 0, ST_OUTOFITER = 1, ST_OUTOFTIME = 2 };
+// ST_NONFINITE: debug mode found a non-finite candidate entry and stopped the solve, where the
+// reference's isFinite throws (src/cache.cuh:52-62, :912)
+enum Status { ST_NOTRUN = -1, ST_CONVERGED = 0, ST_OUTOFITER = 1, ST_OUTOFTIME = 2, ST_NONFINITE = 3 };
 
 constexpr int MAXR = 4;  // row chunks of 64 per wave task (rows <= 256)
 constexpr int MAXV = 4;  // vector chunks of 64 held in registers (length <= 256)

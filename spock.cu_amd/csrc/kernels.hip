@@ -878,8 +878,11 @@ __device__ __forceinline__ void applyT(const Inst<T> &P, const Tc &c, const T *z
 }
 
 // sum x.(y - gamma ell(y)) and max |y/gamma - ell(y)| with ell(y) = [L* y_d; L y_p].
-template <typename T, int R, int V>
-__device__ __noinline__ void reduceEll(const Inst<T> &P, const Tc &c, const T *x, const T *y, T *lds, T &dotOut, T &maxOut) {
+// SPLIT: also the maxima over the primal and the dual part apart, in split[0] and split[1]
+// (err1 and err2 of the debug trace).
+template <typename T, int R, int V, bool SPLIT = false>
+__device__ __noinline__ void reduceEll(const Inst<T> &P, const Tc &c, const T *x, const T *y, T *lds, T &dotOut, T &maxOut,
+                                       T *split = nullptr) {
     const int nx = P.nx, nu = P.nu;
     T dot = T(0), mx = T(0);
     auto acc = [&](long k, T ell) {
@@ -909,7 +912,13 @@ __device__ __noinline__ void reduceEll(const Inst<T> &P, const Tc &c, const T *x
     T v[1] = {dot + h.dot};
     blockSum(v, lds, c);
     dotOut = v[0];
-    maxOut = blockMax(fmax(mx, h.mx), lds, c);
+    if constexpr (SPLIT) {
+        split[0] = blockMax(mx, lds, c);
+        split[1] = blockMax(h.mx, lds, c);
+        maxOut = fmax(split[0], split[1]);
+    } else {
+        maxOut = blockMax(fmax(mx, h.mx), lds, c);
+    }
 }
 
 // ---------------------------------------------------------------- iterate-wide passes
@@ -1188,6 +1197,51 @@ __device__ __noinline__ void andersonApply(const Inst<T> &P, const Tc &c, const 
     sync(c);
 }
 
+// ---------------------------------------------------------------- debug trace
+// Index of the first non-finite entry of v[0, n), or -1.  The index travels through the block's
+// T-valued max reduction as n - k, exact while n < 2^24 in FP32 (2^53 in FP64).
+template <typename T>
+__device__ __noinline__ long firstNonfinite(const T *v, long n, T *lds, const Tc &c) {
+    T bad = T(0);
+    for (long k = c.tid; k < n; k += c.nt)
+        if (!isfinite(v[k])) bad = fmax(bad, T(n - k));
+    bad = blockMax(bad, lds, c);
+    return bad > T(0) ? n - (long) bad : -1;
+}
+
+// One record of the reference's debug caches at an error check (src/cache.cuh:947-959):
+//   err1, err2  the max-norms of the primal and dual parts of r/gamma - [L*; L] r   (split)
+//   err3        ||z - cand||_2 = ||r||_2
+//   err0        the reference computes e <- cand_prim - e on a vector e that starts at zero and
+//               is only zeroed by reset() (its L* product goes to another buffer, so it does not
+//               enter e), then ||e||_2; restated as computed, with e in P.dbgE
+//   callsL      calls to L and L* counted so far: two per application of T (src/cache.cuh:704-716,
+//               :750, :773); the host adds the count of earlier solves
+template <typename T>
+__device__ __noinline__ void debugRecord(const Inst<T> &P, const Tc &c, T *lds, long it, long callsL,
+                                         const T (&split)[2], const T *RR, const T *C, long &nDbg) {
+    T acc[2] = {T(0), T(0)};
+    for (long k = c.tid; k < P.sIter; k += c.nt) acc[0] += RR[k] * RR[k];
+    T *E = P.dbgE;
+    for (long k = c.tid; k < P.sPrim; k += c.nt) {
+        const T e = C[k] - E[k];
+        E[k] = e;
+        acc[1] += e * e;
+    }
+    blockSum(acc, lds, c);
+    if (nDbg < P.dbgCap && c.tid == 0) {
+        double *r = P.dbg + nDbg * DBG_WORDS;
+        r[0] = (double) it;
+        r[1] = (double) callsL;
+        r[2] = sqrt((double) acc[1]);
+        r[3] = (double) split[0];
+        r[4] = (double) split[1];
+        r[5] = sqrt((double) acc[0]);
+    }
+    nDbg++;
+    sync(c);
+}
+
 // ---------------------------------------------------------------- the solve
 // SuperMann with Anderson directions (src/cache.cuh:1008-1132) and runCp (:967-1002).
 // The six iterate buffers rotate by pointer instead of being copied: acceptCandidate,
@@ -1203,7 +1257,7 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
     T *D = P.dir, *DS = P.dirS;
     int head = m - 1;
     const long t0 = wall_clock64();
-    long nT = 0, nEll = 0;
+    long nT = 0, nEll = 0, nDbg = 0, nonfinite = -1;
     PhaseClock clk;
     clk.start();
     // initialisePrev / initialiseIter (src/cache.cuh:611-632)
@@ -1251,6 +1305,15 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
         applyT<T, R, V>(P, c, Z, C, &clk);
         nT++;
         // computeError (src/cache.cuh:910-961): r = z - cand
+        if (P.dbg) {  // isFinite(candidate) (src/cache.cuh:912): stop at the first non-finite entry
+            const long bad = firstNonfinite<T>(C, n, lds, c);
+            if (bad >= 0) {
+                status = ST_NONFINITE;
+                iters = it;
+                nonfinite = bad;
+                break;
+            }
+        }
         stream<8>(
             n, c, [&](long k) { return V2<T>{Z[k], C[k]}; }, [&](long k, const V2<T> &v) { RR[k] = v.a - v.b; });
         sync(c);
@@ -1259,7 +1322,13 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
         T dotRR = T(0);
         if (it % P.checkPeriod == 0) {
             T mx;
-            reduceEll<T, R, V>(P, c, RR, RR, lds, dotRR, mx);
+            if (P.dbg) {
+                T split[2];
+                reduceEll<T, R, V, true>(P, c, RR, RR, lds, dotRR, mx, split);
+                debugRecord<T>(P, c, lds, it, 2 * nT, split, RR, C, nDbg);
+            } else {
+                reduceEll<T, R, V>(P, c, RR, RR, lds, dotRR, mx);
+            }
             nEll++;
             haveDot = true;
             clk.mark(PH_ELL);
@@ -1413,6 +1482,8 @@ __device__ void solveInstance(const Inst<T> &P, Out &out, T *lds, double *ldsd, 
     }
     sync(c);
     if (c.tid == 0) {
+        out.nDbg = nDbg;
+        out.nonfinite = nonfinite;
         out.status = status;
         out.iters = iters;
         out.err = (double) lastErr;

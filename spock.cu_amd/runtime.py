@@ -31,7 +31,7 @@ EXPORTS = [
     "spk_batch_solve", "spk_batch_solve_noalloc", "spk_batch_resume", "spk_batch_results", "spk_batch_debug_op",
     "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
     "spk_batch_refill", "spk_batch_refill_from_batch", "spk_batch_refill_rows", "spk_batch_get_lanes_device",
-    "spk_batch_pool_open", "spk_batch_set_chunk",
+    "spk_batch_pool_open", "spk_batch_set_chunk", "spk_batch_set_debug", "spk_batch_debug_records",
     "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
     "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
     "spk_mlp_loss_backward", "spk_mlp_adam", "spk_mlp_dataset", "spk_mlp_train", "spk_mlp_predict",
@@ -104,6 +104,8 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_resume_graph.argtypes = [vp, C.c_int]
     lib.spk_batch_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
     lib.spk_batch_reset.argtypes = [vp]
+    lib.spk_batch_set_debug.argtypes = [vp, C.c_long]
+    lib.spk_batch_debug_records.argtypes = [vp, C.c_int, C.c_long, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
     lib.spk_batch_set_x0.argtypes = [vp, _dp]
     lib.spk_batch_set_tolerances.argtypes = [vp, _dp]
     lib.spk_batch_set_iterate.argtypes = [vp, C.c_int, _dp]
@@ -250,6 +252,21 @@ class Batch:
 
     def reset(self):
         self.ctx.check(self.ctx.lib.spk_batch_reset(self.h), "reset")
+
+    def set_debug(self, max_records):
+        """Debug traces of later solves (per-instance batches; 0: off): one record per error check."""
+        self.ctx.check(self.ctx.lib.spk_batch_set_debug(self.h, int(max_records)), "set debug")
+        self.debug_cap = int(max_records)
+
+    def debug_records(self, k=0):
+        """Instance k's records of the last solve, rows {idx, callsL, err0, err1, err2, err3}, and the
+        first non-finite candidate entry (-1: none; status SPK_NONFINITE = 3 when found)."""
+        cap = getattr(self, "debug_cap", 0)
+        rec = np.zeros((max(cap, 1), 6))
+        n, bad = C.c_long(), C.c_long()
+        self.ctx.check(self.ctx.lib.spk_batch_debug_records(self.h, k, cap, _d(rec), C.byref(n), C.byref(bad)),
+                       "debug records")
+        return rec[:min(n.value, cap)], bad.value
 
     def set_schedule(self, mv, block=0):
         """Engine and schedule of a shared batch: mv 17 (the default) or 16 the tile engine (MFMA, 16

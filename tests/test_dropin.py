@@ -83,6 +83,31 @@ int main(int argc, char **argv) {
 '''
 
 
+DEBUG = r'''
+#include <cstdio>
+#include "spock_amd.hpp"
+using namespace spock_amd;
+int main(int argc, char **argv) {
+    ScenarioTree<double> tree(argv[1]);
+    ProblemData<double> problem(tree);
+    CacheBuilder<double> builder(tree, problem);
+    Cache<double> cache = builder.tol(1e-30).enableDebug().build();  // maxIters defaults to 1000
+    std::vector<double> x0 = parseFromFile<double>(tree.path() + "initialState" + tree.fpFileExt());
+    const int status = cache.runSpock(x0);
+    std::printf("status %d\n", status);
+    std::vector<double> bad(x0);
+    bad[1] = 0.0 / 0.0;
+    try {
+        cache.runSpock(bad);
+        std::printf("no throw\n");
+    } catch (const std::invalid_argument &e) {
+        std::printf("threw %s", e.what());
+    }
+    return 0;
+}
+'''
+
+
 def _compile(src, out, tmp_path):
     path = tmp_path / "prog.cpp"
     path.write_text(src)
@@ -212,3 +237,103 @@ def test_anderson_buffer_driver_builds(tmp_path):
     exe = tmp_path / "anderson_buffer_main"
     _compile(src, exe, tmp_path)
     assert exe.exists()
+
+
+@needs_lib
+@pytest.mark.gpu
+def test_anderson_buffer_driver_runs(tmp_path):
+    """The Anderson-buffer study on a problem of the reference's random generator (seeded; its
+    first size interval, 1,000 to 10,000 variables): buffers 3, 5 and 7 all converge and the
+    time.csv line carries the sizes and three positive times."""
+    folder = str(tmp_path / "data")
+    configs.generate("random", folder, seed=1, lo=1000, hi=10000)
+    src = open(os.path.join(ROOT, "spock.cu_amd", "examples", "anderson_buffer_main.cpp")).read()
+    exe = tmp_path / "anderson_buffer_main"
+    _compile(src, exe, tmp_path)
+    run = subprocess.run([str(exe), folder + "/"], cwd=str(tmp_path), capture_output=True, text=True, timeout=170)
+    print(run.stdout)
+    assert run.returncode == 0 and "SPOCK failed" not in run.stdout
+    assert run.stdout.count("Run (1), buff (") == 3
+    row = [float(v) for v in (tmp_path / "time.csv").read_text().strip().split(",")]
+    from oracle.oracle import read_info
+    info = read_info(folder)
+    assert row[:3] == [info["numNodes"] * (info["numStates"] + info["numInputs"]), info["numNodes"],
+                       info["numStates"] + info["numInputs"]]
+    assert all(t > 0.0 for t in row[3:])
+
+
+@needs_lib
+@pytest.mark.gpu
+def test_server_air_condition_driver_runs(tmp_path):
+    """The server air-conditioning example (examples/serverAirCondition/main.cu and main.py,
+    horizon 5): three warm-up and two timed runs, each converged, and the average appended."""
+    folder = str(tmp_path / "data")
+    configs.generate("server", folder, horizon=5)
+    src = open(os.path.join(ROOT, "spock.cu_amd", "examples", "server_air_main.cpp")).read()
+    exe = tmp_path / "server_air_main"
+    _compile(src, exe, tmp_path)
+    run = subprocess.run([str(exe), folder + "/"], cwd=str(tmp_path), capture_output=True, text=True, timeout=170)
+    print(run.stdout)
+    assert run.returncode == 0 and "SPOCK failed" not in run.stdout
+    assert all(f"Run ({i}) : " in run.stdout for i in range(5))
+    assert float((tmp_path / "time.csv").read_text().strip()) > 0.0
+
+
+def test_server_air_condition_family(tmp_path):
+    """The generator restates the example's model: the oracle solves it at tol(1e-3) from the
+    example's initial state 0.1 * ones, and the solution keeps |x| <= 1 and |u| <= 1.5."""
+    from oracle.oracle import Oracle
+    from spock_amd.api import btio
+    folder = str(tmp_path / "server")
+    configs.generate("server", folder, horizon=4)
+    o = Oracle(folder)
+    x0 = btio.read_flat(os.path.join(folder, "initialState_d.bt"))
+    assert np.array_equal(x0, 0.1 * np.ones(50))
+    r = o.solve(x0, tol_abs=1e-3, tol_rel=1e-3, max_iters=2000)
+    assert r["status"] == 0
+    nx, nu = o.nx, o.info["numInputs"]
+    x = r["z"][o.sU:o.sU + o.sX] / np.tile(o.scaling[:nx], o.sX // nx)
+    u = r["z"][:o.sU] / np.tile(o.scaling[nx:nx + nu], o.sU // nu)
+    assert np.abs(x).max() <= 1.0 + 1e-2 and np.abs(u).max() <= 1.5 + 1e-2
+    assert o.info["numNodes"] == 31 and o.info["numEvents"] == 2
+
+
+@needs_lib
+@pytest.mark.gpu
+def test_enable_debug_writes_the_reference_cache_json(tmp_path):
+    """enableDebug (src/cache.cuh:1204-1209): maxIters 1,000 by default, the console summary with
+    the K counts, log/cacheSp.json with the reference's fields (src/cache.cuh:1274-1311) whose
+    callsL and err0..err3 are the oracle's debug caches, and isFinite's exception on a NaN state."""
+    import json
+    from oracle.oracle import Oracle
+    folder = str(tmp_path / "data")
+    x0 = _problem_with_initial_state(folder)
+    exe = tmp_path / "debug"
+    _compile(DEBUG, exe, tmp_path)
+    run = subprocess.run([str(exe), folder + "/"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    print(run.stdout, run.stderr)
+    assert run.returncode == 0
+    assert "status 1" in run.stdout and "Max iterations (1000) reached [K0: " in run.stdout
+    assert "threw [isFinite] DTensor has a non-finite entry at (" in run.stdout
+    doc = json.loads((tmp_path / "log" / "cacheSp.json").read_text())
+    assert doc["maxIters"] == doc["sizeCache"] == 1000
+    assert (doc["horizon"], doc["branchFactor"], doc["numEvents"], doc["numStates"], doc["numInputs"]) == \
+        (4, 5, 2, 3, 2)  # branching never stops: find() returns end(), numStages
+    # entries at the error checks (every 25 iterations; the check at iteration 1000 itself falls
+    # outside the caches), zero elsewhere
+    checks = np.arange(0, 1000, 25)
+    mask = np.zeros(1000, dtype=bool)
+    mask[checks] = True
+    for key in ("callsL", "err0", "err1", "err2", "err3"):
+        arr = np.array(doc[key], dtype=np.float64)
+        assert arr.size == 1000 and np.all(arr[~mask] == 0.0) and np.all(arr[mask] > 0.0), key
+    # values: the oracle's caches over the first 150 iterations (at tol 1e-30 the two arithmetics
+    # later take different SuperMann branches in the noise floor, and the traces part)
+    o = Oracle(folder)
+    rec = o.solve_debug(x0, np.zeros(o.n_prim), tol_abs=1e-30, max_iters=150)["records"]
+    idx = rec[:, 0].astype(int)
+    assert np.array_equal(np.array(doc["callsL"], dtype=np.float64)[idx], rec[:, 1])
+    for col, key in ((2, "err0"), (3, "err1"), (4, "err2"), (5, "err3")):
+        v = np.array(doc[key], dtype=np.float64)[idx]
+        assert np.max(np.abs(v - rec[:, col])) <= 1e-8 * np.max(np.abs(rec[:, col])), key
+    assert len(doc["states"]) == o.sX and len(doc["inputs"]) == o.sU

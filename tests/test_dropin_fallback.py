@@ -62,6 +62,8 @@ int spk_batch_results(spk_batch *, spk_result *r, double *) { std::memset(r, 0, 
 int spk_batch_get_iterate(spk_batch *, int, double *) { return 0; }
 int spk_batch_reset(spk_batch *) { return 0; }
 int spk_batch_debug_op(spk_batch *, int, const double *, double *, double *) { return 0; }
+int spk_batch_set_debug(spk_batch *, long) { return 0; }
+int spk_batch_debug_records(spk_batch *, int, long, double *, long *n, long *bad) { *n = 0; *bad = -1; return 0; }
 void spk_default_options(spk_options *o) { std::memset(o, 0, sizeof(*o)); }
 }
 '''

@@ -1,0 +1,129 @@
+"""The uneven-batch workload at full size: 512 independent RAOCPs drawn from the seed (instance k
+from seed 4000 + k: horizon U{5..12}, 2-4 events, branching stops at stage 1-3, nx 10-40,
+nu 4-16, AV@R alpha U[0.8, 0.99], box constraints; x0 ~ N(0, 0.5^2 I)), every instance a
+differently shaped tree, solved in one persistent launch with one workgroup per instance.
+
+* 50 SPOCK iterations in FP64 match the CPU oracle for every one of the 512 instances: the same
+  SuperMann branch counts and iterates to 1e-9 relative.
+* Plain FP64 SPOCK under tol(1e-3), capped at 2,000 iterations; then FP32 bulk and an FP64
+  polish warm-started from it, against the threshold each plain FP64 solve set itself: every
+  instance that plain FP64 SPOCK solves within its cap is accepted after the polish.
+
+The problem folders and the oracle runs are spread over host processes (the GPU box's CPU share)."""
+import multiprocessing as mp
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from spock_amd import runtime as rt
+
+pytestmark = pytest.mark.gpu
+
+COUNT = 512
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _make(args):
+    folder, k = args
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    ge._load_pkg()
+    from spock_amd.api import configs as cf
+    from oracle.oracle import Oracle
+    cf.uneven_instance(folder, k)
+    o = Oracle(folder)
+    x0 = cf.initial_states(4000 + k, o.nx, 1)[0]
+    ref = o.solve(x0, tol_abs=1e-30, max_iters=50)
+    return x0, ref["z"], tuple(ref[key] for key in ("iters", "k0", "k1", "k2", "k2bt", "k3"))
+
+
+@pytest.fixture(scope="module")
+def family(tmp_path_factory):
+    base = tmp_path_factory.mktemp("cfg4")
+    jobs = [(str(base / f"i{k}"), k) for k in range(COUNT)]
+    with mp.get_context("spawn").Pool(min(16, os.cpu_count() or 1)) as pool:
+        out = pool.map(_make, jobs, chunksize=8)
+    return [j[0] for j in jobs], [o[0] for o in out], [o[1] for o in out], [o[2] for o in out]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return rt.Context(0)
+
+
+def test_all_512_instances_match_the_oracle(ctx, family):
+    dirs, x0s, zs, counts = family
+    probs = [rt.Problem(ctx, d) for d in dirs]
+    b = rt.Batch(ctx, probs, rt.F64)
+    b.set_x0(x0s)
+    b.solve(rt.options(tol_abs=1e-30, max_iters=50))
+    res, sec = b.results()
+    z = b.iterates()
+    pos, worst = 0, 0.0
+    for k in range(COUNT):
+        r = res[k]
+        assert (r["iters"], r["k0"], r["k1"], r["k2"], r["k2bt"], r["k3"]) == counts[k], k
+        n = zs[k].size
+        e = rel_err(z[pos:pos + n], zs[k])
+        worst = max(worst, e)
+        assert e < 1e-9, (k, e)
+        pos += n
+    print(f"512 uneven instances, 50 iterations in one launch: {sec:.3f} s, worst iterate error {worst:.2e}")
+
+
+OPTS = dict(tol_abs=1e-3, tol_rel=1e-3, max_iters=2000)
+_plain = {}
+
+
+def test_all_512_plain_fp64(ctx, family):
+    """Plain FP64 SPOCK on every instance, one workgroup each (kept for the next test)."""
+    dirs, x0s, _, _ = family
+    probs = [rt.Problem(ctx, d) for d in dirs]
+    plain = rt.Batch(ctx, probs, rt.F64)
+    plain.set_x0(x0s)
+    plain.solve(rt.options(**OPTS))
+    res_p, sec_p = plain.results()
+    ok = [k for k in range(COUNT) if res_p[k]["status"] == 0]
+    print(f"plain FP64: {len(ok)}/{COUNT} converged within {OPTS['max_iters']} iterations in {sec_p:.1f} s",
+          flush=True)
+    _plain.update(probs=probs, res=res_p, z=plain.iterates(), ok=ok)
+    assert len(ok) >= COUNT // 4
+
+
+def test_all_512_fp32_bulk_and_fp64_polish(ctx, family):
+    """FP32 bulk, then the FP64 polish, on the instances plain FP64 solved within its cap."""
+    dirs, x0s, _, _ = family
+    if not _plain:
+        pytest.skip("needs the plain FP64 solves of the test before")
+    probs, res_p, zp, ok = _plain["probs"], _plain["res"], _plain["z"], _plain["ok"]
+    sub = [probs[k] for k in ok]
+    xs = [x0s[k] for k in ok]
+    b32 = rt.Batch(ctx, sub, rt.F32)
+    b32.set_x0(xs)
+    b32.solve(rt.options(**OPTS))
+    res32, sec32 = b32.results()
+    print(f"FP32 bulk: {sec32:.1f} s", flush=True)
+    b64 = rt.Batch(ctx, sub, rt.F64)
+    b64.set_x0(xs)
+    b64.copy_iterates_from(b32)
+    b64.set_tolerances([res_p[k]["tol"] for k in ok])
+    b64.solve(rt.options(**OPTS), warm=True)
+    res64, sec64 = b64.results()
+    fp32_ok = sum(1 for r in res32 if r["status"] == 0)
+    polished = sum(1 for r in res64 if r["status"] == 0)
+    offs = np.concatenate([[0], np.cumsum([p.n_iter for p in probs])])
+    offs_sub = np.concatenate([[0], np.cumsum([p.n_iter for p in sub])])
+    z64 = b64.iterates()
+    gaps = []
+    for i, k in enumerate(ok):
+        s_full = offs[k] + probs[k].n_prim - probs[k].N  # s_0: the first entry of the s block
+        s_sub = offs_sub[i] + probs[k].n_prim - probs[k].N
+        gaps.append(abs(z64[s_sub] - zp[s_full]) / max(1.0, abs(zp[s_full])))
+    print(f"plain FP64: {len(ok)}/{COUNT} converged; FP32: {fp32_ok} of them in {sec32:.1f} s; "
+          f"FP64 polish: {polished} accepted in {sec64:.1f} s, {np.mean([r['iters'] for r in res64]):.0f} "
+          f"iterations on average; s0 gap to plain: median {np.median(gaps):.2e}, max {np.max(gaps):.2e}")
+    assert polished == len(ok)
