@@ -1,0 +1,117 @@
+"""The learned-warm-start workload on the 229-node family (3 events, horizon 10, branching stops at
+stage 3, nx 50, nu 20) with 4,096 held-out x0 ~ N(0, 0.5^2 I) as one batch.
+
+The network (50 -> 3 x 512 GELU -> 52,360, bf16 weights, FP32 accumulation) is trained by the
+project's own kernels on cold FP32 solves of training x0 (the plain solver's solutions, accepted
+at tol(1e-3)).  Every held-out x0 is then solved twice in FP32, warm-started and cold, for the
+same number of iterations, against the threshold the plain method sets itself for that x0.
+
+* What the network gains: more held-out solves are accepted warm than cold, and sooner.
+* Fallback: warm solves not accepted within the cap fall back to cold solves; their count is
+  reported.
+* Cost error: a sample of accepted warm solutions, polished in FP64, against the CPU oracle's
+  cold FP64 solve of the same x0 at the same threshold: the relative gap of s0 is reported.  At
+  this residual two accepted solutions can differ by several percent, so the test bounds the gap
+  loosely (it checks that the solutions are of the same problem, not the 1e-3 cost target)."""
+import multiprocessing as mp
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import conftest  # noqa: F401  registers the package, also in the spawned workers that import this module
+from spock_amd import runtime as rt
+from spock_amd.api import configs
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(400)]
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HELD_OUT, TRAIN, ITERS = 4096, 1024, 1000
+
+
+def _oracle_s0(args):
+    folder, x0, tol = args
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    ge._load_pkg()
+    from oracle.oracle import Oracle
+    o = Oracle(folder)
+    r = o.solve(x0, tol_abs=tol, tol_rel=tol, max_iters=12000)
+    return r["status"], float(r["z"][o.n_prim - o.info["numNodes"]]), r["iters"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return rt.Context(0)
+
+
+def test_learned_warm_start_on_4096_held_out_states(ctx, tmp_path):
+    d = str(tmp_path / "mid")
+    configs.generate("mid", d)
+    p = rt.Problem(ctx, d)
+    tol = 1e-3
+    # training: cold FP32 solves of training x0, the converged ones kept
+    xt = configs.initial_states(50, p.nx, TRAIN, "normal")
+    bt = rt.Batch(ctx, [p] * TRAIN, rt.F32, shared=True)
+    bt.set_x0(list(xt))
+    bt.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=2500))
+    res, _ = bt.results()
+    ok = np.flatnonzero([r["status"] == 0 for r in res])
+    print(f"training solves: {ok.size}/{TRAIN} converged", flush=True)
+    assert ok.size >= TRAIN // 8
+    mlp = rt.WarmStartMLP(ctx, p.nx, p.n_iter, hidden=512, layers=3, seed=11)
+    mlp.dataset_lanes(bt, ok, xt[ok])
+    losses = mlp.train(epochs=30, minibatch=256, lr=1e-3, seed=3)
+    print(f"MLP loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+    assert losses[-1] < losses[0]
+    del bt
+    # held-out: the plain method's own threshold for each x0 (one FP64 iteration from cold)
+    xh = configs.initial_states(51, p.nx, HELD_OUT, "normal")
+    th = rt.Batch(ctx, [p] * HELD_OUT, rt.F64, shared=True)
+    th.set_x0(list(xh))
+    th.solve(rt.options(tol_abs=tol, tol_rel=tol, max_iters=0, chunk_iters=1))
+    taus = np.array([r["tol"] for r in th.results()[0]])
+    del th
+    out = {}
+    for name in ("warm", "cold"):
+        b = rt.Batch(ctx, [p] * HELD_OUT, rt.F32, shared=True)
+        b.set_x0(list(xh))
+        b.set_tolerances(taus)
+        if name == "warm":
+            mlp.predict(xh, b)
+        b.solve(rt.options(tol_abs=0.0, tol_rel=0.0, max_iters=ITERS), warm=name == "warm")
+        r, sec = b.results()
+        print(f"{name}: {sec:.1f} s", flush=True)
+        st = np.array([v["status"] for v in r])
+        its = np.array([v["iters"] for v in r])
+        out[name] = (st, its, sec, b)
+    (sw, iw, tw, bw), (sc, ic, tc, _) = out["warm"], out["cold"]
+    nw, nc = int((sw == 0).sum()), int((sc == 0).sum())
+    print(f"held-out {HELD_OUT}: accepted within {ITERS} FP32 iterations warm {nw} (mean "
+          f"{iw[sw == 0].mean():.0f} iterations), cold {nc} (mean {ic[sc == 0].mean() if nc else 0:.0f}); "
+          f"fallback (warm not accepted) {HELD_OUT - nw}; kernel {tw:.1f} s warm, {tc:.1f} s cold")
+    assert nw > nc
+    assert nw >= HELD_OUT // 4
+    # cost error: FP64 polish of sampled accepted warm solutions against the CPU oracle's cold solve
+    sample = np.flatnonzero(sw == 0)[:4]
+    b64 = rt.Batch(ctx, [p] * sample.size, rt.F64, shared=True)
+    b64.set_x0(list(xh[sample]))
+    b64.set_iterates(np.concatenate([bw.iterate(int(k)) for k in sample]))
+    b64.set_tolerances(taus[sample])
+    b64.solve(rt.options(tol_abs=0.0, tol_rel=0.0, max_iters=5000), warm=True)
+    r64, _ = b64.results()
+    assert all(v["status"] == 0 for v in r64)
+    s_idx = p.n_prim - p.N
+    with mp.get_context("spawn").Pool(sample.size) as pool:
+        ref = pool.map(_oracle_s0, [(d, xh[k], tol) for k in sample])
+    gaps = []
+    for i, (status, s0_ref, its) in enumerate(ref):
+        if status != 0:  # the oracle's cold solve did not end within its cap: not compared
+            continue
+        s0 = b64.iterate(i)[s_idx]
+        gaps.append(abs(s0 - s0_ref) / max(1.0, abs(s0_ref)))
+    print(f"cost gap of s0 against the CPU oracle (FP64, cold, same threshold): {['%.2e' % g for g in gaps]}")
+    assert len(gaps) >= 2
+    assert max(gaps) < 0.5
