@@ -323,6 +323,32 @@ def train_warm_start(rt, ctx, prob, configs, args, schedule):
     return mlp, int(ok.size), float(losses[-1])
 
 
+def mlp_only(rt, ctx, prob, x0s, epochs):
+    """The MLP's training kernels at the bench's size: GEMM flops per minibatch step of 256 rows
+    (forward, weight gradients, and the input gradients of every layer but the first) and the
+    measured time per step."""
+    b = rt.Batch(ctx, [prob] * len(x0s), rt.F32, shared=True)
+    b.set_schedule(17, 1024)
+    b.set_x0(list(x0s))
+    b.solve(rt.options(tol_abs=0.0, tol_rel=0.0, max_iters=25))
+    b.results()
+    mlp = rt.WarmStartMLP(ctx, prob.nx, prob.n_iter, hidden=512, layers=3, seed=11)
+    mlp.dataset_lanes(b, np.arange(len(x0s)), x0s)
+    mlp.train(epochs=1, minibatch=256, lr=1e-3, seed=3)  # warm-up
+    t = time.perf_counter()
+    mlp.train(epochs=epochs, minibatch=256, lr=1e-3, seed=4)
+    sec = time.perf_counter() - t
+    dims = [prob.nx, 512, 512, 512, prob.n_iter]
+    mm = sum(dims[l] * dims[l + 1] for l in range(len(dims) - 1))
+    back_in = sum(dims[l] * dims[l + 1] for l in range(1, len(dims) - 1))
+    flops = 2 * 256 * (2 * mm + back_in)
+    steps = epochs * (len(x0s) // 256)
+    log(f"MLP: {steps} minibatch steps in {sec:.3f} s ({1e3 * sec / steps:.3f} ms each, "
+        f"{flops / 1e9:.1f} GFLOP each, {flops * steps / sec / 1e12:.1f} TFLOP/s overall)")
+    return {"steps": steps, "seconds": sec, "ms_per_step": 1e3 * sec / steps, "gflop_per_step": flops / 1e9,
+            "output_layer_gemm_gflop": 2 * 256 * 512 * prob.n_iter / 1e9}
+
+
 def kernel_roofline(rt, ctx, prob, x0s, schedule, iters):
     """One launch of the dominant kernel (the FP32 tile engine) over the held-out batch from a cold
     start, a fixed number of iterations: algorithmic bytes / kernel time."""
@@ -413,6 +439,10 @@ def main():
                     help="only the dominant kernel: one launch of 50 cold iterations of --slots FP32 instances "
                          "(schedule 17, 1024 threads), then a 1 GiB device copy whose counters calibrate "
                          "FETCH_SIZE / WRITE_SIZE; prints one JSON line (for rocprofv3) and exits")
+    ap.add_argument("--mlp-only", type=int, default=0, metavar="EPOCHS",
+                    help="only the MLP's training kernels: the 50 -> 3 x 512 -> 52,360 network trained for EPOCHS "
+                         "epochs (minibatch 256) on the iterates of 4,096 held-out x0 after 25 FP32 iterations; "
+                         "prints one JSON line with the time per minibatch step and its GEMM flops, and exits")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
@@ -434,6 +464,9 @@ def main():
 
         if args.absolute_study > 0:
             absolute_study(rt, ctx, prob, configs, args, stream, tol_abs_scaled)
+            return
+        if args.mlp_only > 0:
+            print(json.dumps({"mlp_training": mlp_only(rt, ctx, prob, stream[:4096], args.mlp_only)}), flush=True)
             return
 
         cpu = None

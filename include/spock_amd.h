@@ -253,6 +253,20 @@ int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int l
 /* continue a chunked solve of a shared batch with `launches` resume launches captured as one graph
  * (as spk_batch_resume, without host round trips between the launches). */
 int spk_batch_resume_graph(spk_batch *b, int launches);
+/* per-instance (uneven) batches: time `iters` cold outer iterations under each of
+ * SPK_AUTOTUNE_UNEVEN_CANDIDATES schedules of the persistent launch -- workgroups of 128, 256 and
+ * 512 threads, with the instances in batch order and then largest first (descending iterate size,
+ * so the longest solves start in the first dispatch wave) -- keep the fastest for later solves
+ * that leave spk_options.block_size at 0, and return it (block, largest_first, seconds: nullable;
+ * seconds gets the kernel times in that order).  The order never changes an instance's result;
+ * the workgroup size fixes the order of its sums.  The iterates are reset (initial states and given
+ * tolerances are kept).  No counterpart in the reference, which solves one instance per Cache
+ * (src/cache.cuh:1008-1132). */
+#define SPK_AUTOTUNE_UNEVEN_CANDIDATES 6
+int spk_batch_autotune_uneven(spk_batch *b, const spk_options *opts, int iters, int *block, int *largest_first,
+                              double *seconds);
+/* set that schedule directly (block 0: 256 threads). */
+int spk_batch_set_uneven_schedule(spk_batch *b, int block, int largest_first);
 /* per-instance batches of ONE instance: time `iters` cold outer iterations with 1, 8, 32, 64 and
  * 128 cooperating workgroups (grid mode above 1), keep the fastest for later solves and return it
  * (workgroups, seconds: nullable; seconds gets the 5 kernel times in that order).  The iterates are

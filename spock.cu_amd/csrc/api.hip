@@ -144,6 +144,10 @@ struct spk_batch {
     long dbgCap = 0;
     DevBuf dbgRec, dbgE;
     std::vector<size_t> dbgEOff;
+    // per-instance batches: workgroup size and workgroup order chosen by spk_batch_autotune_uneven
+    // (0 / false: 256 threads, instances in batch order)
+    int fusedBlock = 0;
+    bool largestFirst = false;
     size_t elSize() const { return dtype == SPK_F64 ? 8 : 4; }
 };
 
@@ -297,6 +301,7 @@ void fillInsts(spk_batch *b, const spk_options &o, int warm, T *anders, const st
         I.dynInverse = o.dyn_inverse;
         I.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
         I.gpart = gpart;
+        I.slot = k;
         if (b->dbgCap > 0) {
             I.dbg = static_cast<double *>(b->dbgRec.p) + (size_t) k * b->dbgCap * spk::DBG_WORDS;
             I.dbgCap = b->dbgCap;
@@ -305,10 +310,23 @@ void fillInsts(spk_batch *b, const spk_options &o, int warm, T *anders, const st
     }
 }
 
+// Workgroup w solves the instance of descriptor w.  Largest first: the descriptors in order of
+// decreasing iterate size (ties in batch order), so the longest solves start in the first dispatch
+// wave when a batch holds more instances than the device runs at once.
+template <typename T>
+void orderLargestFirst(spk_batch *b) {
+    auto *arr = reinterpret_cast<spk::Inst<T> *>(b->hInst.data());
+    std::vector<spk::Inst<T>> tmp(arr, arr + b->count);
+    std::stable_sort(tmp.begin(), tmp.end(),
+                     [](const spk::Inst<T> &x, const spk::Inst<T> &y) { return x.sIter > y.sIter; });
+    std::copy(tmp.begin(), tmp.end(), arr);
+}
+
 template <typename T>
 void buildInsts(spk_batch *b, const spk_options &o, int warm) {
     if (b->count == 1 && !b->gpart.p) b->gpart.alloc((size_t) 2 * spk::GRID_MAXK * spk::GRID_MAXWG * sizeof(T));
     fillInsts<T>(b, o, warm, static_cast<T *>(b->anders.p), b->andOff, static_cast<T *>(b->gpart.p));
+    if (b->largestFirst && b->count > 1) orderLargestFirst<T>(b);
     b->dInst.alloc(b->hInst.size());
     check(hipMemcpyAsync(b->dInst.p, b->hInst.data(), b->hInst.size(), hipMemcpyHostToDevice, b->ctx->stream),
           "upload instances");
@@ -437,15 +455,15 @@ void launchKernel(spk_batch *b, int bs, bool debug, int op, const void *insDev =
     }
 }
 
-int solveBlock(const spk_options &o) {
-    const int bs = o.block_size > 0 ? o.block_size : 256;
+int solveBlock(const spk_options &o, const spk_batch *b = nullptr) {
+    const int bs = o.block_size > 0 ? o.block_size : (b && b->fusedBlock > 0 ? b->fusedBlock : 256);
     return std::max(64, std::min(512, bs / 64 * 64));
 }
 
 template <typename T>
 void launchSolve(spk_batch *b, const spk_options &o, int warm) {
     buildInsts<T>(b, o, warm);
-    const int bs = solveBlock(o);
+    const int bs = solveBlock(o, b);
     b->dOut.alloc(sizeof(spk::Out) * b->count);
     check(hipEventRecord(b->ev0, b->ctx->stream), "event");
     launchKernel<T>(b, bs, false, 0);
@@ -510,7 +528,7 @@ void launchSolveNoAlloc(spk_batch *b, const spk_options &o, int warm) {
     check(hipMemcpyAsync(base, b->hInst.data(), b->hInst.size(), hipMemcpyHostToDevice, b->ctx->stream),
           "upload instances");
     check(hipEventRecord(b->ev0, b->ctx->stream), "event");
-    launchKernel<T>(b, solveBlock(o), false, 0, base);
+    launchKernel<T>(b, solveBlock(o, b), false, 0, base);
     check(hipGetLastError(), "launch k_solve");
     check(hipEventRecord(b->ev1, b->ctx->stream), "event");
     b->pending = true;
@@ -1710,6 +1728,69 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
         if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
         if (mv) *mv = b->laneMv;
         if (block) *block = b->laneBlockSz;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_autotune_uneven(spk_batch *b, const spk_options *opts, int iters, int *block, int *largest_first,
+                              double *seconds) {
+    try {
+        if (b->lanes) throw std::runtime_error("the uneven-batch autotuner tunes per-instance batches");
+        if (iters < 1) throw std::runtime_error("iters must be >= 1");
+        spk_options o;
+        if (opts) o = *opts;
+        else spk_default_options(&o);
+        o.tol_abs = o.tol_rel = 0.;  // a fixed number of iterations
+        o.max_iters = iters;
+        o.max_time_secs = 0.;
+        o.block_size = 0;
+        TuneGuard guard(b);
+        const int fb = b->fusedBlock;
+        const bool lf = b->largestFirst;
+        const int cand[SPK_AUTOTUNE_UNEVEN_CANDIDATES][2] = {{128, 0}, {256, 0}, {512, 0}, {128, 1}, {256, 1}, {512, 1}};
+        double best = 1e300;
+        int bi = 1;
+        try {
+            for (int i = 0; i < SPK_AUTOTUNE_UNEVEN_CANDIDATES; i++) {
+                b->fusedBlock = cand[i][0];
+                b->largestFirst = cand[i][1] != 0;
+                if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
+                if (spk_batch_solve(b, &o, 0)) throw std::runtime_error(b->ctx->err);
+                double sec = 0.;
+                if (spk_batch_results(b, nullptr, &sec)) throw std::runtime_error(b->ctx->err);
+                if (seconds) seconds[i] = sec;
+                if (sec < best) {
+                    best = sec;
+                    bi = i;
+                }
+            }
+        } catch (...) {
+            b->fusedBlock = fb;
+            b->largestFirst = lf;
+            throw;
+        }
+        b->fusedBlock = cand[bi][0];
+        b->largestFirst = cand[bi][1] != 0;
+        guard.keep();
+        if (spk_batch_reset(b)) throw std::runtime_error(b->ctx->err);
+        if (block) *block = b->fusedBlock;
+        if (largest_first) *largest_first = b->largestFirst ? 1 : 0;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_set_uneven_schedule(spk_batch *b, int block, int largest_first) {
+    try {
+        if (b->lanes) throw std::runtime_error("uneven-batch schedules apply to per-instance batches");
+        if (block != 0 && (block < 64 || block > 512 || block % 64))
+            throw std::runtime_error("block must be 0 or a multiple of 64 in 64..512");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        b->fusedBlock = block;
+        b->largestFirst = largest_first != 0;
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);

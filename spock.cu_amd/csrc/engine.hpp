@@ -43,6 +43,7 @@ struct Inst {
     double *dbg;
     long dbgCap;
     T *dbgE;
+    int slot;  // this instance's index in the batch (its result record); descriptors may be reordered
 };
 
 // A debug record: outer iteration, calls to L and L* so far, err0, err1, err2, err3.

@@ -1570,7 +1570,7 @@ __global__ void __launch_bounds__(SPK_MAX_THREADS) k_solve(const Inst<T> *__rest
     const int k = grid ? 0 : (int) blockIdx.x;
     if (k >= count) return;
     const Inst<T> &P = stageInstance(insts[k], local, pool, poolInts);
-    solveInstance<T, R, V>(P, outs[k], lds, ldsd, tickRate, grid);
+    solveInstance<T, R, V>(P, outs[P.slot], lds, ldsd, tickRate, grid);
 }
 
 // ---------------------------------------------------------------- single-operator kernels

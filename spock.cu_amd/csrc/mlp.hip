@@ -122,6 +122,168 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
             }
 }
 
+// ---------------------------------------------------------------- CDNA4 GEMM
+// C (M x N) = A (M x K) . B (K x N) on v_mfma_f32_16x16x32_bf16 (gfx950): 256 threads, a 128 x 128
+// tile per workgroup, each wave a 64 x 64 block of 4 x 4 MFMA tiles (64 accumulator registers).
+// Lane layout of the instruction: A[i][k] from lane i + 16 (k / 8) item k % 8, B[k][j] from lane
+// j + 16 (k / 8) item k % 8, D[i][j] in lane j + 16 (i / 4) item i % 4.
+// Both operands sit in LDS k-contiguous ([row][k] and [col][k], rows padded by 8 elements), so a
+// lane's fragment is one 128-bit LDS read.  Global -> LDS staging moves 128-bit vectors along
+// whichever dimension is contiguous in memory: AK / BK say the operand is contiguous along k
+// (vectors go to LDS as they are), otherwise along its row / column (a vector of 8 rows at one k
+// is scattered into 8 LDS rows).  LDS is double buffered: the next k-step's vectors are loaded
+// into registers while the matrix cores work on the current one.  With split-K (gridDim.z > 1)
+// each z-slice writes its partial product to a workspace, summed by k_splitk_reduce.
+constexpr int G2_BM = 128, G2_BN = 128, G2_BK = 32, G2_PAD = 8, G2_LD = G2_BK + G2_PAD;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+struct G2Stage {
+    u32x4 a[2], b[2];
+};
+
+// One operand's 128 x 32 k-step into registers: two 8-element vectors per thread.  kc:
+// contiguous along k; ld: the stride of the other dimension; rowsTotal / K: bounds.
+template <bool KC>
+__device__ __forceinline__ void g2Load(const unsigned short *P, long sRow, long sK, int row0, int rowsTotal, int k0,
+                                       int K, int tid, u32x4 (&out)[2]) {
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+        int r, kk;
+        if (KC) {  // 128 rows x 4 vectors of 8 k
+            r = (tid >> 2) + 64 * v;
+            kk = 8 * (tid & 3);
+        } else {   // 32 k x 16 vectors of 8 rows
+            kk = (tid >> 4) + 16 * v;
+            r = 8 * (tid & 15);
+        }
+        const int gr = row0 + r, gk = k0 + kk;
+        const bool full = KC ? (gr < rowsTotal && gk + 7 < K) : (gr + 7 < rowsTotal && gk < K);
+        if (full) {
+            out[v] = *reinterpret_cast<const u32x4 *>(P + (long) gr * sRow + (long) gk * sK);
+        } else {
+            unsigned short e[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int qr = KC ? gr : gr + q, qk = KC ? gk + q : gk;
+                e[q] = (qr < rowsTotal && qk < K) ? P[(long) qr * sRow + (long) qk * sK] : (unsigned short) 0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) out[v][q] = (unsigned int) e[2 * q] | ((unsigned int) e[2 * q + 1] << 16);
+        }
+    }
+}
+
+template <bool KC>
+__device__ __forceinline__ void g2Store(unsigned short *S, int tid, const u32x4 (&in)[2]) {
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+        if (KC) {
+            const int r = (tid >> 2) + 64 * v, kk = 8 * (tid & 3);
+            *reinterpret_cast<u32x4 *>(S + r * G2_LD + kk) = in[v];
+        } else {
+            const int kk = (tid >> 4) + 16 * v, r = 8 * (tid & 15);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                S[(r + 2 * q) * G2_LD + kk] = (unsigned short) (in[v][q] & 0xffffu);
+                S[(r + 2 * q + 1) * G2_LD + kk] = (unsigned short) (in[v][q] >> 16);
+            }
+        }
+    }
+}
+
+template <bool AK, bool BK>
+__global__ void __launch_bounds__(256) k_gemm2(GemmArgs g, float *partial) {
+    __shared__ __attribute__((aligned(16))) unsigned short As[2][G2_BM * G2_LD];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[2][G2_BN * G2_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i0 = blockIdx.y * G2_BM, j0 = blockIdx.x * G2_BN;
+    const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
+    // this z-slice's k range, a multiple of the k-step
+    const int kPer = ((g.K + gridDim.z - 1) / gridDim.z + G2_BK - 1) / G2_BK * G2_BK;
+    const int kBeg = blockIdx.z * kPer, kEnd = min(g.K, kBeg + kPer);
+    f32x4v acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = f32x4v{0.f, 0.f, 0.f, 0.f};
+    // A as rows i (stride sai) by k (stride sak); B as columns j (stride sbj) by k (stride sbk)
+    G2Stage st;
+    const int nk = kEnd > kBeg ? (kEnd - kBeg + G2_BK - 1) / G2_BK : 0;
+    if (nk > 0) {
+        g2Load<AK>(g.A, g.sai, g.sak, i0, g.M, kBeg, kEnd, tid, st.a);
+        g2Load<BK>(g.B, g.sbj, g.sbk, j0, g.N, kBeg, kEnd, tid, st.b);
+        g2Store<AK>(As[0], tid, st.a);
+        g2Store<BK>(Bs[0], tid, st.b);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) {
+            const int kn = kBeg + (kt + 1) * G2_BK;
+            g2Load<AK>(g.A, g.sai, g.sak, i0, g.M, kn, kEnd, tid, st.a);
+            g2Load<BK>(g.B, g.sbj, g.sbk, j0, g.N, kn, kEnd, tid, st.b);
+        }
+        bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            af[a] = *reinterpret_cast<const bf16x8 *>(&As[cur][(wi + 16 * a + (lane & 15)) * G2_LD + 8 * (lane >> 4)]);
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            bfr[b] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][(wj + 16 * b + (lane & 15)) * G2_LD + 8 * (lane >> 4)]);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        if (kt + 1 < nk) {
+            g2Store<AK>(As[cur ^ 1], tid, st.a);
+            g2Store<BK>(Bs[cur ^ 1], tid, st.b);
+        }
+        __syncthreads();
+    }
+    float *part = gridDim.z > 1 ? partial + (long) blockIdx.z * g.M * g.N : nullptr;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int row = i0 + wi + 16 * a + 4 * (lane >> 4) + t;
+                const int col = j0 + wj + 16 * b + (lane & 15);
+                if (row >= g.M || col >= g.N) continue;
+                float v = acc[a][b][t];
+                if (part) {
+                    part[(long) row * g.N + col] = v;
+                    continue;
+                }
+                const long o = (long) row * g.ldc + col;
+                if (g.epi == EPI_STORE) {
+                    g.C[o] = v;
+                } else if (g.epi == EPI_BIAS) {
+                    g.C[o] = v + g.bias[col];
+                } else if (g.epi == EPI_BIAS_GELU) {
+                    v += g.bias[col];
+                    g.C[o] = v;
+                    g.Cb[o] = toBf16(gelu(v));
+                } else {
+                    g.C[o] = (v + g.bias[col]) * g.scale[col] + g.shift[col];
+                }
+            }
+}
+
+// C = sum over the split-K slices of the partial products (EPI_STORE), in slice order.
+__global__ void k_splitk_reduce(const float *partial, int slices, int M, int N, float *C, long ldc) {
+    const long n = (long) M * N;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < n; e += (long) gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int z = 0; z < slices; z++) s += partial[(long) z * n + e];
+        C[(e / N) * ldc + e % N] = s;
+    }
+}
+
 // dZ = dA . gelu'(Z) (FP32) and its bf16 copy.
 __global__ void k_gelu_back(const float *dA, const float *Z, float *dZ, unsigned short *dZb, long n) {
     for (long t = blockIdx.x * (long) blockDim.x + threadIdx.x; t < n; t += (long) gridDim.x * blockDim.x) {
@@ -262,6 +424,7 @@ struct spk_mlp {
     std::vector<float> hInMean, hInStd;
     int cap = 0;                    // rows the workspace holds
     DevBuf X, Y, dY, dYb, dA, dZ, dZb, Tb, idx, partial, Xf;
+    DevBuf splitWs;                 // split-K partial products of the GEMM
     std::vector<std::unique_ptr<DevBuf>> Z, A;
     long nTrain = 0;
     DevBuf trX, trT;
@@ -274,6 +437,15 @@ namespace {
 
 dim3 grid1(long n) { return dim3((unsigned) std::min<long>(4096, (n + 255) / 256)); }
 
+// 128-bit vectors along an operand's contiguous dimension need that dimension's stride to be 1,
+// the other stride a multiple of 8 elements and a 16-byte aligned base.
+bool vectorizable(const unsigned short *P, long sRow, long sK) {
+    if (((uintptr_t) P & 15) != 0) return false;
+    if (sK == 1) return sRow % 8 == 0;
+    if (sRow == 1) return sK % 8 == 0;
+    return false;
+}
+
 void gemm(spk_mlp *m, const unsigned short *A, long sai, long sak, const unsigned short *B, long sbk, long sbj,
           int Mr, int Nc, int K, float *C, long ldc, unsigned short *Cb, const float *bias, int epi,
           const float *scale = nullptr, const float *shift = nullptr) {
@@ -281,8 +453,35 @@ void gemm(spk_mlp *m, const unsigned short *A, long sai, long sak, const unsigne
     a.A = A; a.B = B; a.sai = sai; a.sak = sak; a.sbk = sbk; a.sbj = sbj;
     a.M = Mr; a.N = Nc; a.K = K; a.C = C; a.ldc = ldc; a.Cb = Cb; a.bias = bias; a.scale = scale; a.shift = shift;
     a.epi = epi;
-    const dim3 grid((unsigned) ((Nc + M::TN - 1) / M::TN), (unsigned) ((Mr + M::TM - 1) / M::TM));
-    hipLaunchKernelGGL(M::k_gemm, grid, dim3(256), 0, m->ctx->stream, a);
+    if (!vectorizable(A, sai, sak) || !vectorizable(B, sbj, sbk)) {  // e.g. the 50-wide input layer
+        const dim3 grid((unsigned) ((Nc + M::TN - 1) / M::TN), (unsigned) ((Mr + M::TM - 1) / M::TM));
+        hipLaunchKernelGGL(M::k_gemm, grid, dim3(256), 0, m->ctx->stream, a);
+        return;
+    }
+    const int tiles = ((Nc + M::G2_BN - 1) / M::G2_BN) * ((Mr + M::G2_BM - 1) / M::G2_BM);
+    // split K when the output has too few tiles to fill the device and K is long (the output
+    // layer's dA = dY W': 8 tiles, K = 52,360): up to 256 workgroups, 16 k-steps each at least
+    int split = 1;
+    if (epi == M::EPI_STORE && tiles < 128)
+        split = std::max(1, std::min(256 / tiles, K / (16 * M::G2_BK)));
+    float *part = nullptr;
+    if (split > 1) {
+        const size_t need = (size_t) split * Mr * Nc * sizeof(float);
+        if (m->splitWs.bytes < need) m->splitWs.alloc(need);
+        part = m->splitWs.as<float>();
+    }
+    const dim3 grid((unsigned) ((Nc + M::G2_BN - 1) / M::G2_BN), (unsigned) ((Mr + M::G2_BM - 1) / M::G2_BM),
+                    (unsigned) split);
+    const bool ak = sak == 1, bk = sbk == 1;
+    if (ak && bk) hipLaunchKernelGGL((M::k_gemm2<true, true>), grid, dim3(256), 0, m->ctx->stream, a, part);
+    else if (ak) hipLaunchKernelGGL((M::k_gemm2<true, false>), grid, dim3(256), 0, m->ctx->stream, a, part);
+    else if (bk) hipLaunchKernelGGL((M::k_gemm2<false, true>), grid, dim3(256), 0, m->ctx->stream, a, part);
+    else hipLaunchKernelGGL((M::k_gemm2<false, false>), grid, dim3(256), 0, m->ctx->stream, a, part);
+    if (split > 1) {
+        const long n = (long) Mr * Nc;
+        hipLaunchKernelGGL(M::k_splitk_reduce, dim3((unsigned) std::min<long>(2048, (n + 255) / 256)), dim3(256), 0,
+                           m->ctx->stream, part, split, Mr, Nc, C, ldc);
+    }
 }
 
 void ensureCap(spk_mlp *m, int rows) {

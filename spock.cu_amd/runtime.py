@@ -32,6 +32,7 @@ EXPORTS = [
     "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
     "spk_batch_refill", "spk_batch_refill_from_batch", "spk_batch_refill_rows", "spk_batch_get_lanes_device",
     "spk_batch_pool_open", "spk_batch_set_chunk", "spk_batch_set_debug", "spk_batch_debug_records",
+    "spk_batch_autotune_uneven", "spk_batch_set_uneven_schedule",
     "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
     "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
     "spk_mlp_loss_backward", "spk_mlp_adam", "spk_mlp_dataset", "spk_mlp_train", "spk_mlp_predict",
@@ -101,6 +102,8 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_set_schedule.argtypes = [vp, C.c_int, C.c_int]
     lib.spk_batch_solve_graph.argtypes = [vp, vp, C.c_int, C.c_int]
     lib.spk_batch_autotune_workgroups.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), _dp]
+    lib.spk_batch_autotune_uneven.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
+    lib.spk_batch_set_uneven_schedule.argtypes = [vp, C.c_int, C.c_int]
     lib.spk_batch_resume_graph.argtypes = [vp, C.c_int]
     lib.spk_batch_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
     lib.spk_batch_reset.argtypes = [vp]
@@ -296,6 +299,22 @@ class Batch:
         self.ctx.check(self.ctx.lib.spk_batch_autotune_workgroups(self.h, C.byref(o), int(iters), C.byref(w), _d(sec)),
                        "autotune_workgroups")
         return w.value, dict(zip([1, 8, 32, 64, 128], sec.tolist()))
+
+    UNEVEN_CANDIDATES = [(128, 0), (256, 0), (512, 0), (128, 1), (256, 1), (512, 1)]
+
+    def autotune_uneven(self, opts=None, iters=25):
+        """Per-instance batch: time workgroups of 128/256/512 threads, instances in batch order or
+        largest first, and keep the fastest; returns ((block, largest_first), {candidate: seconds})."""
+        blk, lf = C.c_int(), C.c_int()
+        sec = np.zeros(len(self.UNEVEN_CANDIDATES))
+        o = opts if opts is not None else options()
+        self.ctx.check(self.ctx.lib.spk_batch_autotune_uneven(self.h, C.byref(o), int(iters), C.byref(blk),
+                                                              C.byref(lf), _d(sec)), "autotune_uneven")
+        return (blk.value, lf.value), dict(zip(self.UNEVEN_CANDIDATES, sec.tolist()))
+
+    def set_uneven_schedule(self, block=0, largest_first=False):
+        self.ctx.check(self.ctx.lib.spk_batch_set_uneven_schedule(self.h, int(block), 1 if largest_first else 0),
+                       "set_uneven_schedule")
 
     def set_workgroups(self, n):
         """Workgroups that solve the single instance together (0 or 1: one workgroup,

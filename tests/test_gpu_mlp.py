@@ -76,7 +76,10 @@ def relnorm(a, b):
     return float(np.linalg.norm(a - b) / max(1e-30, np.linalg.norm(b)))
 
 
-@pytest.mark.parametrize("dims,rows", [((5, 24, 24, 37), 19), ((50, 64, 64, 64, 300), 70)])
+# the last case is the warm start's own network (50 -> 3 x 512 -> 52,360, a minibatch of 256): the
+# output layer's GEMMs run on the 128 x 128 CDNA4 kernel, its input gradient with split K
+@pytest.mark.parametrize("dims,rows", [((5, 24, 24, 37), 19), ((50, 64, 64, 64, 300), 70),
+                                       ((50, 200, 200, 200, 1000), 130), ((50, 512, 512, 512, 52360), 256)])
 def test_mlp_forward_and_gradients_match_cpu(ctx, dims, rows):
     m = rt.WarmStartMLP(ctx, dims[0], dims[-1], hidden=dims[1], layers=len(dims) - 2, seed=5)
     rng = np.random.default_rng(1)

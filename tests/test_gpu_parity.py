@@ -266,6 +266,35 @@ def test_uneven_family_fp32_bulk_and_fp64_polish(ctx, uneven_family):
         assert abs(s_64 - s_p) <= 2e-2 * max(1.0, abs(s_p))
 
 
+def test_uneven_autotune_and_largest_first_order(ctx, uneven_family):
+    """The uneven-batch autotuner keeps the fastest of its six schedules, and the chosen schedule
+    still matches the oracle.  Largest-first dispatch only reorders the workgroups: every
+    instance's result is bitwise the one of batch order at the same workgroup size."""
+    oracles = [Oracle(d) for d in uneven_family]
+    probs = [rt.Problem(ctx, d) for d in uneven_family]
+    x0s = [configs.initial_states(4000 + k, o.nx, 1)[0] for k, o in enumerate(oracles)]
+    b = rt.Batch(ctx, probs, rt.F64)
+    b.set_x0(x0s)
+    (blk, lf), secs = b.autotune_uneven(iters=25)
+    print(f"uneven schedule {blk}, largest first {lf}: {secs}")
+    assert (blk, lf) in secs and secs[(blk, lf)] == min(secs.values())
+    b.solve(rt.options(tol_abs=1e-30, max_iters=50))
+    res, _ = b.results()
+    for k, o in enumerate(oracles):
+        ref = o.solve(x0s[k], tol_abs=1e-30, max_iters=50)
+        assert (res[k]["iters"], res[k]["k0"], res[k]["k2"]) == (ref["iters"], ref["k0"], ref["k2"])
+        assert rel_err(b.iterate(k), ref["z"]) < 1e-9
+    runs = []
+    for order in (False, True):
+        c = rt.Batch(ctx, probs, rt.F64)
+        c.set_uneven_schedule(256, order)
+        c.set_x0(x0s)
+        c.solve(rt.options(tol_abs=1e-30, max_iters=50))
+        r, _ = c.results()
+        runs.append((c.iterates(), [(v["iters"], v["k0"], v["k1"], v["k2"], v["k3"]) for v in r]))
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+
+
 def test_workgroup_autotune_single_instance(ctx, problem_dirs):
     """The workgroup autotuner of one-instance batches keeps the fastest count, leaves no state
     behind, and the tuned solve still matches the oracle over fixed iterations."""
