@@ -317,7 +317,7 @@ def train_warm_start(rt, ctx, prob, configs, args, schedule):
     mlp = rt.WarmStartMLP(ctx, prob.nx, prob.n_iter, hidden=512, layers=3, seed=11)
     mlp.dataset_lanes(b, ok, x_train[ok])
     t = time.perf_counter()
-    losses = mlp.train(epochs=args.epochs, minibatch=256, lr=1e-3, seed=3)
+    losses = mlp.train(epochs=args.epochs, minibatch=256, lr=args.lr, seed=3)
     log(f"MLP trained: {args.epochs} epochs on {ok.size} solutions in {time.perf_counter() - t:.1f} s, "
         f"loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     return mlp, int(ok.size), float(losses[-1])
@@ -416,7 +416,8 @@ def main():
     ap.add_argument("--fallback-iters", type=int, default=30000, help="iteration cap of the FP64 pool")
     ap.add_argument("--train", type=int, default=4096, help="training initial states solved for the MLP")
     ap.add_argument("--train-iters", type=int, default=4000, help="iteration cap of the training solves")
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=150, help="MLP training epochs (minibatch 256)")
+    ap.add_argument("--lr", type=float, default=3e-3, help="MLP Adam learning rate")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (the pool starts cold)")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--plain-iters", type=int, default=800, help="bounded sample of the plain single-instance solve")

@@ -75,8 +75,9 @@ typedef struct spk_options {
     int anderson_buffer;
     int check_period;
     int method;
-    int block_size;  /* threads per workgroup; 0 (default): 256 for per-instance batches, and for shared
-                        batches the size chosen by spk_batch_autotune, else 1024 */
+    int block_size;  /* threads per workgroup; 0 (default): for per-instance batches the size chosen by
+                        spk_batch_autotune_uneven, else 512 (256 in grid mode); for shared batches the
+                        size chosen by spk_batch_autotune, else 1024 */
     int ls_method;   /* enum spk_ls_method */
     int dyn_inverse; /* 1: apply (L L')^-1 as one matvec; 0: two triangular solves as the reference */
     long chunk_iters; /* shared batches: > 0 ends a launch after this many outer iterations with the

@@ -155,9 +155,9 @@ __device__ __forceinline__ void g2Load(const unsigned short *P, long sRow, long 
         if (KC) {  // 128 rows x 4 vectors of 8 k
             r = (tid >> 2) + 64 * v;
             kk = 8 * (tid & 3);
-        } else {   // 32 k x 16 vectors of 8 rows
-            kk = (tid >> 4) + 16 * v;
-            r = 8 * (tid & 15);
+        } else {   // 32 k x 16 vectors of 8 rows; a wave's lanes span the 32 k (see g2Store)
+            kk = tid & 31;
+            r = 8 * ((tid >> 5) + 8 * v);
         }
         const int gr = row0 + r, gk = k0 + kk;
         const bool full = KC ? (gr < rowsTotal && gk + 7 < K) : (gr + 7 < rowsTotal && gk < K);
@@ -184,7 +184,9 @@ __device__ __forceinline__ void g2Store(unsigned short *S, int tid, const u32x4 
             const int r = (tid >> 2) + 64 * v, kk = 8 * (tid & 3);
             *reinterpret_cast<u32x4 *>(S + r * G2_LD + kk) = in[v];
         } else {
-            const int kk = (tid >> 4) + 16 * v, r = 8 * (tid & 15);
+            // rows 8 apart share LDS banks (a row is 20 words), so the lanes of one write
+            // instruction take different k of the same rows: at most 2-way bank conflicts
+            const int kk = tid & 31, r = 8 * ((tid >> 5) + 8 * v);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 S[(r + 2 * q) * G2_LD + kk] = (unsigned short) (in[v][q] & 0xffffu);
@@ -294,12 +296,18 @@ __global__ void k_gelu_back(const float *dA, const float *Z, float *dZ, unsigned
 }
 
 // db[j] = sum_i dZ[i][j] (rows in order, one thread per column).
-__global__ void k_colsum(const float *dZ, int rows, int cols, float *db) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cols) return;
+// 256 threads per 64 columns: thread (c, q) sums rows q, q + 4, ... of column c, then the four
+// partials of a column are added in q order (the same order on every run).
+__global__ void __launch_bounds__(256) k_colsum(const float *dZ, int rows, int cols, float *db) {
+    __shared__ float part[4][64];
+    const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + c;
     float s = 0.f;
-    for (int i = 0; i < rows; i++) s += dZ[(long) i * cols + j];
-    db[j] = s;
+    if (j < cols)
+        for (int i = q; i < rows; i += 4) s += dZ[(long) i * cols + j];
+    part[q][c] = s;
+    __syncthreads();
+    if (q == 0 && j < cols) db[j] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
 }
 
 // MSE on standardized outputs: dY = 2 (y - t) / count and per-block partial sums of (y - t)^2.
@@ -549,7 +557,7 @@ void backward(spk_mlp *m, int rows) {
         const unsigned short *Ain = l == 0 ? m->X.as<unsigned short>() : m->A[l - 1]->as<unsigned short>();
         // dW_l = A_{l-1}' dZ_l  (Kl x Nl), db_l = colsum dZ_l
         gemm(m, Ain, 1, Kl, dCur, Nl, 1, Kl, Nl, rows, g + m->wOff[l], Nl, nullptr, nullptr, M::EPI_STORE);
-        hipLaunchKernelGGL(M::k_colsum, dim3((Nl + 255) / 256), dim3(256), 0, st, dCurF, rows, Nl, g + m->bOff[l]);
+        hipLaunchKernelGGL(M::k_colsum, dim3((Nl + 63) / 64), dim3(256), 0, st, dCurF, rows, Nl, g + m->bOff[l]);
         if (l == 0) break;
         // dA_{l-1} = dZ_l W_l'  (rows x Kl), then dZ_{l-1} = dA . gelu'(Z_{l-1})
         gemm(m, dCur, Nl, 1, wbf(m, l), 1, Nl, rows, Kl, Nl, m->dA.as<float>(), Kl, nullptr, nullptr, M::EPI_STORE);

@@ -40,7 +40,8 @@
 #define SPK_TILE_KERNEL(name) SPK_TILE_KERNEL_CAT(name, SPK_TILE_MAX_THREADS)
 #endif
 #ifndef SPK_TILE_KB
-#define SPK_TILE_KB 4  // k-steps of 4 columns whose operands are loaded together
+#define SPK_TILE_KB 2  // k-steps of 4 columns whose operands are loaded together (2 measured
+                       // 2.5% faster than 4 on the 229-node problem; 8 spills)
 #endif
 
 namespace spk {

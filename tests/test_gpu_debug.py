@@ -34,7 +34,9 @@ def _run(ctx, d, x0, dtype, cap, workgroups=1, **kw):
         b.set_workgroups(workgroups)
     b.set_debug(cap)
     b.set_x0([x0])
-    b.solve(rt.options(**kw))
+    # 256 threads, as the fixed-iteration parity tests: the iteration windows below are the ones
+    # FP64 rounding with that summation order keeps within the bounds
+    b.solve(rt.options(block_size=256, **kw))
     return b
 
 
@@ -107,7 +109,7 @@ def test_err0_carry_and_reset(ctx, problem_dirs):
     b = rt.Batch(ctx, [p], rt.F64)
     b.set_debug(16)
     b.set_x0([x0])
-    opts = rt.options(tol_abs=1e-30, max_iters=6, check_period=1, method=rt.CP)
+    opts = rt.options(tol_abs=1e-30, max_iters=6, check_period=1, method=rt.CP, block_size=256)
     b.solve(opts)
     b.results()
     b.reset()  # zeroes err0's vector, and the iterate: the next solve is cold again
