@@ -124,7 +124,8 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 
 // ---------------------------------------------------------------- CDNA4 GEMM
 // C (M x N) = A (M x K) . B (K x N) on v_mfma_f32_16x16x32_bf16 (gfx950): 256 threads, a 128 x 128
-// tile per workgroup, each wave a 64 x 64 block of 4 x 4 MFMA tiles (64 accumulator registers).
+// tile per workgroup in k-steps of 64, each wave a 64 x 64 block of 4 x 4 MFMA tiles (64
+// accumulator registers).
 // Lane layout of the instruction: A[i][k] from lane i + 16 (k / 8) item k % 8, B[k][j] from lane
 // j + 16 (k / 8) item k % 8, D[i][j] in lane j + 16 (i / 4) item i % 4.
 // Both operands sit in LDS k-contiguous ([row][k] and [col][k], rows padded by 8 elements), so a
@@ -134,31 +135,40 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 // is scattered into 8 LDS rows).  LDS is double buffered: the next k-step's vectors are loaded
 // into registers while the matrix cores work on the current one.  With split-K (gridDim.z > 1)
 // each z-slice writes its partial product to a workspace, summed by k_splitk_reduce.
-constexpr int G2_BM = 128, G2_BN = 128, G2_BK = 32, G2_PAD = 8, G2_LD = G2_BK + G2_PAD;
+constexpr int G2_BM = 128, G2_BN = 128, G2_BK = 64, G2_PAD = 8, G2_LD = G2_BK + G2_PAD;
+constexpr int G2_VEC = G2_BM * G2_BK / 8 / 256;  // 8-element vectors per thread and operand
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 struct G2Stage {
-    u32x4 a[2], b[2];
+    u32x4 a[G2_VEC], b[G2_VEC];
 };
 
-// One operand's 128 x 32 k-step into registers: two 8-element vectors per thread.  kc:
-// contiguous along k; ld: the stride of the other dimension; rowsTotal / K: bounds.
+// Thread tid's v-th vector of a 128 x 64 k-step: KC (contiguous along k): row r, k kk..kk+7;
+// otherwise rows r..r+7 at k kk, a wave's 64 lanes taking the 64 k of the same rows, so that
+// their scattered LDS writes (one row at a time) fall in 32 different banks.
+template <bool KC>
+__device__ __forceinline__ void g2Slot(int tid, int v, int &r, int &kk) {
+    if (KC) {
+        r = (tid >> 3) + 32 * v;
+        kk = 8 * (tid & 7);
+    } else {
+        kk = tid & 63;
+        r = 8 * ((tid >> 6) + 4 * v);
+    }
+}
+
+// One operand's 128 x 64 k-step into registers: G2_VEC 8-element vectors per thread.  sRow /
+// sK: strides of the row and k dimensions; rowsTotal / K: bounds.
 template <bool KC>
 __device__ __forceinline__ void g2Load(const unsigned short *P, long sRow, long sK, int row0, int rowsTotal, int k0,
-                                       int K, int tid, u32x4 (&out)[2]) {
+                                       int K, int tid, u32x4 (&out)[G2_VEC]) {
 #pragma unroll
-    for (int v = 0; v < 2; v++) {
+    for (int v = 0; v < G2_VEC; v++) {
         int r, kk;
-        if (KC) {  // 128 rows x 4 vectors of 8 k
-            r = (tid >> 2) + 64 * v;
-            kk = 8 * (tid & 3);
-        } else {   // 32 k x 16 vectors of 8 rows; a wave's lanes span the 32 k (see g2Store)
-            kk = tid & 31;
-            r = 8 * ((tid >> 5) + 8 * v);
-        }
+        g2Slot<KC>(tid, v, r, kk);
         const int gr = row0 + r, gk = k0 + kk;
         const bool full = KC ? (gr < rowsTotal && gk + 7 < K) : (gr + 7 < rowsTotal && gk < K);
         if (full) {
@@ -177,16 +187,14 @@ __device__ __forceinline__ void g2Load(const unsigned short *P, long sRow, long 
 }
 
 template <bool KC>
-__device__ __forceinline__ void g2Store(unsigned short *S, int tid, const u32x4 (&in)[2]) {
+__device__ __forceinline__ void g2Store(unsigned short *S, int tid, const u32x4 (&in)[G2_VEC]) {
 #pragma unroll
-    for (int v = 0; v < 2; v++) {
+    for (int v = 0; v < G2_VEC; v++) {
+        int r, kk;
+        g2Slot<KC>(tid, v, r, kk);
         if (KC) {
-            const int r = (tid >> 2) + 64 * v, kk = 8 * (tid & 3);
             *reinterpret_cast<u32x4 *>(S + r * G2_LD + kk) = in[v];
         } else {
-            // rows 8 apart share LDS banks (a row is 20 words), so the lanes of one write
-            // instruction take different k of the same rows: at most 2-way bank conflicts
-            const int kk = tid & 31, r = 8 * ((tid >> 5) + 8 * v);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 S[(r + 2 * q) * G2_LD + kk] = (unsigned short) (in[v][q] & 0xffffu);
@@ -228,18 +236,23 @@ __global__ void __launch_bounds__(256) k_gemm2(GemmArgs g, float *partial) {
             g2Load<AK>(g.A, g.sai, g.sak, i0, g.M, kn, kEnd, tid, st.a);
             g2Load<BK>(g.B, g.sbj, g.sbk, j0, g.N, kn, kEnd, tid, st.b);
         }
-        bf16x8 af[4], bfr[4];
 #pragma unroll
-        for (int a = 0; a < 4; a++)
-            af[a] = *reinterpret_cast<const bf16x8 *>(&As[cur][(wi + 16 * a + (lane & 15)) * G2_LD + 8 * (lane >> 4)]);
+        for (int ks = 0; ks < G2_BK; ks += 32) {
+            bf16x8 af[4], bfr[4];
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            bfr[b] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][(wj + 16 * b + (lane & 15)) * G2_LD + 8 * (lane >> 4)]);
-#pragma unroll
-        for (int a = 0; a < 4; a++)
+            for (int a = 0; a < 4; a++)
+                af[a] = *reinterpret_cast<const bf16x8 *>(
+                    &As[cur][(wi + 16 * a + (lane & 15)) * G2_LD + ks + 8 * (lane >> 4)]);
 #pragma unroll
             for (int b = 0; b < 4; b++)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+                bfr[b] = *reinterpret_cast<const bf16x8 *>(
+                    &Bs[cur][(wj + 16 * b + (lane & 15)) * G2_LD + ks + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
         if (kt + 1 < nk) {
             g2Store<AK>(As[cur ^ 1], tid, st.a);
             g2Store<BK>(Bs[cur ^ 1], tid, st.b);
