@@ -29,7 +29,7 @@ iteration: FP64 SPOCK on one instance per launch (grid mode), cold.  After it: t
 sampled accepted solutions against plain FP64 SPOCK (cold, same thresholds), whose iteration counts
 scale the plain and CPU per-iteration times to solves/s.
 
-Prints one JSON line; ``--gpus`` above 1 is refused: this build runs on one GPU.
+Prints one JSON line; ``--gpus`` above 1 is refused: the program runs on one GPU.
 """
 import argparse
 import json
@@ -411,13 +411,15 @@ def main():
     ap.add_argument("--polish-slots", type=int, default=1024, help="FP64 pool slots (polish and fallback)")
     ap.add_argument("--step-iters", type=int, default=200, help="FP32 pool iterations per timed step")
     ap.add_argument("--warmup-iters", type=int, default=500, help="FP32 pool iterations per warmup step")
-    ap.add_argument("--polish-iters", type=int, default=25, help="FP64 pool iterations per step")
+    ap.add_argument("--polish-iters", type=int, default=5,
+                    help="FP64 pool iterations per step (polish solves mostly pass at their first check; cold "
+                         "fallbacks advance this many iterations per step)")
     ap.add_argument("--max-iters", type=int, default=6000, help="FP32 cap of a warm solve before the fallback")
     ap.add_argument("--fallback-iters", type=int, default=30000, help="iteration cap of the FP64 pool")
     ap.add_argument("--train", type=int, default=4096, help="training initial states solved for the MLP")
     ap.add_argument("--train-iters", type=int, default=4000, help="iteration cap of the training solves")
     ap.add_argument("--epochs", type=int, default=150, help="MLP training epochs (minibatch 256)")
-    ap.add_argument("--lr", type=float, default=3e-3, help="MLP Adam learning rate")
+    ap.add_argument("--lr", type=float, default=5e-3, help="MLP Adam learning rate")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (the pool starts cold)")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--plain-iters", type=int, default=800, help="bounded sample of the plain single-instance solve")
@@ -446,7 +448,7 @@ def main():
                          "prints one JSON line with the time per minibatch step and its GEMM flops, and exits")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
     rt, configs = _pkg()
     ctx = rt.Context(0)

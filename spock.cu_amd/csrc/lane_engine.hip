@@ -24,7 +24,7 @@
 #define SPK_LANE_MAX_THREADS 1024
 #endif
 
-// Matrix-vector schedule of this build: 0 reads the factor entries with scalar loads; 4, 8 or 16
+// Matrix-vector schedule of this compilation: 0 reads the factor entries with scalar loads; 4, 8 or 16
 // stage them through LDS in chunks of that many columns.  The library is built once per schedule;
 // the kernels of a build carry its suffix (k_lane_solve, k_lane_solve_x8, ...), and the autotuner
 // of the host side picks one per batch.

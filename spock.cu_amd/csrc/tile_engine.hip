@@ -26,7 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
-// Launch bound of this build: 512 threads (two waves per SIMD, up to 256 registers each) or 1024
+// Launch bound of this compilation: 512 threads (two waves per SIMD, up to 256 registers each) or 1024
 // (four waves per SIMD, 128 registers each: more waves to hide memory latency, more spills).  The
 // library is built once per bound; the 1024 build's kernels carry the suffix _x1024.
 #ifndef SPK_TILE_MAX_THREADS

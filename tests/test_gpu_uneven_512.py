@@ -10,7 +10,7 @@ differently shaped tree, solved in one persistent launch with one workgroup per 
   itself: every instance that plain FP64 SPOCK solves within its cap is accepted, by the polish or
   (at most 1% of them) by the cold FP64 fallback.
 
-The problem folders and the oracle runs are spread over host processes (the GPU box's CPU share)."""
+The problem folders and the oracle runs are spread over up to 16 host processes."""
 import multiprocessing as mp
 import os
 import sys
@@ -44,7 +44,7 @@ def _make(args):
 
 @pytest.fixture(scope="module")
 def family(tmp_path_factory):
-    base = tmp_path_factory.mktemp("cfg4")
+    base = tmp_path_factory.mktemp("uneven512")
     jobs = [(str(base / f"i{k}"), k) for k in range(COUNT)]
     with mp.get_context("spawn").Pool(min(16, os.cpu_count() or 1)) as pool:
         out = pool.map(_make, jobs, chunksize=8)
