@@ -284,7 +284,7 @@ def plain_single(rt, ctx, prob, x0, iters, workgroups):
     b.set_workgroups(workgroups)
     b.set_x0([x0])
     t = time.perf_counter()
-    b.solve(rt.options(tol_abs=1e-30, tol_rel=0.0, max_iters=iters, block_size=512))
+    b.solve(rt.options(tol_abs=1e-30, tol_rel=0.0, max_iters=iters))  # default threads: 512, 256 in grid mode
     res, sec = b.results()
     dt = time.perf_counter() - t
     return dt / max(1, res[0]["iters"]), res[0]["iters"]
@@ -423,7 +423,8 @@ def main():
     ap.add_argument("--cold", action="store_true", help="no learned warm start (the pool starts cold)")
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--plain-iters", type=int, default=800, help="bounded sample of the plain single-instance solve")
-    ap.add_argument("--plain-workgroups", type=int, default=64)
+    ap.add_argument("--plain-workgroups", type=int, default=0,
+                    help="cooperating workgroups of the plain single-instance baseline (0: autotuned)")
     ap.add_argument("--cpu-seconds", type=float, default=5.0)
     ap.add_argument("--cpu-workers", type=int, default=0, help="host processes of the CPU baseline (0: min(16, cpus))")
     ap.add_argument("--tune-iters", type=int, default=25)
@@ -502,7 +503,14 @@ def main():
 
         # the plain method: FP64, one instance per launch, cold (bounded sample of its iterations)
         plain = {}
-        for wgs in (1, args.plain_workgroups):
+        grid_wgs = args.plain_workgroups
+        if grid_wgs == 0:  # the workgroup autotuner's choice for this instance (1, 8, 32, 64 or 128)
+            tb = rt.Batch(ctx, [prob], rt.F64)
+            tb.set_x0([stream[0]])
+            grid_wgs, wsec = tb.autotune_workgroups(iters=25)
+            log(f"plain FP64 single instance: autotuned {grid_wgs} workgroups; {wsec}")
+            del tb
+        for wgs in sorted({1, grid_wgs}):
             spi, its = plain_single(rt, ctx, prob, stream[0], args.plain_iters, wgs)
             plain[wgs] = {"seconds_per_iteration": spi, "iterations_timed": its, "workgroups": wgs}
             log(f"plain FP64 single instance, {wgs} workgroup(s): {1e3 * spi:.3f} ms/iteration")
