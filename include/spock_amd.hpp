@@ -198,11 +198,13 @@ private:
 template <typename T>
 class Cache {
 public:
-    // Iterate sizes for which the automatic choice spreads one instance over 64 cooperating
-    // workgroups (grid mode): the range of the 229-node, 52,360-element problem this has been run
-    // and measured on (several times faster than one workgroup there).  Elsewhere one workgroup
-    // solves the instance unless workgroups(n) asks otherwise.
+    // Iterate sizes for which the automatic choice spreads one instance over kAutoGridWorkgroups
+    // cooperating workgroups (grid mode): the range of the 229-node, 52,360-element problem this
+    // has been run and measured on (2.0 ms per outer iteration with 32 workgroups, 2.4 with 64,
+    // 8.9 with one).  Elsewhere one workgroup solves the instance unless workgroups(n) asks
+    // otherwise.
     static constexpr size_t kAutoGridMin = 50000, kAutoGridMax = 100000;
+    static constexpr int kAutoGridWorkgroups = 32;
 
     // workgroups: how many workgroups solve the instance together (0: automatic, see above)
     // debug: CacheBuilder::enableDebug (src/cache.cuh:1204-1209, 312-318)
@@ -229,7 +231,7 @@ public:
         }
         if (workgroups == 0) {
             const size_t n = data.sizePrimal() + data.sizeDual();
-            workgroups = (n >= kAutoGridMin && n <= kAutoGridMax) ? 64 : 1;
+            workgroups = (n >= kAutoGridMin && n <= kAutoGridMax) ? kAutoGridWorkgroups : 1;
             m_autoGrid = workgroups > 1;
         }
         m_workgroups = workgroups;

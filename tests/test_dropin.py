@@ -191,7 +191,7 @@ def test_drop_in_accessors(tmp_path):
 @pytest.mark.gpu
 def test_reference_driver_on_the_229_node_problem(tmp_path):
     """The timing driver on the 229-node problem (52,360-element iterate): the drop-in's Cache
-    spreads the single instance over 64 cooperating workgroups (grid mode) and converges."""
+    spreads the single instance over 32 cooperating workgroups (grid mode) and converges."""
     folder = str(tmp_path / "data")
     prob = configs.generate("mid", folder)
     x0 = configs.initial_states(2, 50, 1)[0]
@@ -202,13 +202,13 @@ def test_reference_driver_on_the_229_node_problem(tmp_path):
     run = subprocess.run([str(exe), folder + "/"], cwd=str(tmp_path), capture_output=True, text=True, timeout=170)
     print(run.stdout)
     assert run.returncode == 0 and "Saved" in run.stdout
-    # which launch shape the Cache used: the automatic 64-workgroup grid mode, with no fallback
+    # which launch shape the Cache used: the automatic 32-workgroup grid mode, with no fallback
     exe2 = tmp_path / "gridmode"
     _compile(GRIDMODE, exe2, tmp_path)
     out = subprocess.run([str(exe2), folder + "/"], capture_output=True, text=True, timeout=120)
     print(out.stdout)
     assert out.returncode == 0
-    assert out.stdout.split() == ["0", "64", "0"]
+    assert out.stdout.split() == ["0", "32", "0"]
 
 
 GRIDMODE = r'''

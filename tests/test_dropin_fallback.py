@@ -121,9 +121,9 @@ def run(driver, prim, wgs, other_error=0):
 
 
 def test_size_based_grid_falls_back_to_one_workgroup(driver):
-    # an iterate of 60,000 elements: the Cache picks 64 workgroups, the launch is refused, the
+    # an iterate of 60,000 elements: the Cache picks 32 workgroups, the launch is refused, the
     # solve is repeated with one workgroup, and later solves use one workgroup directly
-    assert run(driver, 30000, 0) == ["workgroups 64", "solve 64", "workgroups 1", "solve 1",
+    assert run(driver, 30000, 0) == ["workgroups 32", "solve 32", "workgroups 1", "solve 1",
                                      "status 0 iters 7 workgroups 1 fallback 1",
                                      "solve 1", "status 0 iters 7 workgroups 1 fallback 1"]
 
@@ -141,7 +141,7 @@ def test_large_iterates_stay_on_one_workgroup(driver):
 
 def test_other_errors_are_not_retried(driver):
     lines = run(driver, 30000, 0, other_error=1)
-    assert lines == ["workgroups 64", "solve 64", "error [runSpock] hipMalloc: out of memory"]
+    assert lines == ["workgroups 32", "solve 32", "error [runSpock] hipMalloc: out of memory"]
 
 
 def test_requested_workgroups_report_the_refusal(driver):
