@@ -323,6 +323,24 @@ def train_warm_start(rt, ctx, prob, configs, args, schedule):
     return mlp, int(ok.size), float(losses[-1])
 
 
+def plain_baseline(rt, ctx, prob, x0, args):
+    """The plain FP64 single-instance solve timed per iteration with one workgroup and in grid mode
+    with --plain-workgroups cooperating workgroups (0: the workgroup autotuner's choice)."""
+    plain = {}
+    grid_wgs = args.plain_workgroups
+    if grid_wgs == 0:  # the workgroup autotuner's choice for this instance (1, 8, 32, 64 or 128)
+        tb = rt.Batch(ctx, [prob], rt.F64)
+        tb.set_x0([x0])
+        grid_wgs, wsec = tb.autotune_workgroups(iters=25)
+        log(f"plain FP64 single instance: autotuned {grid_wgs} workgroups; {wsec}")
+        del tb
+    for wgs in sorted({1, grid_wgs}):
+        spi, its = plain_single(rt, ctx, prob, x0, args.plain_iters, wgs)
+        plain[wgs] = {"seconds_per_iteration": spi, "iterations_timed": its, "workgroups": wgs}
+        log(f"plain FP64 single instance, {wgs} workgroup(s): {1e3 * spi:.3f} ms/iteration")
+    return plain
+
+
 def mlp_only(rt, ctx, prob, x0s, epochs):
     """The MLP's training kernels at the bench's size: GEMM flops per minibatch step of 256 rows
     (forward, weight gradients, and the input gradients of every layer but the first) and the
@@ -443,6 +461,9 @@ def main():
                     help="only the dominant kernel: one launch of 50 cold iterations of --slots FP32 instances "
                          "(schedule 17, 1024 threads), then a 1 GiB device copy whose counters calibrate "
                          "FETCH_SIZE / WRITE_SIZE; prints one JSON line (for rocprofv3) and exits")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only the plain FP64 single-instance baseline: the workgroup autotuner, then --plain-iters "
+                         "cold iterations with one workgroup and with the chosen count; prints one JSON line and exits")
     ap.add_argument("--mlp-only", type=int, default=0, metavar="EPOCHS",
                     help="only the MLP's training kernels: the 50 -> 3 x 512 -> 52,360 network trained for EPOCHS "
                          "epochs (minibatch 256) on the iterates of 4,096 held-out x0 after 25 FP32 iterations; "
@@ -471,6 +492,10 @@ def main():
             return
         if args.mlp_only > 0:
             print(json.dumps({"mlp_training": mlp_only(rt, ctx, prob, stream[:4096], args.mlp_only)}), flush=True)
+            return
+        if args.plain_only:
+            print(json.dumps({"plain_fp64_single": list(plain_baseline(rt, ctx, prob, stream[0], args).values())}),
+                  flush=True)
             return
 
         cpu = None
@@ -502,18 +527,7 @@ def main():
             f"{roof['achieved']:.0f} GB/s algorithmic")
 
         # the plain method: FP64, one instance per launch, cold (bounded sample of its iterations)
-        plain = {}
-        grid_wgs = args.plain_workgroups
-        if grid_wgs == 0:  # the workgroup autotuner's choice for this instance (1, 8, 32, 64 or 128)
-            tb = rt.Batch(ctx, [prob], rt.F64)
-            tb.set_x0([stream[0]])
-            grid_wgs, wsec = tb.autotune_workgroups(iters=25)
-            log(f"plain FP64 single instance: autotuned {grid_wgs} workgroups; {wsec}")
-            del tb
-        for wgs in sorted({1, grid_wgs}):
-            spi, its = plain_single(rt, ctx, prob, stream[0], args.plain_iters, wgs)
-            plain[wgs] = {"seconds_per_iteration": spi, "iterations_timed": its, "workgroups": wgs}
-            log(f"plain FP64 single instance, {wgs} workgroup(s): {1e3 * spi:.3f} ms/iteration")
+        plain = plain_baseline(rt, ctx, prob, stream[0], args)
         best_plain = min(plain.values(), key=lambda v: v["seconds_per_iteration"])
 
         mlp, n_train, train_loss = None, 0, None
