@@ -1110,6 +1110,7 @@ int spk_batch_debug_records(spk_batch *b, int inst, long cap, double *records, l
         if (b->lanes || b->dbgCap == 0)
             throw std::invalid_argument("spk_batch_debug_records: debug traces are off (spk_batch_set_debug)");
         if (inst < 0 || inst >= b->count) throw std::invalid_argument("spk_batch_debug_records: no such instance");
+        if (!b->pending) throw std::invalid_argument("spk_batch_debug_records: no solve has been run on this batch");
         check(hipStreamSynchronize(b->ctx->stream), "sync");
         spk::Out o;
         check(hipMemcpy(&o, static_cast<spk::Out *>(b->dOut.p) + inst, sizeof(o), hipMemcpyDeviceToHost),

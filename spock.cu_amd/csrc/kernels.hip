@@ -1626,8 +1626,8 @@ __global__ void __launch_bounds__(SPK_MAX_THREADS) k_debug(const Inst<T> *__rest
         T dot, mx;
         reduceEll<T, R, V>(P, c, z, cand, lds, dot, mx);
         if (c.tid == 0) {
-            outs[kInst].w = (double) dot;
-            outs[kInst].err = (double) mx;
+            outs[P.slot].w = (double) dot;
+            outs[P.slot].err = (double) mx;
         }
     }
 }
