@@ -248,9 +248,20 @@ int spk_batch_autotune(spk_batch *b, const spk_options *opts, int iters, int *mv
  * kernel launches (the first and launches - 1 resumes) and replayed with no host round trip between
  * chunks; workgroups whose instances have all finished return at once from the remaining launches.
  * The iterates are those of one uninterrupted solve.  If launches x chunk_iters is too few, results
- * report SPK_NOT_RUN for unfinished instances and spk_batch_resume continues.  No counterpart in the
- * reference. */
+ * report SPK_NOT_RUN for unfinished instances and spk_batch_resume continues.
+ * Per-instance (uneven) batches: the solve is one persistent launch; launches must be 1 and
+ * chunk_iters 0, and that launch is captured as a graph with one kernel node and launched (not
+ * in grid mode: a cooperative launch is not captured).  spk_batch_replay_graph launches it again.
+ * No counterpart in the reference. */
 int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int launches);
+/* per-instance batches: launch the solve captured by spk_batch_solve_graph again, with its options
+ * and warm flag, on whatever the batch's buffers now hold: initial states written by
+ * spk_batch_set_x0, iterates by spk_batch_set_iterate(s) or spk_batch_reset, tolerances by
+ * spk_batch_set_tolerances (the descriptors are rewritten in place first).  Repeated solves of the
+ * same problems (a receding-horizon loop) then cost one graph launch.  A later spk_batch_solve,
+ * debug operator, spk_batch_set_debug or larger Anderson memory reallocates buffers the graph points
+ * to and drops it; replaying then fails with a message.  No counterpart in the reference. */
+int spk_batch_replay_graph(spk_batch *b);
 /* continue a chunked solve of a shared batch with `launches` resume launches captured as one graph
  * (as spk_batch_resume, without host round trips between the launches). */
 int spk_batch_resume_graph(spk_batch *b, int launches);

@@ -148,6 +148,9 @@ struct spk_batch {
     // (0 / false: 256 threads, instances in batch order)
     int fusedBlock = 0;
     bool largestFirst = false;
+    // per-instance batches: graph/graphExec hold the captured persistent solve launch
+    // (spk_batch_solve_graph); any reallocation of the buffers it points to drops it
+    bool instGraph = false;
     size_t elSize() const { return dtype == SPK_F64 ? 8 : 4; }
 };
 
@@ -322,8 +325,18 @@ void orderLargestFirst(spk_batch *b) {
     std::copy(tmp.begin(), tmp.end(), arr);
 }
 
+void dropInstGraph(spk_batch *b) {
+    if (!b->instGraph) return;
+    if (b->graphExec) (void) hipGraphExecDestroy(b->graphExec);
+    if (b->graph) (void) hipGraphDestroy(b->graph);
+    b->graphExec = nullptr;
+    b->graph = nullptr;
+    b->instGraph = false;
+}
+
 template <typename T>
 void buildInsts(spk_batch *b, const spk_options &o, int warm) {
+    dropInstGraph(b);  // the descriptor buffer is reallocated below
     if (b->count == 1 && !b->gpart.p) b->gpart.alloc((size_t) 2 * spk::GRID_MAXK * spk::GRID_MAXWG * sizeof(T));
     fillInsts<T>(b, o, warm, static_cast<T *>(b->anders.p), b->andOff, static_cast<T *>(b->gpart.p));
     if (b->largestFirst && b->count > 1) orderLargestFirst<T>(b);
@@ -334,6 +347,7 @@ void buildInsts(spk_batch *b, const spk_options &o, int warm) {
 
 void ensureAnderson(spk_batch *b, int m) {
     if (m <= b->mAlloc) return;
+    dropInstGraph(b);
     b->andOff.resize(b->count);
     size_t off = 0;
     for (int k = 0; k < b->count; k++) {
@@ -474,6 +488,47 @@ void launchSolve(spk_batch *b, const spk_options &o, int warm) {
     launchKernel<T>(b, bs, false, 0);
     check(hipGetLastError(), "launch k_solve");
     check(hipEventRecord(b->ev1, b->ctx->stream), "event");
+    b->pending = true;
+}
+
+// Per-instance batches: the persistent solve launch captured as a graph with one kernel node and
+// launched; spk_batch_replay_graph launches it again after new initial states, iterates or
+// tolerances were written into the batch's buffers (no rebuild of the launch, no reallocation).
+template <typename T>
+void launchInstGraph(spk_batch *b, const spk_options &o, int warm) {
+    if (b->count == 1 && b->wgs > 1)
+        throw std::runtime_error("grid mode (a cooperative launch) is not captured in a graph; use spk_batch_solve");
+    hipStream_t st = b->ctx->stream;
+    buildInsts<T>(b, o, warm);  // drops an earlier captured solve
+    if (b->dOut.bytes != sizeof(spk::Out) * b->count) b->dOut.alloc(sizeof(spk::Out) * b->count);
+    check(hipStreamSynchronize(st), "sync before capture");
+    if (b->graphExec) check(hipGraphExecDestroy(b->graphExec), "graph exec destroy");
+    if (b->graph) check(hipGraphDestroy(b->graph), "graph destroy");
+    b->graphExec = nullptr;
+    b->graph = nullptr;
+    check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+    launchKernel<T>(b, solveBlock(o, b), false, 0);
+    check(hipStreamEndCapture(st, &b->graph), "end capture");
+    check(hipGraphInstantiate(&b->graphExec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+    b->instGraph = true;
+    b->lastOpts = o;
+    b->lastWarm = warm;
+    check(hipEventRecord(b->ev0, st), "event");
+    check(hipGraphLaunch(b->graphExec, st), "graph launch");
+    check(hipEventRecord(b->ev1, st), "event");
+    b->pending = true;
+}
+
+template <typename T>
+void replayInstGraph(spk_batch *b) {
+    hipStream_t st = b->ctx->stream;
+    // the descriptors again, into the same buffer: given tolerances may have changed
+    fillInsts<T>(b, b->lastOpts, b->lastWarm, static_cast<T *>(b->anders.p), b->andOff, static_cast<T *>(b->gpart.p));
+    if (b->largestFirst && b->count > 1) orderLargestFirst<T>(b);
+    check(hipMemcpyAsync(b->dInst.p, b->hInst.data(), b->hInst.size(), hipMemcpyHostToDevice, st), "instances");
+    check(hipEventRecord(b->ev0, st), "event");
+    check(hipGraphLaunch(b->graphExec, st), "graph launch");
+    check(hipEventRecord(b->ev1, st), "event");
     b->pending = true;
 }
 
@@ -1082,6 +1137,7 @@ int spk_batch_set_debug(spk_batch *b, long max_records) {
             throw std::invalid_argument("spk_batch_set_debug: debug traces are recorded by per-instance batches "
                                         "(spk_batch_create), not shared ones");
         check(hipStreamSynchronize(b->ctx->stream), "sync");
+        dropInstGraph(b);  // the descriptors point into the debug buffers
         b->dbgCap = max_records;
         if (max_records == 0) {
             b->dbgRec.release();
@@ -1648,7 +1704,22 @@ int spk_batch_resume(spk_batch *b) {
 
 int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int launches) {
     try {
-        if (!b->lanes) throw std::runtime_error("graph-captured chunked solves need a shared batch");
+        if (!b->lanes) {  // the persistent solve launch of a per-instance batch, captured and launched
+            spk_options o;
+            if (opts) o = *opts;
+            else spk_default_options(&o);
+            if (launches != 1) throw std::runtime_error("a per-instance batch's solve is one launch: launches must be 1");
+            if (o.chunk_iters > 0) throw std::runtime_error("chunked solves need a shared batch");
+            if (o.anderson_buffer < 1 || o.anderson_buffer > 8) throw std::runtime_error("anderson_buffer must be 1..8");
+            if (o.max_inner < 0) throw std::runtime_error("max_inner must be >= 0");
+            if (o.ls_method != SPK_LS_HOUSEHOLDER && o.ls_method != SPK_LS_GRAM)
+                throw std::runtime_error("ls_method must be SPK_LS_HOUSEHOLDER or SPK_LS_GRAM");
+            check(hipSetDevice(b->ctx->device), "hipSetDevice");
+            ensureAnderson(b, o.anderson_buffer);
+            if (b->dtype == SPK_F64) launchInstGraph<double>(b, o, warm);
+            else launchInstGraph<float>(b, o, warm);
+            return 0;
+        }
         if (!opts || opts->chunk_iters <= 0) throw std::runtime_error("a graph-captured solve needs chunk_iters > 0");
         if (launches < 1) throw std::runtime_error("launches must be >= 1");
         if (opts->anderson_buffer < 1 || opts->anderson_buffer > spk_batch::LANE_MAXM)
@@ -1656,6 +1727,20 @@ int spk_batch_solve_graph(spk_batch *b, const spk_options *opts, int warm, int l
         check(hipSetDevice(b->ctx->device), "hipSetDevice");
         if (b->dtype == SPK_F64) launchLaneGraph<double>(b, *opts, warm, launches);
         else launchLaneGraph<float>(b, *opts, warm, launches);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_replay_graph(spk_batch *b) {
+    try {
+        if (b->lanes || !b->instGraph || !b->graphExec)
+            throw std::runtime_error("no captured solve of a per-instance batch to replay (spk_batch_solve_graph)");
+        check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->ctx->stream), "sync");
+        if (b->dtype == SPK_F64) replayInstGraph<double>(b);
+        else replayInstGraph<float>(b);
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);

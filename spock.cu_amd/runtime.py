@@ -32,7 +32,7 @@ EXPORTS = [
     "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
     "spk_batch_refill", "spk_batch_refill_from_batch", "spk_batch_refill_rows", "spk_batch_get_lanes_device",
     "spk_batch_pool_open", "spk_batch_set_chunk", "spk_batch_set_debug", "spk_batch_debug_records",
-    "spk_batch_autotune_uneven", "spk_batch_set_uneven_schedule",
+    "spk_batch_autotune_uneven", "spk_batch_set_uneven_schedule", "spk_batch_replay_graph",
     "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
     "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
     "spk_mlp_loss_backward", "spk_mlp_adam", "spk_mlp_dataset", "spk_mlp_train", "spk_mlp_predict",
@@ -104,6 +104,7 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_autotune_workgroups.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), _dp]
     lib.spk_batch_autotune_uneven.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
     lib.spk_batch_set_uneven_schedule.argtypes = [vp, C.c_int, C.c_int]
+    lib.spk_batch_replay_graph.argtypes = [vp]
     lib.spk_batch_resume_graph.argtypes = [vp, C.c_int]
     lib.spk_batch_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
     lib.spk_batch_reset.argtypes = [vp]
@@ -378,9 +379,15 @@ class Batch:
         """A chunked solve (opts.chunk_iters > 0) captured as one HIP graph of `launches` launches
         (default: enough for opts.max_iters).  Enqueues only; results() waits."""
         if launches is None:
-            launches = max(1, -(-max(1, opts.max_iters) // max(1, opts.chunk_iters)) + 1)
+            launches = (max(1, -(-max(1, opts.max_iters) // max(1, opts.chunk_iters)) + 1)
+                        if self.shared else 1)
         self.ctx.check(self.ctx.lib.spk_batch_solve_graph(self.h, C.byref(opts), 1 if warm else 0, int(launches)),
                        "solve_graph")
+
+    def replay_graph(self):
+        """Per-instance batch: launch the solve captured by solve_graph again on the batch's current
+        initial states, iterates and tolerances.  Enqueues only; results() waits."""
+        self.ctx.check(self.ctx.lib.spk_batch_replay_graph(self.h), "replay_graph")
 
     def resume_graph(self, launches):
         """Continue a chunked solve with `launches` resume launches captured as one HIP graph."""

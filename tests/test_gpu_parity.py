@@ -295,6 +295,44 @@ def test_uneven_autotune_and_largest_first_order(ctx, uneven_family):
     assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
 
 
+def test_uneven_batch_graph_capture_and_replay(ctx, uneven_family):
+    """The persistent solve launch of an uneven batch captured as a graph (spk_batch_solve_graph) and
+    replayed (spk_batch_replay_graph) after a reset and new initial states: every replay equals a
+    plain solve of the same initial states bit for bit, with eager work between the replays; a
+    later plain solve drops the captured graph and a replay then fails with a message."""
+    probs = [rt.Problem(ctx, d) for d in uneven_family]
+    x0_sets = [[configs.initial_states(4100 + 17 * r + k, p.nx, 1)[0] for k, p in enumerate(probs)]
+               for r in range(3)]
+    opts = rt.options(tol_abs=1e-3, tol_rel=1e-3, max_iters=300)
+
+    def plain(x0s):
+        b = rt.Batch(ctx, probs, rt.F64)
+        b.set_x0(x0s)
+        b.solve(opts)
+        res, _ = b.results()
+        return b.iterates(), [(r["status"], r["iters"], r["k0"], r["k1"], r["k2"], r["k3"]) for r in res]
+
+    g = rt.Batch(ctx, probs, rt.F64)
+    g.set_x0(x0_sets[0])
+    g.solve_graph(opts)
+    res, _ = g.results()
+    z, counts = plain(x0_sets[0])
+    assert np.array_equal(g.iterates(), z)
+    assert [(r["status"], r["iters"], r["k0"], r["k1"], r["k2"], r["k3"]) for r in res] == counts
+    for x0s in (x0_sets[1], x0_sets[2], x0_sets[1]):
+        z, counts = plain(x0s)  # eager launches between the replays
+        g.reset()
+        g.set_x0(x0s)
+        g.replay_graph()
+        res, _ = g.results()
+        assert np.array_equal(g.iterates(), z)
+        assert [(r["status"], r["iters"], r["k0"], r["k1"], r["k2"], r["k3"]) for r in res] == counts
+    g.solve(opts)
+    g.results()
+    with pytest.raises(RuntimeError, match="no captured solve"):
+        g.replay_graph()
+
+
 def test_workgroup_autotune_single_instance(ctx, problem_dirs):
     """The workgroup autotuner of one-instance batches keeps the fastest count, leaves no state
     behind, and the tuned solve still matches the oracle over fixed iterations."""
