@@ -18,9 +18,11 @@ from the learned warm start (the MLP's prediction for its x0) and ends when the 
 passes; the slot is then refilled with the next held-out x0 while the other slots run on.  Every
 FP32 solution is polished in FP64 in a second pool (warm-started from it; already accepted when its
 FP64 residual passes at its first check).  A warm solve that reaches its FP32 iteration cap falls
-back to a cold FP64 solve in the same FP64 pool.  A step is one chunk of --step-iters lockstep
-iterations of the FP32 pool plus one chunk of the FP64 pool, with the refills between them; warmup
-steps (longer) fill the pipeline first.  value = held-out solutions accepted in FP64 during the
+back to a cold FP64 solve in the same FP64 pool.  A step is one launch of the FP32 pool, a
+wall-clock slice of --step-seconds in which every workgroup runs as many lockstep iterations as
+fit (so fast workgroups never wait for slow ones; --step-seconds 0: chunks of --step-iters
+iterations), plus one chunk of the FP64 pool, with the refills between them; warmup steps (chunks
+of --warmup-iters iterations) fill the pipeline first.  value = held-out solutions accepted in FP64 during the
 timed steps / timed wall time.
 
 Before the timed part: the thresholds of the stream, the MLP trained on cold FP32 solves of
@@ -148,6 +150,7 @@ class Pool:
         self.b.set_tolerances(self.tols)
         self.b.pool_open(rt.options(tol_abs=0.0, tol_rel=0.0, max_iters=max_iters, chunk_iters=25))
         self.slots = slots
+        self.max_iters = max_iters
         self.job = np.full(slots, -1, dtype=np.int64)  # job id, -1 idle, -2 ended and waiting to be freed
         self.kernel_s = 0.0
         self.launches = 0
@@ -164,9 +167,12 @@ class Pool:
             self.b.refill(slots, x0s)
         self.job[slots] = jobs
 
-    def step(self, iters):
-        """One chunk of `iters` lockstep iterations; (slot, job, result) of every solve that ended."""
-        self.b.set_chunk(iters)
+    def step(self, iters, seconds=0.0):
+        """One launch of `iters` lockstep iterations, or (seconds > 0) a wall-clock slice of that
+        many seconds in which each workgroup runs as many iterations as fit; (slot, job, result) of
+        every solve that ended."""
+        self.b.set_chunk(iters if seconds <= 0 else self.max_iters)
+        self.b.set_chunk_time(max(0.0, seconds))
         self.b.resume()
         res, sec = self.b.results()
         self.kernel_s += sec
@@ -427,7 +433,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--slots", type=int, default=4096, help="FP32 pool slots (the held-out batch)")
     ap.add_argument("--polish-slots", type=int, default=1024, help="FP64 pool slots (polish and fallback)")
-    ap.add_argument("--step-iters", type=int, default=200, help="FP32 pool iterations per timed step")
+    ap.add_argument("--step-seconds", type=float, default=4.0,
+                    help="wall-clock slice of the FP32 pool's launch in a timed step (0: --step-iters iterations)")
+    ap.add_argument("--step-iters", type=int, default=200,
+                    help="FP32 pool iterations per timed step when --step-seconds is 0 (and, x4, per training chunk)")
     ap.add_argument("--warmup-iters", type=int, default=500, help="FP32 pool iterations per warmup step")
     ap.add_argument("--polish-iters", type=int, default=5,
                     help="FP64 pool iterations per step (polish solves mostly pass at their first check; cold "
@@ -555,9 +564,9 @@ def main():
         p64_queue = []  # (job, FP32 slot): FP32 solutions waiting for an FP64 slot (their FP32 slot held)
         fb_queue = []   # jobs whose warm FP32 solve failed: cold FP64 solves
 
-        def run_step(iters32, iters64, timed):
+        def run_step(iters32, iters64, timed, seconds32=0.0):
             accepted = 0
-            for k, j, r in p32.step(iters32):
+            for k, j, r in p32.step(iters32, seconds32):
                 jobs[j]["fp32_iters"] = r["iters"]
                 if r["status"] == 0:
                     p64_queue.append((j, k))
@@ -607,7 +616,7 @@ def main():
         accepted = 0
         steps_done = 0
         for s in range(args.steps):
-            accepted += run_step(args.step_iters, args.polish_iters, True)
+            accepted += run_step(args.step_iters, args.polish_iters, True, args.step_seconds)
             steps_done += 1
             log(f"step {s}: {accepted} accepted in the timed steps ({time.perf_counter() - t0:.1f} s)")
             if time.perf_counter() - T_START > args.budget:
@@ -658,7 +667,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "RAOCP Markov tree 229 nodes (3 events, horizon 10, branching stops at stage 3), "
                                "nx=50, nu=20, box, AV@R 0.9; held-out x0~N(0,0.5^2 I)",
-                   "slots": args.slots, "polish_slots": args.polish_slots, "step_iters": args.step_iters,
+                   "slots": args.slots, "polish_slots": args.polish_slots, "step_iters": args.step_iters, "step_seconds": args.step_seconds,
                    "acceptance": "per x0, the threshold plain FP64 SPOCK sets itself under the reference's "
                                  "builder tol(1e-3) (absolute and relative 1e-3, as its timed drivers, "
                                  "examples/random/main.cu:41-45): max(1e-3, 1e-3 e0) x min(scaling), e0 the "

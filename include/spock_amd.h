@@ -220,6 +220,14 @@ int spk_batch_refill(spk_batch *b, int count, const int *lanes, const double *x0
 int spk_batch_pool_open(spk_batch *b, const spk_options *opts);
 /* outer iterations per launch of the following resumes of a chunked solve */
 int spk_batch_set_chunk(spk_batch *b, long chunk_iters);
+/* A time slice for the launches of the chunked solves of a shared batch (no counterpart in the
+ * reference; for continuous batching): each workgroup also leaves a launch, its state saved as at
+ * the end of a chunk, once it has run `seconds` of wall-clock time and at least one outer iteration
+ * in it, so fast workgroups keep iterating while slow ones finish instead of idling until the
+ * slowest has done chunk_iters.  Where a launch ends never changes the iterates, branch decisions
+ * or counters (those of one uninterrupted solve); only the iterations done per launch vary.
+ * 0 (the default) turns it off.  Applies from the next launch. */
+int spk_batch_set_chunk_time(spk_batch *b, double seconds);
 int spk_batch_refill_from_batch(spk_batch *b, int count, const int *lanes, const double *x0, spk_batch *src,
                                 const int *src_lanes);
 int spk_batch_refill_rows(spk_batch *b, int count, const int *lanes, const double *x0, const float *z, long ld);

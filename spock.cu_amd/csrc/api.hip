@@ -129,6 +129,7 @@ struct spk_batch {
     spk_options lastOpts{};
     int lastWarm = 0;
     bool resumable = false;
+    double chunkSeconds = 0.;  // wall-clock slice of each launch of a chunked solve (0: none)
     size_t lRegion[16] = {};  // element offsets: work[0..8], zio, rio, hS, hY, qd, dd, x0
     static constexpr int LANE_MAXM = 4;
     int laneMv = 17;     // schedule: 16, 17 the tile engine (MFMA; 512- or 1024-thread build); 0, 8 the lane engine
@@ -661,6 +662,7 @@ void buildLaneInst(spk_batch *b, const spk_options &o, int warm, int resume = 0)
     L.warm = warm;
     L.maxTicks = o.max_time_secs > 0 ? o.max_time_secs * b->ctx->tickRateHz : 0.;
     L.chunkIters = o.chunk_iters > 0 ? o.chunk_iters : 0;
+    L.chunkTicks = (L.chunkIters > 0 && b->chunkSeconds > 0.) ? b->chunkSeconds * b->ctx->tickRateHz : 0.;
     L.resume = resume;
     L.save = static_cast<spk::LaneSave *>(b->lsave.p);
     L.gsave = static_cast<spk::LaneGroupSave *>(b->lgsave.p);
@@ -1683,6 +1685,17 @@ int spk_batch_set_chunk(spk_batch *b, long chunk_iters) {
         if (!b->lanes || !b->resumable) throw std::runtime_error("no chunked solve of a shared batch");
         if (chunk_iters <= 0) throw std::runtime_error("chunk_iters must be positive");
         b->lastOpts.chunk_iters = chunk_iters;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int spk_batch_set_chunk_time(spk_batch *b, double seconds) {
+    try {
+        if (!b->lanes) throw std::runtime_error("a time slice applies to the chunked solves of a shared batch");
+        if (!(seconds >= 0.) || !std::isfinite(seconds)) throw std::runtime_error("the time slice must be finite and >= 0");
+        b->chunkSeconds = seconds;
         return 0;
     } catch (const std::exception &e) {
         return fail(b->ctx, e);

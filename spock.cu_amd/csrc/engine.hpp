@@ -105,6 +105,8 @@ struct LaneInst {
     int maxInner, andBuff, checkPeriod, method, warm;
     double maxTicks;
     long chunkIters;          // > 0: stop after this many outer iterations, state saved for a resume
+    double chunkTicks;        // > 0 (chunked solves): also stop, after at least one outer iteration, once
+                              // the launch has run this many wall-clock ticks
     int resume;               // 1: continue the saved solve
     LaneSave *save;           // B entries
     LaneGroupSave *gsave;     // B / 64 entries

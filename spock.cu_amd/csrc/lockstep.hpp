@@ -271,7 +271,12 @@ __device__ void lockstepSolve(const LaneInst<T> *Pp, Out *outs, double *lds, lon
             active = false;
         }
         if (!anyLane(active)) break;
-        if (P.chunkIters > 0 && it >= itStart + P.chunkIters) {  // uniform: save and leave
+        // uniform: save and leave.  The time slice is thread 0's clock, made uniform by the
+        // workgroup-wide vote, so every wave leaves at the same iteration.
+        if (P.chunkIters > 0 &&
+            (it >= itStart + P.chunkIters ||
+             (P.chunkTicks > 0. && it > itStart &&
+              anyLane(threadIdx.x == 0 && (double) (wall_clock64() - t0) >= P.chunkTicks)))) {
             chunkExit = true;
             break;
         }

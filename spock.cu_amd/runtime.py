@@ -31,7 +31,7 @@ EXPORTS = [
     "spk_batch_solve", "spk_batch_solve_noalloc", "spk_batch_resume", "spk_batch_results", "spk_batch_debug_op",
     "spk_batch_set_iterates_device", "spk_batch_get_iterates_device",
     "spk_batch_refill", "spk_batch_refill_from_batch", "spk_batch_refill_rows", "spk_batch_get_lanes_device",
-    "spk_batch_pool_open", "spk_batch_set_chunk", "spk_batch_set_debug", "spk_batch_debug_records",
+    "spk_batch_pool_open", "spk_batch_set_chunk", "spk_batch_set_chunk_time", "spk_batch_set_debug", "spk_batch_debug_records",
     "spk_batch_autotune_uneven", "spk_batch_set_uneven_schedule", "spk_batch_replay_graph",
     "spk_mlp_create", "spk_mlp_destroy", "spk_mlp_num_params", "spk_mlp_get_params", "spk_mlp_set_params",
     "spk_mlp_get_grads", "spk_mlp_get_normalization", "spk_mlp_set_normalization", "spk_mlp_forward",
@@ -133,6 +133,7 @@ def load_library(path=LIB_PATH):
     lib.spk_batch_get_lanes_device.argtypes = [vp, C.c_int, _ip, vp, C.c_long]
     lib.spk_batch_pool_open.argtypes = [vp, C.POINTER(Options)]
     lib.spk_batch_set_chunk.argtypes = [vp, C.c_long]
+    lib.spk_batch_set_chunk_time.argtypes = [vp, C.c_double]
     _fp = C.POINTER(C.c_float)
     lib.spk_mlp_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)]
     lib.spk_mlp_destroy.argtypes = [vp]
@@ -421,6 +422,10 @@ class Batch:
     def set_chunk(self, chunk_iters):
         """Outer iterations per launch of the following resumes (``spk_batch_set_chunk``)."""
         self.ctx.check(self.ctx.lib.spk_batch_set_chunk(self.h, int(chunk_iters)), "set_chunk")
+
+    def set_chunk_time(self, seconds):
+        """Wall-clock slice of each launch of a chunked solve, 0 for none (``spk_batch_set_chunk_time``)."""
+        self.ctx.check(self.ctx.lib.spk_batch_set_chunk_time(self.h, float(seconds)), "set_chunk_time")
 
     def solve_chunked(self, opts, warm=False, on_chunk=None):
         """Run a chunked solve to its end; on_chunk(results, kernel_seconds) after every chunk.

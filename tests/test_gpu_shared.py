@@ -251,6 +251,37 @@ def test_chunked_solve_equals_uninterrupted(ctx, problem_dirs, dtype):
     assert np.array_equal(a.iterates(), b.iterates())
 
 
+@pytest.mark.parametrize("engine,dtype", [(17, rt.F32), (17, rt.F64), (16, rt.F32), (0, rt.F64)])
+def test_time_sliced_chunks_equal_uninterrupted(ctx, problem_dirs, engine, dtype):
+    """Launches cut by a wall-clock slice (spk_batch_set_chunk_time) end at whatever iteration each
+    workgroup has reached; the solve still equals an uninterrupted one bit for bit."""
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    count = 70
+    x0s = x0s_for("small", o.nx, count)
+    p, a = shared(ctx, d, count, dtype, engine)
+    a.set_x0(x0s)
+    opts = dict(tol_abs=1e-3, tol_rel=1e-3, max_iters=500)
+    a.solve(rt.options(**opts))
+    ra, _ = a.results()
+    b = rt.Batch(ctx, [p] * count, dtype, shared=True, engine=engine)
+    b.set_chunk_time(2e-4)
+    b.set_x0(x0s)
+    chunks = []
+    rb, _ = b.solve_chunked(rt.options(chunk_iters=400, **opts), on_chunk=lambda r, s: chunks.append(r))
+    print(f"engine {engine}: {len(chunks)} launches of a 200 us slice")
+    assert len(chunks) > 3  # 400-iteration chunks alone would end this solve in at most 2 launches
+    for k in range(count):
+        for f in ("status", "iters", "k0", "k1", "k2", "k2bt", "k3", "num_T"):
+            assert ra[k][f] == rb[k][f], (k, f)
+    assert np.array_equal(a.iterates(), b.iterates())
+    with pytest.raises(rt.SpockError, match="finite"):
+        b.set_chunk_time(-1.0)
+    q = rt.Batch(ctx, [p], dtype)
+    with pytest.raises(rt.SpockError, match="shared batch"):
+        q.set_chunk_time(1.0)
+
+
 @pytest.mark.parametrize("name,dtype", [("small", rt.F64), ("mid", rt.F64), ("mid", rt.F32), ("poly", rt.F64)])
 def test_lds_schedule_is_bitwise_identical(ctx, problem_dirs, name, dtype):
     """The LDS-staged matrix-vector schedule accumulates in the same order as the scalar one."""
