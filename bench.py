@@ -568,6 +568,8 @@ def main():
             accepted = 0
             for k, j, r in p32.step(iters32, seconds32):
                 jobs[j]["fp32_iters"] = r["iters"]
+                jobs[j]["fp32_T"] = r["num_T"]
+                jobs[j]["fp32_k0"] = r["k0"]
                 if r["status"] == 0:
                     p64_queue.append((j, k))
                 else:
@@ -629,6 +631,10 @@ def main():
         fp32_it = np.array([v["fp32_iters"] for v in acc if v["fp32_iters"] is not None] or [0])
         fp64_it = np.array([v["fp64_iters"] for v in acc if v["fp64_iters"] is not None] or [0])
         mean_iters = float(np.mean([(v["fp32_iters"] or 0) + (v["fp64_iters"] or 0) for v in acc])) if acc else None
+        # applications of T per FP32 iteration of a slot's own solve (1 + its line-search steps) and
+        # the share of its iterations that took the blind K0 step (no line search)
+        br = [(v["fp32_T"] / v["fp32_iters"], v["fp32_k0"] / v["fp32_iters"]) for v in acc
+              if v.get("fp32_iters")] or [(0.0, 0.0)]
         del p32, p64
         # cost check: sampled accepted solutions against plain FP64 SPOCK, cold, at its own threshold;
         # its iteration counts are the plain method's iterations per solve
@@ -683,7 +689,9 @@ def main():
         "jobs_fallback": sum(1 for v in jobs.values() if v["fallback"]),
         "iterations_per_accepted_solve": {"mean": mean_iters,
                                           "fp32_mean": float(fp32_it.mean()) if acc else None,
-                                          "fp64_mean": float(fp64_it.mean()) if acc else None},
+                                          "fp64_mean": float(fp64_it.mean()) if acc else None,
+                                          "fp32_T_per_iteration_mean": float(np.mean([a for a, _ in br])),
+                                          "fp32_k0_share_mean": float(np.mean([b for _, b in br]))},
         "plain_fp64_single": {"solves_per_s": plain_rate, "variants": list(plain.values()),
                               "iterations_per_solve": per_solve,
                               "basis": "measured seconds per iteration of the plain path (cold, bounded sample) "

@@ -117,7 +117,11 @@ def test_refill_rejects_bad_requests(ctx, problem_dirs):
 @pytest.mark.parametrize("engine", [17, 0])
 def test_empty_pool_runs_only_filled_slots(ctx, problem_dirs, engine):
     """spk_batch_pool_open: every slot idle (SPK_NOT_RUN) until a refill; 9 slots spread over the
-    workgroups get cold solves of 50 iterations and match fresh solves; the idle slots stay idle."""
+    workgroups get cold solves of 50 iterations and match the oracle's solves of the same initial
+    states (branch counts exactly, iterates to 1e-9, as the fixed-iteration parity tests); the idle
+    slots stay idle.  (A fresh GPU batch of the same states is no better a reference: its lanes
+    share a workgroup with different neighbours, and a SuperMann decision within rounding of its
+    threshold may go the other way there.)"""
     d = problem_dirs["small"]
     o = Oracle(d)
     p = rt.Problem(ctx, d)
@@ -129,11 +133,15 @@ def test_empty_pool_runs_only_filled_slots(ctx, problem_dirs, engine):
     b.refill(slots, x0)
     b.resume()
     res = run_out(b, slots)
-    ref, zr = fresh(ctx, p, x0, iters, engine)
+    ref = [o.solve(x, tol_abs=1e-30, max_iters=iters, ls="gram", adjoint_dots=True) for x in x0]
     z = b.iterates().reshape(count, -1)
+
+    def oracle_counts(r):
+        return (r["status"], r["iters"], r["k0"], r["k1"], r["k2"], r["k2bt"], r["k3"])
+
     for i, k in enumerate(slots):
-        assert counts(res[k]) == counts(ref[i]), k
-        assert rel_err(z[k], zr[i]) < 1e-10
+        assert counts(res[k]) == oracle_counts(ref[i]), k
+        assert rel_err(z[k], ref[i]["z"]) < 1e-9
     for k in set(range(count)) - set(slots):
         assert res[k]["status"] == -1 and res[k]["iters"] == 0
     # a second round in other slots, with a shorter chunk
@@ -143,5 +151,5 @@ def test_empty_pool_runs_only_filled_slots(ctx, problem_dirs, engine):
     res = run_out(b, [1, 2])
     z = b.iterates().reshape(count, -1)
     for i, k in enumerate([1, 2]):
-        assert counts(res[k]) == counts(ref[i])
-        assert rel_err(z[k], zr[i]) < 1e-10
+        assert counts(res[k]) == oracle_counts(ref[i])
+        assert rel_err(z[k], ref[i]["z"]) < 1e-9
