@@ -206,6 +206,30 @@ def test_shared_fp32_bulk_and_fp64_polish(ctx, problem_dirs):
         assert abs(s0 - s0_ref) <= 2e-2 * max(1.0, abs(s0_ref))
 
 
+@pytest.mark.parametrize("engine,dtype", [(17, rt.F64), (17, rt.F32), (0, rt.F64)])
+def test_single_instance_iterate_round_trips(ctx, problem_dirs, engine, dtype):
+    """spk_batch_get_iterate / spk_batch_set_iterate on a shared batch (one instance's column of
+    the interleaved layout, 16- and 64-instance blocks) against the whole-batch copies."""
+    d = problem_dirs["small"]
+    o = Oracle(d)
+    count, n = 67, o.n_iter
+    _, b = shared(ctx, d, count, dtype, engine)
+    rng = np.random.default_rng(4)
+    Z = rng.standard_normal(count * n)
+    if dtype == rt.F32:
+        Z = Z.astype(np.float32).astype(np.float64)  # exactly representable
+    b.set_iterates(Z)
+    for k in range(count):
+        assert np.array_equal(b.iterate(k), Z[k * n:(k + 1) * n]), k
+    lib = ctx.lib
+    for k in (0, 15, 16, 63, 64, 66):
+        v = np.ascontiguousarray(np.arange(n, dtype=np.float64) + 0.5 * k)
+        ctx.check(lib.spk_batch_set_iterate(b.h, k, v.ctypes.data_as(rt.C.POINTER(rt.C.c_double))), "set_iterate")
+        Z[k * n:(k + 1) * n] = v
+        assert np.array_equal(b.iterate(k), v)
+    assert np.array_equal(b.iterates(), Z)  # the other instances are untouched
+
+
 def test_shared_layout_round_trips(ctx, problem_dirs):
     """Host and device copies between the interleaved layout and per-instance batches."""
     d = problem_dirs["small"]
